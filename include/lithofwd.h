@@ -1,0 +1,128 @@
+/*
+ * lithofwd.h -- C ABI of the MI355X (gfx950) polyhedral gravity / magnetic forward model
+ * and its adjoint.
+ *
+ * The library replaces the per-station x per-facet loops of LithoInv3D's litho-forward
+ * module.  A context holds one lithology model resident in HBM:
+ *   - vertices  (shared by every unit; the reference keeps them identical across group
+ *     meshes, modules/litho-model/src/litho/lithology_model.cpp:230-247),
+ *   - triangles stored once each, oriented outward for their `front` unit, with the unit
+ *     on the other side in `back` (-1: outside the model).  A reference contact face that
+ *     two group meshes carry with opposite winding is one triangle here,
+ *   - per-unit density contrast (g/cc) and magnetization vector (reference units:
+ *     amplitude chi*F/100 times the unit direction, magnetic_forward.cpp:100-134),
+ *   - the stations (gravity and magnetic data share one station set per context).
+ *
+ * Conventions: z positive up; gz = downward attraction in mGal; TMI in nT; arrays are
+ * host pointers unless the entry point name ends in _device.  Every entry point returns
+ * 0 on success and a negative LF_E* code on failure; lf_last_error() then describes it.
+ * A context is not thread-safe (neither is the reference's ForwardModel) and all its
+ * work runs on one HIP stream.
+ */
+#ifndef LITHOFWD_H
+#define LITHOFWD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LITHOFWD_ABI_VERSION 1
+
+/* precision */
+#define LF_FP64 0  /* plain FP64 path, the reference's arithmetic */
+#define LF_FP32C 1 /* FP32 in station-centred face frames, compensated facet sums,
+                      FP64 for edges whose line passes next to the station */
+/* TMI edge coefficient */
+#define LF_TMI_REFERENCE 0 /* (M.t)(F.t), magnetic_forward.cpp:338-343 */
+#define LF_TMI_TEXTBOOK 1  /* (M.n)(F.t), the uniformly-magnetized polyhedron field */
+/* outputs wanted */
+#define LF_WANT_GZ 1
+#define LF_WANT_TMI 2
+
+/* errors */
+#define LF_OK 0
+#define LF_EINVAL -1
+#define LF_EHIP -2
+#define LF_ENOMEM -3
+#define LF_ESTATE -4
+
+typedef struct lf_ctx lf_ctx;
+
+typedef struct {
+    int64_t n_vertices;
+    int64_t n_faces;
+    int64_t n_stations;
+    int32_t n_units;
+    int32_t precision; /* LF_FP64 | LF_FP32C */
+    int32_t tmi_mode;  /* LF_TMI_REFERENCE | LF_TMI_TEXTBOOK */
+    int32_t want;      /* LF_WANT_GZ | LF_WANT_TMI */
+    int32_t device;    /* HIP device ordinal */
+    int32_t reserved;
+    const double* vertices;  /* n_vertices*3, xyz interleaved (metres) */
+    const uint32_t* faces;   /* n_faces*3 vertex ids, outward for the front unit */
+    const int32_t* front;    /* n_faces unit ids (>= 0) */
+    const int32_t* back;     /* n_faces unit ids, -1 = outside */
+    const double* stations;  /* n_stations*3, xyz interleaved (metres) */
+    const double* density;   /* n_units, g/cc */
+    const double* magnetization; /* n_units*3: amp * direction (amp = chi*F/100) */
+    double field_dir[3];     /* inducing-field unit vector (magnetic_forward.cpp:61-66) */
+} lf_desc;
+
+/* Creates a context and uploads the model.  Replaces the construction of GravityForward
+ * (modules/litho-forward/include/litho_invert/forward/gravity_forward.h:10-11) and
+ * MagneticForward (magnetic_forward.h:10-12): the forward keeps the model and stations. */
+int lf_create(lf_ctx** ctx, const lf_desc* desc);
+void lf_destroy(lf_ctx* ctx);
+/* Last error message of ctx (or of the last failed lf_create when ctx is NULL). */
+const char* lf_last_error(const lf_ctx* ctx);
+
+/* New vertex positions (n_vertices*3, host).  Replaces the vertex write-back of
+ * LithologyModel::applyParameterVector (lithology_model.cpp:210-248) that
+ * GravityForward::compute performs first (gravity_forward.cpp:106). */
+int lf_set_vertices(lf_ctx* ctx, const double* xyz);
+/* Same, from device memory already on this context's device. */
+int lf_set_vertices_device(lf_ctx* ctx, const double* d_xyz);
+/* New unit properties (either pointer may be NULL to keep the current values). */
+int lf_set_properties(lf_ctx* ctx, const double* density, const double* magnetization);
+
+/* gz (mGal) and TMI (nT) at every station; either output may be NULL.
+ * Replaces GravityForward::compute (gravity_forward.cpp:105-118) and
+ * MagneticForward::compute (magnetic_forward.cpp:81-94) after the parameter write. */
+int lf_forward(lf_ctx* ctx, double* gz, double* tmi);
+/* Same into device buffers of n_stations doubles; enqueued on the context stream, returns
+ * without waiting. */
+int lf_forward_device(lf_ctx* ctx, double* d_gz, double* d_tmi);
+
+/* Matrix-free adjoint: grad[v*3+axis] = sum_i (w_gz[i] dgz_i/dx_{v,axis}
+ *                                          + w_tmi[i] dTMI_i/dx_{v,axis})
+ * for every vertex coordinate.  With w = -W^2 r this is the data-misfit gradient that
+ * ObjectiveFunction::dataMisfitGradient forms densely as -J^T W^2 r
+ * (modules/litho-inversion/src/inversion/objective.cpp:42-55,
+ * joint_objective.cpp:47-58) from computeGravityAnalyticJacobian /
+ * computeMagneticAnalyticJacobian (gravity_forward.cpp:184-292,
+ * magnetic_forward.cpp:196-306), before the vertex -> DOF column mapping.
+ * Either weight vector may be NULL (treated as zero). */
+int lf_adjoint(lf_ctx* ctx, const double* w_gz, const double* w_tmi, double* grad);
+/* Device-pointer variant (n_stations weights each, 3*n_vertices gradient doubles). */
+int lf_adjoint_device(lf_ctx* ctx, const double* d_w_gz, const double* d_w_tmi,
+                      double* d_grad);
+
+/* Waits for the context stream. */
+int lf_sync(lf_ctx* ctx);
+
+/* Per-kernel HIP-event timing on the context stream.  When enabled, each launch is
+ * bracketed by events; lf_timing() returns, for kernel class k (0 = face setup,
+ * 1 = forward, 2 = forward reduce, 3 = adjoint pairs, 4 = adjoint finalize,
+ * 5 = vertex gather), the summed milliseconds and launch count since the last reset. */
+int lf_set_timing(lf_ctx* ctx, int enable);
+int lf_timing(lf_ctx* ctx, double* ms, int64_t* launches, int n_classes);
+
+/* Diagnostics of the last forward: edges whose station-edge geometry took the FP64
+ * branch in FP32C (0 in FP64 mode). */
+int lf_stats(lf_ctx* ctx, int64_t* fp64_edges);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

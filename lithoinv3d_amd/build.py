@@ -1,0 +1,49 @@
+"""Builds liblithofwd.so (gfx950) in-tree with hipcc.
+
+    python -m lithoinv3d_amd.build          # or build() from __graft_entry__
+
+All HIP code is compiled with packed-FP32 instruction selection disabled
+(-Xclang -target-feature -Xclang -packed-fp32-ops).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "liblithofwd.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+    "-Wno-unused-function",
+    "-Wno-unused-variable",
+    "-Wno-unused-value",
+    "-Wno-unused-result",
+]
+
+
+def sources():
+    return [os.path.join(CSRC, "lithofwd.hip")]
+
+
+def build(verbose: bool = False, extra=None, out: str = LIB) -> str:
+    cmd = [HIPCC] + HIP_FLAGS + list(extra or []) + sources() + ["-o", out + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd, cwd=CSRC)
+    os.replace(out + ".tmp", out)
+    return out
+
+
+if __name__ == "__main__":
+    build(verbose=True, extra=sys.argv[1:])
+    print("built", LIB)

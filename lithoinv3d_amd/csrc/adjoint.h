@@ -1,0 +1,496 @@
+// adjoint.h -- matrix-free adjoint of the polyhedral gz / TMI forward.
+//
+// Replaces the reference's dense per-face one-sided-FD Jacobian followed by -J^T W^2 r
+// (gravity_forward.cpp:184-292, magnetic_forward.cpp:196-306, objective.cpp:42-55) with
+// the exact derivative of the frame-form facet integral, in one pass over all
+// (station, facet) pairs:
+//
+//   grad_v = sum_i [ w_gz_i * d gz_i / d x_v  +  w_tmi_i * d TMI_i / d x_v ]
+//
+// Structure.  A facet's contribution at station o is F(p; theta): p = R (o - a) is the
+// station offset in the facet frame R = (u, v, n) and theta are the facet constants
+// (nine in-plane shape numbers, 2A, and six coefficients).  Per pair the kernel runs the
+// frame-form forward and its reverse, and accumulates over the stations of a chunk:
+//   Gp    = sum w dF/dp                                  (3)
+//   A_jk  = sum w (dF/dp_j p_k - dF/dp_k p_j), j<k     (3)  -- the frame's rotation
+//   tbar  = sum w dF/dtheta                              (16)
+// 22 numbers per facet, lane = facet, stations wave-uniform (scalar loads).  A finalize
+// kernel then pushes them through R(a,b,c) and theta(a,b,c) once per facet with
+// forward-mode duals (9 directions):
+//   dPhi/dx_k = sum_{j<k'} (dR_j/dx_k . R_k') A_jk' + tbar . dtheta/dx_k - [x_k in a] (R^T Gp)_k
+// and a gather kernel sums facet corners into vertices through a vertex -> corner CSR
+// list built once from the topology (deterministic, no atomics).
+//
+// The edge log uses the cancellation-free three-case form of the forward; the FP32
+// adjoint has no FP64 edge branch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "pair_math.h"
+
+namespace lfk {
+
+constexpr int kSeeds = 22;
+enum SeedIdx {
+    S_GPX = 0, S_GPY, S_GPZ, S_AXY, S_AXZ, S_AYZ,
+    S_L0, S_CU, S_CV, S_E1U, S_E1V, S_L1, S_E2U, S_E2V, S_L2, S_TWOA,
+    S_K1, S_K2, S_W0, S_W1, S_W2, S_W3
+};
+
+struct StationW {
+    double x, y, z, wg, wt, pad0, pad1, pad2;
+};
+
+struct AdjointWork {
+    int64_t n_v = 0, n_f = 0, n_s = 0;
+    int precision = 0;
+    int n_chunks = 1;
+    int64_t st_per_chunk = 1;
+    void* d_seeds = nullptr;
+    StationW* d_st = nullptr;
+    double* d_face_grad = nullptr;
+    int64_t* d_csr_off = nullptr;
+    int32_t* d_csr_ent = nullptr;
+    double* d_wg = nullptr;
+    double* d_wt = nullptr;
+    double* d_grad = nullptr;
+    double* d_zero = nullptr;  // n_s zeros for a missing weight vector
+};
+
+struct AdjointLaunch {
+    hipStream_t stream;
+    int64_t n_v, n_f, n_s;
+    int precision;
+    const FaceFrame<float>* f32;
+    const FaceFrame<double>* f64;
+    const double *sx, *sy, *sz;
+    const double *wg, *wt;
+    double* grad;
+};
+
+// d log2(Q)/d(s1, s2, d1, d2, rho^2) for the three cases of edge_log2.
+template <class T>
+__device__ __forceinline__ T edge_log2_grad(T s1, T s2, T d1, T d2, T rho2, T& gs1, T& gs2,
+                                            T& gd1, T& gd2, T& grho)
+{
+    const T X1 = fabs(s1) + d1;
+    const T X2 = fabs(s2) + d2;
+    const bool pos1 = s1 >= T(0);
+    const bool neg2 = s2 < T(0);
+    const bool mid = !(pos1 || neg2);
+    T num = neg2 ? X1 : X2;
+    T den = pos1 ? X1 : (neg2 ? X2 : rho2);
+    num = mid ? num * X1 : num;
+    const T k = T(1.4426950408889634);  // 1/ln 2
+    const T iX1 = k * fast_rcp(X1);
+    const T iX2 = k * fast_rcp(X2);
+    gs1 = -iX1;
+    gs2 = iX2;
+    gd1 = pos1 ? -iX1 : iX1;
+    gd2 = neg2 ? -iX2 : iX2;
+    grho = mid ? -k * fast_rcp(rho2) : T(0);
+    return fast_log2(num * fast_rcp(den));
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_adjoint_pairs(const FaceFrame<T>* __restrict__ faces,
+                                                       int64_t n_f,
+                                                       const StationW* __restrict__ st,
+                                                       int64_t n_s, int64_t st_per_chunk,
+                                                       T* __restrict__ seeds)
+{
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int chunk = blockIdx.y;
+    const bool active = f < n_f;
+    const FaceFrame<T> F = faces[active ? f : n_f - 1];
+    T acc[kSeeds];
+#pragma unroll
+    for (int k = 0; k < kSeeds; ++k) acc[k] = T(0);
+    const int64_t i0 = (int64_t)chunk * st_per_chunk;
+    const int64_t i1 = min(n_s, i0 + st_per_chunk);
+    for (int64_t i = i0; i < i1; ++i) {
+        const StationW S = st[i];
+        if (S.wg == 0.0 && S.wt == 0.0) continue;
+        const T lam = T(S.wg), mu = T(S.wt);
+        const T Px = T(S.x - F.ax), Py = T(S.y - F.ay), Pz = T(S.z - F.az);
+        const T px = Px * F.ux + Py * F.uy + Pz * F.uz;
+        const T py = Px * F.vx + Py * F.vy + Pz * F.vz;
+        const T pz = Px * F.nx + Py * F.ny + Pz * F.nz;
+        // ---- forward
+        const T pz2 = pz * pz;
+        const T rho0 = py * py + pz2;
+        const T da2 = px * px + rho0;
+        const T t = F.l0 - px;
+        const T db2 = t * t + rho0;
+        const T dcx = F.cu - px, dcy = F.cv - py;
+        const T dc2 = dcx * dcx + dcy * dcy + pz2;
+        const T da = fast_sqrt(da2), db = fast_sqrt(db2), dc = fast_sqrt(dc2);
+        const T rab = da2 - F.l0 * px;
+        const T rac = da2 - (F.cu * px + F.cv * py);
+        const T rbc = t * dcx - py * dcy + pz2;
+        const T D = da * db * dc + rab * dc + rac * db + rbc * da;
+        const T N = -F.twoA * pz;
+        const T th = atan2_t(N, D);
+        const T dp1 = t * F.e1v + py * F.e1u;
+        const T dp2 = dcx * F.e2v - dcy * F.e2u;
+        const T s1b = t * F.e1u - py * F.e1v;
+        const T s1c = dcx * F.e2u + dcy * F.e2v;
+        const T rho1 = dp1 * dp1 + pz2;
+        const T rho2 = dp2 * dp2 + pz2;
+        T g0s1, g0s2, g0d1, g0d2, g0r, g1s1, g1s2, g1d1, g1d2, g1r, g2s1, g2s2, g2d1, g2d2, g2r;
+        const T lam0 = edge_log2_grad(-px, t, da, db, rho0, g0s1, g0s2, g0d1, g0d2, g0r);
+        const T lam1 = edge_log2_grad(s1b, s1b + F.l1, db, dc, rho1, g1s1, g1s2, g1d1, g1d2, g1r);
+        const T lam2 = edge_log2_grad(s1c, s1c + F.l2, dc, da, rho2, g2s1, g2s2, g2d1, g2d2, g2r);
+        const T es = py * lam0 + dp1 * lam1 + dp2 * lam2;
+        // ---- coefficient seeds
+        acc[S_K1] += lam * es;
+        acc[S_K2] += lam * (pz * th);
+        acc[S_W0] += mu * th;
+        acc[S_W1] += mu * lam0;
+        acc[S_W2] += mu * lam1;
+        acc[S_W3] += mu * lam2;
+        // ---- reverse
+        const T ses = lam * F.K1;
+        const T sth = lam * F.K2 * pz + mu * F.W0;
+        T spz = lam * F.K2 * th;
+        const T sl0 = ses * py + mu * F.W1;
+        const T sl1 = ses * dp1 + mu * F.W2;
+        const T sl2 = ses * dp2 + mu * F.W3;
+        T spy = ses * lam0;
+        T sdp1 = ses * lam1;
+        T sdp2 = ses * lam2;
+        // edge 0: s1 = -px, s2 = t, d1 = da, d2 = db
+        T spx = -sl0 * g0s1;
+        T st_ = sl0 * g0s2;
+        T sda = sl0 * g0d1;
+        T sdb = sl0 * g0d2;
+        T srho0 = sl0 * g0r;
+        // edge 1: s1 = s1b, s2 = s1b + l1, d1 = db, d2 = dc
+        const T ss1b = sl1 * (g1s1 + g1s2);
+        const T sL1 = sl1 * g1s2;
+        sdb += sl1 * g1d1;
+        T sdc = sl1 * g1d2;
+        const T srho1 = sl1 * g1r;
+        // edge 2: s1 = s1c, s2 = s1c + l2, d1 = dc, d2 = da
+        const T ss1c = sl2 * (g2s1 + g2s2);
+        const T sL2 = sl2 * g2s2;
+        sdc += sl2 * g2d1;
+        sda += sl2 * g2d2;
+        const T srho2 = sl2 * g2r;
+        // rho1, rho2
+        sdp1 += T(2) * dp1 * srho1;
+        sdp2 += T(2) * dp2 * srho2;
+        T spz2 = srho1 + srho2;
+        // s1b = t e1u - py e1v
+        st_ += ss1b * F.e1u;
+        spy -= ss1b * F.e1v;
+        T se1u = ss1b * t;
+        T se1v = -ss1b * py;
+        // s1c = dcx e2u + dcy e2v
+        T sdcx = ss1c * F.e2u;
+        T sdcy = ss1c * F.e2v;
+        T se2u = ss1c * dcx;
+        T se2v = ss1c * dcy;
+        // dp1 = t e1v + py e1u
+        st_ += sdp1 * F.e1v;
+        spy += sdp1 * F.e1u;
+        se1v += sdp1 * t;
+        se1u += sdp1 * py;
+        // dp2 = dcx e2v - dcy e2u
+        sdcx += sdp2 * F.e2v;
+        sdcy -= sdp2 * F.e2u;
+        se2v += sdp2 * dcx;
+        se2u -= sdp2 * dcy;
+        // th = atan2(N, D)
+        const T q = fast_rcp(N * N + D * D);
+        const T sN = sth * D * q;
+        const T sD = -sth * N * q;
+        const T stwoA = -sN * pz;
+        spz -= sN * F.twoA;
+        // D = da db dc + rab dc + rac db + rbc da
+        sda += sD * (db * dc + rbc);
+        sdb += sD * (da * dc + rac);
+        sdc += sD * (da * db + rab);
+        const T srab = sD * dc;
+        const T srac = sD * db;
+        const T srbc = sD * da;
+        // rbc = t dcx - py dcy + pz2
+        st_ += srbc * dcx;
+        sdcx += srbc * t;
+        spy -= srbc * dcy;
+        sdcy -= srbc * py;
+        spz2 += srbc;
+        // rac = da2 - cu px - cv py ; rab = da2 - l0 px
+        T sda2 = srac + srab;
+        T scu = -srac * px;
+        T scv = -srac * py;
+        spx -= srac * F.cu + srab * F.l0;
+        spy -= srac * F.cv;
+        T sl0c = -srab * px;
+        // square roots
+        sda2 += T(0.5) * sda * fast_rcp(da);
+        const T sdb2 = T(0.5) * sdb * fast_rcp(db);
+        const T sdc2 = T(0.5) * sdc * fast_rcp(dc);
+        // dc2 = dcx^2 + dcy^2 + pz2
+        sdcx += T(2) * dcx * sdc2;
+        sdcy += T(2) * dcy * sdc2;
+        spz2 += sdc2;
+        // dcx = cu - px, dcy = cv - py
+        scu += sdcx;
+        spx -= sdcx;
+        scv += sdcy;
+        spy -= sdcy;
+        // db2 = t^2 + rho0
+        st_ += T(2) * t * sdb2;
+        srho0 += sdb2;
+        // t = l0 - px
+        sl0c += st_;
+        spx -= st_;
+        // da2 = px^2 + rho0 ; rho0 = py^2 + pz2 ; pz2 = pz^2
+        spx += T(2) * px * sda2;
+        srho0 += sda2;
+        spy += T(2) * py * srho0;
+        spz2 += srho0;
+        spz += T(2) * pz * spz2;
+        // ---- accumulate
+        acc[S_GPX] += spx;
+        acc[S_GPY] += spy;
+        acc[S_GPZ] += spz;
+        acc[S_AXY] += spx * py - spy * px;
+        acc[S_AXZ] += spx * pz - spz * px;
+        acc[S_AYZ] += spy * pz - spz * py;
+        acc[S_L0] += sl0c;
+        acc[S_CU] += scu;
+        acc[S_CV] += scv;
+        acc[S_E1U] += se1u;
+        acc[S_E1V] += se1v;
+        acc[S_L1] += sL1;
+        acc[S_E2U] += se2u;
+        acc[S_E2V] += se2v;
+        acc[S_L2] += sL2;
+        acc[S_TWOA] += stwoA;
+    }
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < kSeeds; ++k)
+            seeds[((int64_t)chunk * kSeeds + k) * n_f + f] = acc[k];
+    }
+}
+
+// Per facet: sum the chunk seeds in FP64, then push them through the facet frame and
+// constants along the nine vertex-coordinate directions.
+template <class T>
+__global__ __launch_bounds__(128) void k_adjoint_finalize(
+    const T* __restrict__ seeds, int n_chunks, int64_t n_f, const double* __restrict__ verts,
+    const uint32_t* __restrict__ tri, const int32_t* __restrict__ front,
+    const int32_t* __restrict__ back, const double* __restrict__ density,
+    const double* __restrict__ mag, double fx, double fy, double fz, int tmi_mode,
+    double* __restrict__ face_grad)
+{
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n_f) return;
+    double s[kSeeds];
+    for (int k = 0; k < kSeeds; ++k) {
+        double a = 0.0;
+        for (int c = 0; c < n_chunks; ++c) a += (double)seeds[((int64_t)c * kSeeds + k) * n_f + f];
+        s[k] = a;
+    }
+    double P[3][3];
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t vi = tri[3 * f + k];
+        for (int j = 0; j < 3; ++j) P[k][j] = verts[3 * (int64_t)vi + j];
+    }
+    const int uf = front[f], ub = back[f];
+    const double drho = density[uf] - (ub >= 0 ? density[ub] : 0.0);
+    double Md[3], Ms[3];
+    for (int j = 0; j < 3; ++j) {
+        const double mf = mag[3 * uf + j], mb = ub >= 0 ? mag[3 * ub + j] : 0.0;
+        Md[j] = mf - mb;
+        Ms[j] = mf + mb;
+    }
+    const double Fd[3] = {fx, fy, fz};
+    const bool textbook = tmi_mode == 1;
+    FrameVals<double> fv;
+    facet_frame(P, drho, Md, Ms, Fd, textbook, fv);
+    double* out = face_grad + 9 * f;
+    if (fv.bad) {
+        for (int k = 0; k < 9; ++k) out[k] = 0.0;
+        return;
+    }
+    double RtG[3];
+    for (int j = 0; j < 3; ++j)
+        RtG[j] = fv.u[j] * s[S_GPX] + fv.v[j] * s[S_GPY] + fv.n[j] * s[S_GPZ];
+    for (int k = 0; k < 9; ++k) {
+        Dual Q[3][3];
+        for (int c = 0; c < 3; ++c)
+            for (int j = 0; j < 3; ++j) Q[c][j] = Dual(P[c][j], (c * 3 + j == k) ? 1.0 : 0.0);
+        FrameVals<Dual> fd;
+        facet_frame(Q, drho, Md, Ms, Fd, textbook, fd);
+        double wxy = 0, wxz = 0, wyz = 0;
+        for (int j = 0; j < 3; ++j) {
+            wxy += fd.u[j].d * fv.v[j];
+            wxz += fd.u[j].d * fv.n[j];
+            wyz += fd.v[j].d * fv.n[j];
+        }
+        double g = wxy * s[S_AXY] + wxz * s[S_AXZ] + wyz * s[S_AYZ];
+        g += s[S_L0] * fd.l0.d + s[S_CU] * fd.cu.d + s[S_CV] * fd.cv.d;
+        g += s[S_E1U] * fd.e1u.d + s[S_E1V] * fd.e1v.d + s[S_L1] * fd.l1.d;
+        g += s[S_E2U] * fd.e2u.d + s[S_E2V] * fd.e2v.d + s[S_L2] * fd.l2.d;
+        g += s[S_TWOA] * fd.twoA.d;
+        g += s[S_K1] * fd.K1.d + s[S_K2] * fd.K2.d;
+        g += s[S_W0] * fd.W0.d + s[S_W1] * fd.W1.d + s[S_W2] * fd.W2.d + s[S_W3] * fd.W3.d;
+        if (k < 3) g -= RtG[k];
+        out[k] = g;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_vertex_gather(const double* __restrict__ face_grad,
+                                                       const int64_t* __restrict__ off,
+                                                       const int32_t* __restrict__ ent,
+                                                       int64_t n_v, double* __restrict__ grad)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n_v) return;
+    double gx = 0, gy = 0, gz = 0;
+    for (int64_t e = off[v]; e < off[v + 1]; ++e) {
+        const int64_t fc = ent[e];
+        const double* g = face_grad + 9 * (fc / 3) + 3 * (fc % 3);
+        gx += g[0];
+        gy += g[1];
+        gz += g[2];
+    }
+    grad[3 * v] = gx;
+    grad[3 * v + 1] = gy;
+    grad[3 * v + 2] = gz;
+}
+
+__global__ __launch_bounds__(256) void k_pack_stations(const double* __restrict__ sx,
+                                                       const double* __restrict__ sy,
+                                                       const double* __restrict__ sz,
+                                                       const double* __restrict__ wg,
+                                                       const double* __restrict__ wt,
+                                                       int64_t n_s, StationW* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_s) return;
+    StationW s;
+    s.x = sx[i];
+    s.y = sy[i];
+    s.z = sz[i];
+    s.wg = wg[i];
+    s.wt = wt[i];
+    s.pad0 = s.pad1 = s.pad2 = 0.0;
+    out[i] = s;
+}
+
+__global__ __launch_bounds__(256) void k_fill_zero(double* __restrict__ p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+// ----------------------------------------------------------------------- host side
+
+inline int adjoint_init(AdjointWork& w, int64_t n_v, int64_t n_f, int64_t n_s,
+                        const uint32_t* faces, int precision)
+{
+    w.n_v = n_v;
+    w.n_f = n_f;
+    w.n_s = n_s;
+    w.precision = precision;
+    // station chunks: enough blocks to fill the GPU, and at most 2048 stations per
+    // chunk so the per-chunk FP32 sums stay short
+    const int64_t face_blocks = std::max<int64_t>(1, (n_f + 255) / 256);
+    int64_t chunks = std::max<int64_t>(1, (4096 + face_blocks - 1) / face_blocks);
+    chunks = std::max<int64_t>(chunks, (n_s + 2047) / 2048);
+    chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, (n_s + 15) / 16));
+    chunks = std::min<int64_t>(chunks, 65535);
+    w.st_per_chunk = std::max<int64_t>(1, (n_s + chunks - 1) / chunks);
+    w.n_chunks = (int)std::max<int64_t>(1, (n_s + w.st_per_chunk - 1) / w.st_per_chunk);
+    const size_t esz = precision == 1 ? sizeof(float) : sizeof(double);
+    const size_t nseed = (size_t)w.n_chunks * kSeeds * std::max<int64_t>(n_f, 1);
+    if (hipMalloc(&w.d_seeds, nseed * esz) != hipSuccess) return -1;
+    if (hipMalloc((void**)&w.d_st, sizeof(StationW) * std::max<int64_t>(n_s, 1)) != hipSuccess) return -1;
+    if (hipMalloc((void**)&w.d_face_grad, sizeof(double) * 9 * std::max<int64_t>(n_f, 1)) != hipSuccess) return -1;
+    if (hipMalloc((void**)&w.d_csr_off, sizeof(int64_t) * (n_v + 1)) != hipSuccess) return -1;
+    if (hipMalloc((void**)&w.d_csr_ent, sizeof(int32_t) * std::max<int64_t>(3 * n_f, 1)) != hipSuccess) return -1;
+    if (hipMalloc((void**)&w.d_wg, sizeof(double) * std::max<int64_t>(n_s, 1)) != hipSuccess) return -1;
+    if (hipMalloc((void**)&w.d_wt, sizeof(double) * std::max<int64_t>(n_s, 1)) != hipSuccess) return -1;
+    if (hipMalloc((void**)&w.d_zero, sizeof(double) * std::max<int64_t>(n_s, 1)) != hipSuccess) return -1;
+    if (hipMalloc((void**)&w.d_grad, sizeof(double) * std::max<int64_t>(3 * n_v, 1)) != hipSuccess) return -1;
+    // vertex -> (facet, corner) CSR, in facet order
+    std::vector<int64_t> off(n_v + 1, 0);
+    for (int64_t i = 0; i < 3 * n_f; ++i) off[faces[i] + 1]++;
+    for (int64_t v = 0; v < n_v; ++v) off[v + 1] += off[v];
+    std::vector<int32_t> ent(std::max<int64_t>(3 * n_f, 1));
+    std::vector<int64_t> pos(off.begin(), off.end() - 1);
+    for (int64_t i = 0; i < 3 * n_f; ++i) ent[pos[faces[i]]++] = (int32_t)i;
+    if (hipMemcpy(w.d_csr_off, off.data(), sizeof(int64_t) * (n_v + 1), hipMemcpyHostToDevice) != hipSuccess) return -1;
+    if (n_f && hipMemcpy(w.d_csr_ent, ent.data(), sizeof(int32_t) * 3 * n_f, hipMemcpyHostToDevice) != hipSuccess) return -1;
+    std::vector<double> z(std::max<int64_t>(n_s, 1), 0.0);
+    if (hipMemcpy(w.d_zero, z.data(), sizeof(double) * z.size(), hipMemcpyHostToDevice) != hipSuccess) return -1;
+    return 0;
+}
+
+inline void adjoint_free(AdjointWork& w)
+{
+    void* bufs[] = {w.d_seeds, w.d_st, w.d_face_grad, w.d_csr_off, w.d_csr_ent,
+                    w.d_wg,    w.d_wt, w.d_zero,      w.d_grad};
+    for (void* p : bufs)
+        if (p) hipFree(p);
+    w = AdjointWork();
+}
+
+inline hipError_t adjoint_pairs(AdjointWork& w, const AdjointLaunch& L)
+{
+    if (L.n_s == 0 || L.n_f == 0) return hipSuccess;
+    const double* wg = L.wg ? L.wg : w.d_zero;
+    const double* wt = L.wt ? L.wt : w.d_zero;
+    hipLaunchKernelGGL(k_pack_stations, dim3((unsigned)((L.n_s + 255) / 256)), dim3(256), 0,
+                       L.stream, L.sx, L.sy, L.sz, wg, wt, L.n_s, w.d_st);
+    const dim3 grid((unsigned)((L.n_f + 255) / 256), (unsigned)w.n_chunks);
+    if (L.precision == 1)
+        hipLaunchKernelGGL(k_adjoint_pairs<float>, grid, dim3(256), 0, L.stream, L.f32, L.n_f,
+                           w.d_st, L.n_s, w.st_per_chunk, (float*)w.d_seeds);
+    else
+        hipLaunchKernelGGL(k_adjoint_pairs<double>, grid, dim3(256), 0, L.stream, L.f64, L.n_f,
+                           w.d_st, L.n_s, w.st_per_chunk, (double*)w.d_seeds);
+    return hipGetLastError();
+}
+
+inline hipError_t adjoint_finalize(AdjointWork& w, const AdjointLaunch& L, const double* verts,
+                                   const uint32_t* tri, const int32_t* front,
+                                   const int32_t* back, const double* density,
+                                   const double* mag, const double* fdir, int tmi_mode)
+{
+    if (L.n_f == 0) return hipSuccess;
+    const dim3 grid((unsigned)((L.n_f + 127) / 128));
+    const int nch = L.n_s == 0 ? 0 : w.n_chunks;
+    if (L.precision == 1)
+        hipLaunchKernelGGL(k_adjoint_finalize<float>, grid, dim3(128), 0, L.stream,
+                           (const float*)w.d_seeds, nch, L.n_f, verts, tri, front, back,
+                           density, mag, fdir[0], fdir[1], fdir[2], tmi_mode, w.d_face_grad);
+    else
+        hipLaunchKernelGGL(k_adjoint_finalize<double>, grid, dim3(128), 0, L.stream,
+                           (const double*)w.d_seeds, nch, L.n_f, verts, tri, front, back,
+                           density, mag, fdir[0], fdir[1], fdir[2], tmi_mode, w.d_face_grad);
+    return hipGetLastError();
+}
+
+inline hipError_t adjoint_gather(AdjointWork& w, const AdjointLaunch& L)
+{
+    if (L.n_v == 0) return hipSuccess;
+    if (L.n_f == 0) {
+        hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((3 * L.n_v + 255) / 256)), dim3(256), 0,
+                           L.stream, L.grad, 3 * L.n_v);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(k_vertex_gather, dim3((unsigned)((L.n_v + 255) / 256)), dim3(256), 0,
+                       L.stream, w.d_face_grad, w.d_csr_off, w.d_csr_ent, L.n_v, L.grad);
+    return hipGetLastError();
+}
+
+}  // namespace lfk

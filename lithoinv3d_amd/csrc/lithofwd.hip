@@ -1,0 +1,776 @@
+// lithofwd.hip -- gfx950 kernels and the C ABI (include/lithofwd.h) of the polyhedral
+// gz / TMI forward model and its adjoint.
+//
+// Kernels
+//   k_face_setup      one thread per facet: builds the station-independent facet record
+//                     (normal, edge frame, coefficients) from FP64 vertices
+//                     (the per-pair recomputation in gravity_forward.cpp:17-35 and
+//                     magnetic_forward.cpp:317-341 hoisted out of the station loop).
+//   k_forward_f32c    lane = station, wave-uniform facet: facet records arrive through
+//   k_forward_f64     scalar loads into SGPRs and feed the VALU directly; each lane keeps
+//                     its own (Kahan) accumulators, so the facet reduction needs no
+//                     cross-lane traffic.  Grid = station tiles x facet chunks.
+//   k_reduce_chunks   sums the per-chunk partials of each station in a fixed order.
+//   k_adjoint_*       see adjoint.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lithofwd.h"
+#include "pair_math.h"
+#include "adjoint.h"
+
+using namespace lfk;
+
+namespace {
+
+constexpr double kEps = 1e-15;                         // geometry.cpp:5
+constexpr int kBlock = 256;
+
+// ------------------------------------------------------------------- face setup
+
+__global__ __launch_bounds__(128) void k_face_setup(
+    const double* __restrict__ verts, const uint32_t* __restrict__ tri,
+    const int32_t* __restrict__ front, const int32_t* __restrict__ back,
+    const double* __restrict__ density, const double* __restrict__ mag, double fx,
+    double fy, double fz, int tmi_mode, int64_t n_f, FaceF32* __restrict__ out32,
+    FaceF64* __restrict__ out64, FaceFrame<double>* __restrict__ outfr64,
+    double* __restrict__ geo64)
+{
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n_f) return;
+    double P[3][3];
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t vi = tri[3 * f + k];
+        for (int j = 0; j < 3; ++j) P[k][j] = verts[3 * (int64_t)vi + j];
+    }
+    const int uf = front[f], ub = back[f];
+    const double drho = density[uf] - (ub >= 0 ? density[ub] : 0.0);
+    double Md[3], Ms[3];
+    for (int j = 0; j < 3; ++j) {
+        const double mf = mag[3 * uf + j];
+        const double mb = ub >= 0 ? mag[3 * ub + j] : 0.0;
+        Md[j] = mf - mb;  // the Omega term flips with the winding
+        Ms[j] = mf + mb;  // the reference edge term does not: t = n x e is reversal-invariant
+    }
+    const double Fd[3] = {fx, fy, fz};
+    FrameVals<double> fv;
+    facet_frame(P, drho, Md, Ms, Fd, tmi_mode == LF_TMI_TEXTBOOK, fv);
+    if (out64) {
+        FaceF64 r;
+        for (int j = 0; j < 3; ++j) {
+            r.a[j] = P[0][j];
+            r.b[j] = P[1][j];
+            r.c[j] = P[2][j];
+            r.n[j] = fv.n[j];
+            for (int k = 0; k < 3; ++k) {
+                r.eh[k][j] = fv.eh[k][j];
+                r.perp[k][j] = fv.edge_ok[k] ? fv.perp[k][j] : 0.0;
+            }
+        }
+        r.kz = fv.kz;
+        r.wom = fv.wom;
+        for (int k = 0; k < 3; ++k) r.we[k] = fv.we[k];
+        r.pad = 0.0;
+        out64[f] = r;
+    }
+    if (outfr64) {
+        FaceFrame<double> r;
+        store_frame(fv, P[0], r);
+        outfr64[f] = r;
+    }
+    if (out32) {
+        FaceF32 r;
+        store_frame(fv, P[0], r);
+        out32[f] = r;
+    }
+    if (geo64) {
+        for (int j = 0; j < 3; ++j) {
+            geo64[6 * f + j] = P[1][j];
+            geo64[6 * f + 3 + j] = P[2][j];
+        }
+    }
+}
+
+__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+
+// Block -> (station tile, facet chunk).  Consecutive block ids are dealt round-robin to
+// the 8 XCDs; remap so that each XCD works through a contiguous range of chunks and the
+// facet records of those chunks stay in that XCD's L2.
+__device__ __forceinline__ void block_coords(int n_tiles, int& tile, int& chunk)
+{
+    int64_t nb = gridDim.x;
+    int64_t b = blockIdx.x;
+    if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+    tile = (int)(b % n_tiles);
+    chunk = (int)(b / n_tiles);
+}
+
+// ----------------------------------------------------------------- FP32C forward
+
+template <bool GZ, bool TMI>
+__global__ __launch_bounds__(256) void k_forward_f32c(
+    const FaceF32* __restrict__ faces, const double* __restrict__ geo64, int64_t n_f,
+    int64_t faces_per_chunk, const double* __restrict__ sx, const double* __restrict__ sy,
+    const double* __restrict__ sz, int64_t n_s, int n_tiles, double* __restrict__ part_gz,
+    double* __restrict__ part_tmi, unsigned long long* __restrict__ fb_count)
+{
+    int tile, chunk;
+    block_coords(n_tiles, tile, chunk);
+    const int64_t s = (int64_t)tile * kBlock + threadIdx.x;
+    const bool active = s < n_s;
+    const int64_t si = active ? s : n_s - 1;
+    const double ox = sx[si], oy = sy[si], oz = sz[si];
+    const int64_t f0 = (int64_t)chunk * faces_per_chunk;
+    const int64_t f1 = min(n_f, f0 + faces_per_chunk);
+
+    float gs = 0.f, gc = 0.f, ts = 0.f, tc = 0.f;
+    unsigned nfb = 0;
+    for (int64_t f = f0; f < f1; ++f) {
+        const FaceF32& F = faces[f];
+        // station offset from vertex a: FP64 difference, one rounding
+        const float Px = (float)(ox - F.ax);
+        const float Py = (float)(oy - F.ay);
+        const float Pz = (float)(oz - F.az);
+        const float px = Px * F.ux + Py * F.uy + Pz * F.uz;
+        const float py = Px * F.vx + Py * F.vy + Pz * F.vz;
+        const float pz = Px * F.nx + Py * F.ny + Pz * F.nz;  // h = -pz
+        const float pz2 = pz * pz;
+        const float rho0 = fmaf(py, py, pz2);
+        const float da2 = fmaf(px, px, rho0);
+        const float t = F.l0 - px;  // (b - o).u
+        const float db2 = fmaf(t, t, rho0);
+        const float dcx = F.cu - px, dcy = F.cv - py;
+        const float dc2 = fmaf(dcx, dcx, fmaf(dcy, dcy, pz2));
+        const float da = fast_sqrt(da2), db = fast_sqrt(db2), dc = fast_sqrt(dc2);
+        // solid angle: N = r_a.(r_b x r_c) = 2A*h, D from the pairwise dots
+        const float rab = fmaf(-F.l0, px, da2);
+        const float rac = da2 - fmaf(F.cu, px, F.cv * py);
+        const float rbc = fmaf(t, dcx, fmaf(-py, dcy, pz2));
+        const float D = fmaf(da * db, dc, fmaf(rab, dc, fmaf(rac, db, rbc * da)));
+        const float N = -F.twoA * pz;
+        const float th = atan2_t(N, D);  // Omega = 2*th
+        // edges: (s1, s2, d1, d2, dperp, rho^2)
+        const float dp1 = fmaf(t, F.e1v, py * F.e1u);
+        const float dp2 = fmaf(dcx, F.e2v, -dcy * F.e2u);
+        const float s1b = fmaf(t, F.e1u, -py * F.e1v);
+        const float s1c = fmaf(dcx, F.e2u, dcy * F.e2v);
+        const float rho1 = fmaf(dp1, dp1, pz2);
+        const float rho2 = fmaf(dp2, dp2, pz2);
+        float lam0 = edge_log2<float>(-px, t, da, db, rho0);
+        float lam1 = edge_log2<float>(s1b, s1b + F.l1, db, dc, rho1);
+        float lam2 = edge_log2<float>(s1c, s1c + F.l2, dc, da, rho2);
+        // FP64 for edges whose line passes close to the station while the station
+        // projects inside the edge (rho < d/32): there the FP32 offsets' absolute
+        // error would dominate the log.
+        const bool fb0 = (-px < 0.f) && (t >= 0.f) && (rho0 * 1024.f < da * db);
+        const bool fb1 = (s1b < 0.f) && (s1b + F.l1 >= 0.f) && (rho1 * 1024.f < db * dc);
+        const bool fb2 = (s1c < 0.f) && (s1c + F.l2 >= 0.f) && (rho2 * 1024.f < dc * da);
+        if (fb0 | fb1 | fb2) {
+            const double* g = geo64 + 6 * f;
+            if (fb0) lam0 = edge_log2_f64(ox, oy, oz, F.ax, F.ay, F.az, g[0], g[1], g[2]);
+            if (fb1) lam1 = edge_log2_f64(ox, oy, oz, g[0], g[1], g[2], g[3], g[4], g[5]);
+            if (fb2) lam2 = edge_log2_f64(ox, oy, oz, g[3], g[4], g[5], F.ax, F.ay, F.az);
+            nfb += (unsigned)fb0 + (unsigned)fb1 + (unsigned)fb2;
+        }
+        if (GZ) {
+            const float es = fmaf(py, lam0, fmaf(dp1, lam1, dp2 * lam2));
+            kahan_add(gs, gc, fmaf(F.K1, es, F.K2 * (pz * th)));
+        }
+        if (TMI) {
+            kahan_add(ts, tc, fmaf(F.W0, th, fmaf(F.W1, lam0, fmaf(F.W2, lam1, F.W3 * lam2))));
+        }
+    }
+    if (active) {
+        if (GZ) part_gz[(int64_t)chunk * n_s + s] = (double)gs - (double)gc;
+        if (TMI) part_tmi[(int64_t)chunk * n_s + s] = (double)ts - (double)tc;
+        if (nfb) atomicAdd(fb_count, (unsigned long long)nfb);
+    }
+}
+
+// ------------------------------------------------------------------ FP64 forward
+
+template <bool GZ, bool TMI>
+__global__ __launch_bounds__(256) void k_forward_f64(
+    const FaceF64* __restrict__ faces, int64_t n_f, int64_t faces_per_chunk,
+    const double* __restrict__ sx, const double* __restrict__ sy,
+    const double* __restrict__ sz, int64_t n_s, int n_tiles, double* __restrict__ part_gz,
+    double* __restrict__ part_tmi)
+{
+    int tile, chunk;
+    block_coords(n_tiles, tile, chunk);
+    const int64_t s = (int64_t)tile * kBlock + threadIdx.x;
+    const bool active = s < n_s;
+    const int64_t si = active ? s : n_s - 1;
+    const double o[3] = {sx[si], sy[si], sz[si]};
+    const int64_t f0 = (int64_t)chunk * faces_per_chunk;
+    const int64_t f1 = min(n_f, f0 + faces_per_chunk);
+    double gsum = 0.0, tsum = 0.0;
+    for (int64_t f = f0; f < f1; ++f) {
+        const FaceF64& F = faces[f];
+        double r[3][3], d[3];
+        for (int j = 0; j < 3; ++j) {
+            r[0][j] = F.a[j] - o[j];
+            r[1][j] = F.b[j] - o[j];
+            r[2][j] = F.c[j] - o[j];
+        }
+        for (int k = 0; k < 3; ++k)
+            d[k] = sqrt(r[k][0] * r[k][0] + r[k][1] * r[k][1] + r[k][2] * r[k][2]);
+        // solidAngle, geometry.cpp:10-30
+        double omega = 0.0;
+        if (!(d[0] < kEps || d[1] < kEps || d[2] < kEps)) {
+            const double cx = r[1][1] * r[2][2] - r[1][2] * r[2][1];
+            const double cy = r[1][2] * r[2][0] - r[1][0] * r[2][2];
+            const double cz = r[1][0] * r[2][1] - r[1][1] * r[2][0];
+            const double num = r[0][0] * cx + r[0][1] * cy + r[0][2] * cz;
+            const double dab = r[0][0] * r[1][0] + r[0][1] * r[1][1] + r[0][2] * r[1][2];
+            const double dac = r[0][0] * r[2][0] + r[0][1] * r[2][1] + r[0][2] * r[2][2];
+            const double dbc = r[1][0] * r[2][0] + r[1][1] * r[2][1] + r[1][2] * r[2][2];
+            const double den = d[0] * d[1] * d[2] + dab * d[2] + dac * d[1] + dbc * d[0];
+            omega = 2.0 * atan2(num, den);
+        }
+        const double h = r[0][0] * F.n[0] + r[0][1] * F.n[1] + r[0][2] * F.n[2];
+        double dsum = 0.0, tm = F.wom * omega;
+        for (int k = 0; k < 3; ++k) {
+            const int k2 = (k + 1) % 3;
+            // lineIntegralTerm, geometry.cpp:32-52
+            const double s1 = r[k][0] * F.eh[k][0] + r[k][1] * F.eh[k][1] + r[k][2] * F.eh[k][2];
+            const double s2 = r[k2][0] * F.eh[k][0] + r[k2][1] * F.eh[k][1] + r[k2][2] * F.eh[k][2];
+            const double num = s2 + d[k2];
+            const double den = s1 + d[k];
+            double L = 0.0;
+            if (!(fabs(den) < kEps) && num > 0.0 && den > 0.0) L = log(num / den);
+            // (o - p1).(n x e_hat)
+            const double dperp =
+                -(r[k][0] * F.perp[k][0] + r[k][1] * F.perp[k][1] + r[k][2] * F.perp[k][2]);
+            dsum += dperp * L;
+            tm += F.we[k] * L;
+        }
+        if (GZ) gsum += F.kz * (dsum - h * omega);
+        if (TMI) tsum += tm;
+    }
+    if (active) {
+        if (GZ) part_gz[(int64_t)chunk * n_s + s] = gsum;
+        if (TMI) part_tmi[(int64_t)chunk * n_s + s] = tsum;
+    }
+}
+
+// ------------------------------------------------------------------------ reduce
+
+__global__ __launch_bounds__(256) void k_reduce_chunks(const double* __restrict__ part,
+                                                       int n_chunks, int64_t n_s,
+                                                       double* __restrict__ out)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_s) return;
+    double acc = 0.0;
+    for (int c = 0; c < n_chunks; ++c) acc += part[(int64_t)c * n_s + s];
+    out[s] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_split_xyz(const double* __restrict__ xyz, int64_t n,
+                                                   double* __restrict__ x,
+                                                   double* __restrict__ y,
+                                                   double* __restrict__ z)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x[i] = xyz[3 * i];
+    y[i] = xyz[3 * i + 1];
+    z[i] = xyz[3 * i + 2];
+}
+
+}  // namespace
+
+// =========================================================================== context
+
+static thread_local std::string g_create_error;
+
+struct lf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t n_v = 0, n_f = 0, n_s = 0;
+    int32_t n_u = 0;
+    int precision = LF_FP64, tmi_mode = LF_TMI_REFERENCE, want = 3;
+    double fdir[3] = {0, 0, 1};
+    std::string err;
+    bool faces_dirty = true;
+
+    // model
+    double* d_verts = nullptr;  // n_v*3
+    uint32_t* d_tri = nullptr;
+    int32_t* d_front = nullptr;
+    int32_t* d_back = nullptr;
+    double* d_density = nullptr;
+    double* d_mag = nullptr;
+    double* d_sx = nullptr;
+    double* d_sy = nullptr;
+    double* d_sz = nullptr;
+    // facet records
+    FaceF32* d_f32 = nullptr;
+    FaceF64* d_f64 = nullptr;
+    FaceFrame<double>* d_fr64 = nullptr;
+    double* d_geo64 = nullptr;
+    // forward work
+    int n_chunks = 1, n_tiles = 1;
+    int64_t faces_per_chunk = 0;
+    double* d_part_gz = nullptr;
+    double* d_part_tmi = nullptr;
+    double* d_out_gz = nullptr;
+    double* d_out_tmi = nullptr;
+    unsigned long long* d_fb = nullptr;
+    // adjoint work
+    AdjointWork adj;
+    // timing
+    bool timing = false;
+    struct Ev { int cls; hipEvent_t a, b; };
+    std::vector<Ev> events;
+    double ms[8] = {0};
+    int64_t launches[8] = {0};
+};
+
+#define LF_TRY(ctx, expr)                                                              \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);            \
+            return LF_EHIP;                                                            \
+        }                                                                              \
+    } while (0)
+
+namespace {
+
+int set_err(lf_ctx* c, const std::string& m, int code)
+{
+    c->err = m;
+    return code;
+}
+
+template <class T>
+hipError_t dalloc(T** p, size_t count)
+{
+    return hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T));
+}
+
+struct TimedLaunch {
+    lf_ctx* c;
+    int cls;
+    hipEvent_t a = nullptr, b = nullptr;
+    TimedLaunch(lf_ctx* c_, int cls_) : c(c_), cls(cls_)
+    {
+        if (c->timing) {
+            hipEventCreate(&a);
+            hipEventCreate(&b);
+            hipEventRecord(a, c->stream);
+        }
+    }
+    ~TimedLaunch()
+    {
+        if (c->timing) {
+            hipEventRecord(b, c->stream);
+            c->events.push_back({cls, a, b});
+        }
+    }
+};
+
+int choose_chunks(lf_ctx* c)
+{
+    c->n_tiles = (int)((c->n_s + kBlock - 1) / kBlock);
+    int dev_cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
+        dev_cus = prop.multiProcessorCount;
+    const char* env = getenv("LITHOFWD_BLOCKS_PER_CU");
+    const int per_cu = env ? std::max(1, atoi(env)) : 16;
+    const int64_t target = (int64_t)dev_cus * per_cu;
+    const int64_t tiles = std::max(1, c->n_tiles);
+    int64_t chunks = (target + tiles - 1) / tiles;
+    const int64_t max_chunks = std::max<int64_t>(1, c->n_f / 32);
+    chunks = std::max<int64_t>(1, std::min(chunks, max_chunks));
+    c->faces_per_chunk = (c->n_f + chunks - 1) / chunks;
+    if (c->faces_per_chunk < 1) c->faces_per_chunk = 1;
+    c->n_chunks = (int)((c->n_f + c->faces_per_chunk - 1) / c->faces_per_chunk);
+    if (c->n_chunks < 1) c->n_chunks = 1;
+    return LF_OK;
+}
+
+int ensure_faces(lf_ctx* c)
+{
+    if (!c->faces_dirty) return LF_OK;
+    if (c->n_f > 0) {
+        TimedLaunch tl(c, 0);
+        const int64_t blocks = (c->n_f + 127) / 128;
+        hipLaunchKernelGGL(k_face_setup, dim3((unsigned)blocks), dim3(128), 0, c->stream,
+                           c->d_verts, c->d_tri, c->d_front, c->d_back, c->d_density, c->d_mag,
+                           c->fdir[0], c->fdir[1], c->fdir[2], c->tmi_mode, c->n_f,
+                           c->d_f32, c->d_f64, c->d_fr64, c->d_geo64);
+        LF_TRY(c, hipGetLastError());
+    }
+    c->faces_dirty = false;
+    return LF_OK;
+}
+
+template <bool GZ, bool TMI>
+void launch_forward(lf_ctx* c)
+{
+    const dim3 grid((unsigned)((int64_t)c->n_tiles * c->n_chunks));
+    if (c->precision == LF_FP32C)
+        hipLaunchKernelGGL((k_forward_f32c<GZ, TMI>), grid, dim3(kBlock), 0, c->stream, c->d_f32,
+                           c->d_geo64, c->n_f, c->faces_per_chunk, c->d_sx, c->d_sy, c->d_sz,
+                           c->n_s, c->n_tiles, c->d_part_gz, c->d_part_tmi, c->d_fb);
+    else
+        hipLaunchKernelGGL((k_forward_f64<GZ, TMI>), grid, dim3(kBlock), 0, c->stream, c->d_f64,
+                           c->n_f, c->faces_per_chunk, c->d_sx, c->d_sy, c->d_sz, c->n_s,
+                           c->n_tiles, c->d_part_gz, c->d_part_tmi);
+}
+
+int forward_device(lf_ctx* c, double* d_gz, double* d_tmi)
+{
+    int rc = ensure_faces(c);
+    if (rc) return rc;
+    const bool gz = d_gz && (c->want & LF_WANT_GZ);
+    const bool tm = d_tmi && (c->want & LF_WANT_TMI);
+    if (c->n_s == 0) return LF_OK;
+    const unsigned sblocks = (unsigned)((c->n_s + 255) / 256);
+    if (c->n_f == 0) {
+        if (gz) hipLaunchKernelGGL(k_fill_zero, dim3(sblocks), dim3(256), 0, c->stream, d_gz, c->n_s);
+        if (tm) hipLaunchKernelGGL(k_fill_zero, dim3(sblocks), dim3(256), 0, c->stream, d_tmi, c->n_s);
+        LF_TRY(c, hipGetLastError());
+        return LF_OK;
+    }
+    if (!gz && !tm) return LF_OK;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, c->stream, c->d_fb);
+    {
+        TimedLaunch tl(c, 1);
+        if (gz && tm) launch_forward<true, true>(c);
+        else if (gz) launch_forward<true, false>(c);
+        else launch_forward<false, true>(c);
+        LF_TRY(c, hipGetLastError());
+    }
+    {
+        TimedLaunch tl(c, 2);
+        const unsigned blocks = (unsigned)((c->n_s + 255) / 256);
+        if (gz)
+            hipLaunchKernelGGL(k_reduce_chunks, dim3(blocks), dim3(256), 0, c->stream,
+                               c->d_part_gz, c->n_chunks, c->n_s, d_gz);
+        if (tm)
+            hipLaunchKernelGGL(k_reduce_chunks, dim3(blocks), dim3(256), 0, c->stream,
+                               c->d_part_tmi, c->n_chunks, c->n_s, d_tmi);
+        LF_TRY(c, hipGetLastError());
+    }
+    return LF_OK;
+}
+
+int check_ctx(const lf_ctx* c) { return c ? LF_OK : LF_EINVAL; }
+
+}  // namespace
+
+// ============================================================================== ABI
+
+extern "C" {
+
+const char* lf_last_error(const lf_ctx* ctx)
+{
+    return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+void lf_destroy(lf_ctx* c)
+{
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto& e : c->events) {
+        hipEventDestroy(e.a);
+        hipEventDestroy(e.b);
+    }
+    void* bufs[] = {c->d_verts, c->d_tri,  c->d_front,    c->d_back,       c->d_density,
+                    c->d_mag,   c->d_sx,   c->d_sy,       c->d_sz,         c->d_f32,
+                    c->d_f64,   c->d_fr64, c->d_geo64, c->d_part_gz, c->d_part_tmi, c->d_out_gz,
+                    c->d_out_tmi, c->d_fb};
+    for (void* p : bufs)
+        if (p) hipFree(p);
+    adjoint_free(c->adj);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int lf_create(lf_ctx** out, const lf_desc* d)
+{
+    g_create_error.clear();
+    if (!out || !d) {
+        g_create_error = "lf_create: null argument";
+        return LF_EINVAL;
+    }
+    *out = nullptr;
+    if (d->n_vertices < 0 || d->n_faces < 0 || d->n_stations < 0 || d->n_units <= 0 ||
+        (d->n_faces > 0 && (!d->vertices || !d->faces || !d->front || !d->back)) ||
+        (d->n_stations > 0 && !d->stations) || !d->density || !d->magnetization ||
+        (d->precision != LF_FP64 && d->precision != LF_FP32C) ||
+        (d->tmi_mode != LF_TMI_REFERENCE && d->tmi_mode != LF_TMI_TEXTBOOK) ||
+        (d->want & ~3) != 0 || d->n_vertices > 0xffffffffLL) {
+        g_create_error = "lf_create: invalid descriptor";
+        return LF_EINVAL;
+    }
+    for (int64_t f = 0; f < d->n_faces; ++f) {
+        for (int k = 0; k < 3; ++k)
+            if ((int64_t)d->faces[3 * f + k] >= d->n_vertices) {
+                g_create_error = "lf_create: face " + std::to_string(f) + " references vertex " +
+                                 std::to_string(d->faces[3 * f + k]) + " >= n_vertices";
+                return LF_EINVAL;
+            }
+        if (d->front[f] < 0 || d->front[f] >= d->n_units || d->back[f] < -1 ||
+            d->back[f] >= d->n_units) {
+            g_create_error = "lf_create: face " + std::to_string(f) + " has an invalid unit id";
+            return LF_EINVAL;
+        }
+    }
+    lf_ctx* c = new lf_ctx();
+    c->device = d->device;
+    c->n_v = d->n_vertices;
+    c->n_f = d->n_faces;
+    c->n_s = d->n_stations;
+    c->n_u = d->n_units;
+    c->precision = d->precision;
+    c->tmi_mode = d->tmi_mode;
+    c->want = d->want ? d->want : 3;
+    for (int j = 0; j < 3; ++j) c->fdir[j] = d->field_dir[j];
+
+    auto fail = [&](int rc) {
+        g_create_error = c->err;
+        lf_destroy(c);
+        return rc;
+    };
+#define CR(expr)                                                                          \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            c->err = std::string(#expr) + ": " + hipGetErrorString(e_);                   \
+            return fail(LF_EHIP);                                                         \
+        }                                                                                 \
+    } while (0)
+    CR(hipSetDevice(c->device));
+    CR(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    CR(dalloc(&c->d_verts, 3 * c->n_v));
+    CR(dalloc(&c->d_tri, 3 * c->n_f));
+    CR(dalloc(&c->d_front, c->n_f));
+    CR(dalloc(&c->d_back, c->n_f));
+    CR(dalloc(&c->d_density, c->n_u));
+    CR(dalloc(&c->d_mag, 3 * c->n_u));
+    CR(dalloc(&c->d_sx, c->n_s));
+    CR(dalloc(&c->d_sy, c->n_s));
+    CR(dalloc(&c->d_sz, c->n_s));
+    if (c->precision == LF_FP32C) {
+        CR(dalloc(&c->d_f32, c->n_f));
+        CR(dalloc(&c->d_geo64, 6 * c->n_f));
+    } else {
+        CR(dalloc(&c->d_f64, c->n_f));
+        CR(dalloc(&c->d_fr64, c->n_f));
+    }
+    choose_chunks(c);
+    CR(dalloc(&c->d_part_gz, (size_t)c->n_chunks * c->n_s));
+    CR(dalloc(&c->d_part_tmi, (size_t)c->n_chunks * c->n_s));
+    CR(dalloc(&c->d_out_gz, c->n_s));
+    CR(dalloc(&c->d_out_tmi, c->n_s));
+    CR(dalloc(&c->d_fb, 1));
+    if (c->n_v) CR(hipMemcpy(c->d_verts, d->vertices, sizeof(double) * 3 * c->n_v, hipMemcpyHostToDevice));
+    if (c->n_f) {
+        CR(hipMemcpy(c->d_tri, d->faces, sizeof(uint32_t) * 3 * c->n_f, hipMemcpyHostToDevice));
+        CR(hipMemcpy(c->d_front, d->front, sizeof(int32_t) * c->n_f, hipMemcpyHostToDevice));
+        CR(hipMemcpy(c->d_back, d->back, sizeof(int32_t) * c->n_f, hipMemcpyHostToDevice));
+    }
+    CR(hipMemcpy(c->d_density, d->density, sizeof(double) * c->n_u, hipMemcpyHostToDevice));
+    CR(hipMemcpy(c->d_mag, d->magnetization, sizeof(double) * 3 * c->n_u, hipMemcpyHostToDevice));
+    if (c->n_s) {
+        std::vector<double> tmp(3 * c->n_s);
+        for (int64_t i = 0; i < c->n_s; ++i) {
+            tmp[i] = d->stations[3 * i];
+            tmp[c->n_s + i] = d->stations[3 * i + 1];
+            tmp[2 * c->n_s + i] = d->stations[3 * i + 2];
+        }
+        CR(hipMemcpy(c->d_sx, tmp.data(), sizeof(double) * c->n_s, hipMemcpyHostToDevice));
+        CR(hipMemcpy(c->d_sy, tmp.data() + c->n_s, sizeof(double) * c->n_s, hipMemcpyHostToDevice));
+        CR(hipMemcpy(c->d_sz, tmp.data() + 2 * c->n_s, sizeof(double) * c->n_s, hipMemcpyHostToDevice));
+    }
+    {
+        int rc = adjoint_init(c->adj, c->n_v, c->n_f, c->n_s, d->faces, c->precision);
+        if (rc) {
+            c->err = "adjoint workspace: " + std::string(hipGetErrorString(hipGetLastError()));
+            return fail(LF_EHIP);
+        }
+    }
+#undef CR
+    c->faces_dirty = true;
+    *out = c;
+    return LF_OK;
+}
+
+int lf_set_vertices(lf_ctx* c, const double* xyz)
+{
+    if (check_ctx(c)) return LF_EINVAL;
+    if (!xyz && c->n_v) return set_err(c, "lf_set_vertices: null vertices", LF_EINVAL);
+    LF_TRY(c, hipSetDevice(c->device));
+    if (c->n_v)
+        LF_TRY(c, hipMemcpyAsync(c->d_verts, xyz, sizeof(double) * 3 * c->n_v,
+                                 hipMemcpyHostToDevice, c->stream));
+    LF_TRY(c, hipStreamSynchronize(c->stream));
+    c->faces_dirty = true;
+    return LF_OK;
+}
+
+int lf_set_vertices_device(lf_ctx* c, const double* d_xyz)
+{
+    if (check_ctx(c)) return LF_EINVAL;
+    if (!d_xyz && c->n_v) return set_err(c, "lf_set_vertices_device: null vertices", LF_EINVAL);
+    LF_TRY(c, hipSetDevice(c->device));
+    if (c->n_v)
+        LF_TRY(c, hipMemcpyAsync(c->d_verts, d_xyz, sizeof(double) * 3 * c->n_v,
+                                 hipMemcpyDeviceToDevice, c->stream));
+    c->faces_dirty = true;
+    return LF_OK;
+}
+
+int lf_set_properties(lf_ctx* c, const double* density, const double* mag)
+{
+    if (check_ctx(c)) return LF_EINVAL;
+    LF_TRY(c, hipSetDevice(c->device));
+    if (density)
+        LF_TRY(c, hipMemcpyAsync(c->d_density, density, sizeof(double) * c->n_u,
+                                 hipMemcpyHostToDevice, c->stream));
+    if (mag)
+        LF_TRY(c, hipMemcpyAsync(c->d_mag, mag, sizeof(double) * 3 * c->n_u,
+                                 hipMemcpyHostToDevice, c->stream));
+    LF_TRY(c, hipStreamSynchronize(c->stream));
+    c->faces_dirty = true;
+    return LF_OK;
+}
+
+int lf_forward_device(lf_ctx* c, double* d_gz, double* d_tmi)
+{
+    if (check_ctx(c)) return LF_EINVAL;
+    LF_TRY(c, hipSetDevice(c->device));
+    return forward_device(c, d_gz, d_tmi);
+}
+
+int lf_forward(lf_ctx* c, double* gz, double* tmi)
+{
+    if (check_ctx(c)) return LF_EINVAL;
+    LF_TRY(c, hipSetDevice(c->device));
+    int rc = forward_device(c, gz ? c->d_out_gz : nullptr, tmi ? c->d_out_tmi : nullptr);
+    if (rc) return rc;
+    if (gz && c->n_s)
+        LF_TRY(c, hipMemcpyAsync(gz, c->d_out_gz, sizeof(double) * c->n_s,
+                                 hipMemcpyDeviceToHost, c->stream));
+    if (tmi && c->n_s)
+        LF_TRY(c, hipMemcpyAsync(tmi, c->d_out_tmi, sizeof(double) * c->n_s,
+                                 hipMemcpyDeviceToHost, c->stream));
+    LF_TRY(c, hipStreamSynchronize(c->stream));
+    return LF_OK;
+}
+
+int lf_adjoint_device(lf_ctx* c, const double* d_wg, const double* d_wt, double* d_grad)
+{
+    if (check_ctx(c)) return LF_EINVAL;
+    if (!d_grad) return set_err(c, "lf_adjoint_device: null gradient", LF_EINVAL);
+    LF_TRY(c, hipSetDevice(c->device));
+    int rc = ensure_faces(c);
+    if (rc) return rc;
+    const double* wg = (c->want & LF_WANT_GZ) ? d_wg : nullptr;
+    const double* wt = (c->want & LF_WANT_TMI) ? d_wt : nullptr;
+    AdjointLaunch L{c->stream, c->n_v, c->n_f, c->n_s, c->precision, c->d_f32, c->d_fr64,
+                    c->d_sx, c->d_sy, c->d_sz, wg, wt, d_grad};
+    hipError_t e;
+    {
+        TimedLaunch t3(c, 3);
+        e = adjoint_pairs(c->adj, L);
+    }
+    if (e != hipSuccess) return set_err(c, std::string("adjoint pairs: ") + hipGetErrorString(e), LF_EHIP);
+    {
+        TimedLaunch t4(c, 4);
+        e = adjoint_finalize(c->adj, L, c->d_verts, c->d_tri, c->d_front, c->d_back,
+                             c->d_density, c->d_mag, c->fdir, c->tmi_mode);
+    }
+    if (e != hipSuccess) return set_err(c, std::string("adjoint finalize: ") + hipGetErrorString(e), LF_EHIP);
+    {
+        TimedLaunch t5(c, 5);
+        e = adjoint_gather(c->adj, L);
+    }
+    if (e != hipSuccess) return set_err(c, std::string("adjoint gather: ") + hipGetErrorString(e), LF_EHIP);
+    return LF_OK;
+}
+
+int lf_adjoint(lf_ctx* c, const double* w_gz, const double* w_tmi, double* grad)
+{
+    if (check_ctx(c)) return LF_EINVAL;
+    if (!grad) return set_err(c, "lf_adjoint: null gradient", LF_EINVAL);
+    LF_TRY(c, hipSetDevice(c->device));
+    double* dwg = c->adj.d_wg;
+    double* dwt = c->adj.d_wt;
+    if (w_gz && c->n_s)
+        LF_TRY(c, hipMemcpyAsync(dwg, w_gz, sizeof(double) * c->n_s, hipMemcpyHostToDevice, c->stream));
+    if (w_tmi && c->n_s)
+        LF_TRY(c, hipMemcpyAsync(dwt, w_tmi, sizeof(double) * c->n_s, hipMemcpyHostToDevice, c->stream));
+    int rc = lf_adjoint_device(c, w_gz ? dwg : nullptr, w_tmi ? dwt : nullptr, c->adj.d_grad);
+    if (rc) return rc;
+    if (c->n_v)
+        LF_TRY(c, hipMemcpyAsync(grad, c->adj.d_grad, sizeof(double) * 3 * c->n_v,
+                                 hipMemcpyDeviceToHost, c->stream));
+    LF_TRY(c, hipStreamSynchronize(c->stream));
+    return LF_OK;
+}
+
+int lf_sync(lf_ctx* c)
+{
+    if (check_ctx(c)) return LF_EINVAL;
+    LF_TRY(c, hipSetDevice(c->device));
+    LF_TRY(c, hipStreamSynchronize(c->stream));
+    return LF_OK;
+}
+
+int lf_set_timing(lf_ctx* c, int enable)
+{
+    if (check_ctx(c)) return LF_EINVAL;
+    c->timing = enable != 0;
+    return LF_OK;
+}
+
+int lf_timing(lf_ctx* c, double* ms, int64_t* launches, int n_classes)
+{
+    if (check_ctx(c)) return LF_EINVAL;
+    LF_TRY(c, hipSetDevice(c->device));
+    LF_TRY(c, hipStreamSynchronize(c->stream));
+    for (auto& e : c->events) {
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, e.a, e.b) == hipSuccess && e.cls >= 0 && e.cls < 8) {
+            c->ms[e.cls] += t;
+            c->launches[e.cls] += 1;
+        }
+        hipEventDestroy(e.a);
+        hipEventDestroy(e.b);
+    }
+    c->events.clear();
+    for (int k = 0; k < n_classes && k < 8; ++k) {
+        if (ms) ms[k] = c->ms[k];
+        if (launches) launches[k] = c->launches[k];
+        c->ms[k] = 0.0;
+        c->launches[k] = 0;
+    }
+    return LF_OK;
+}
+
+int lf_stats(lf_ctx* c, int64_t* fp64_edges)
+{
+    if (check_ctx(c)) return LF_EINVAL;
+    LF_TRY(c, hipSetDevice(c->device));
+    unsigned long long v = 0;
+    LF_TRY(c, hipMemcpyAsync(&v, c->d_fb, sizeof v, hipMemcpyDeviceToHost, c->stream));
+    LF_TRY(c, hipStreamSynchronize(c->stream));
+    if (fp64_edges) *fp64_edges = (int64_t)v;
+    return LF_OK;
+}
+
+}  // extern "C"

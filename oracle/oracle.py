@@ -1,0 +1,176 @@
+"""ctypes bindings for the FP64 CPU oracle (liblithofwd_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the lithoinv3d_amd package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liblithofwd_oracle.so")
+_lib = None
+
+_D = ctypes.POINTER(ctypes.c_double)
+
+
+class _Model(ctypes.Structure):
+    _fields_ = [
+        ("n_groups", ctypes.c_int32),
+        ("n_vertices", ctypes.c_int64),
+        ("verts", _D),
+        ("tri", ctypes.POINTER(ctypes.c_uint32)),
+        ("group_tri_begin", ctypes.POINTER(ctypes.c_int64)),
+        ("density", _D),
+        ("mag_amp", _D),
+        ("mag_dir", _D),
+    ]
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        v3 = ctypes.POINTER(ctypes.c_double)
+        L.lfo_solid_angle.restype = ctypes.c_double
+        L.lfo_solid_angle.argtypes = [v3, v3, v3, v3]
+        L.lfo_line_integral.restype = ctypes.c_double
+        L.lfo_line_integral.argtypes = [v3, v3, v3]
+        L.lfo_face_nagy.restype = ctypes.c_double
+        L.lfo_face_nagy.argtypes = [v3, v3, v3, v3]
+        L.lfo_magnetic_facet.restype = ctypes.c_double
+        L.lfo_magnetic_facet.argtypes = [v3] * 6
+        L.lfo_magnetic_facet_textbook.restype = ctypes.c_double
+        L.lfo_magnetic_facet_textbook.argtypes = [v3] * 6
+        L.lfo_field_unit_vector.argtypes = [ctypes.c_double, ctypes.c_double, v3]
+        L.lfo_group_magnetization.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, v3,
+                                              ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                              v3, v3, v3]
+        MP = ctypes.POINTER(_Model)
+        i64 = ctypes.c_int64
+        L.lfo_gravity_forward.argtypes = [MP, i64, v3, v3, ctypes.c_int]
+        L.lfo_magnetic_forward.argtypes = [MP, v3, i64, v3, v3, ctypes.c_int, ctypes.c_int]
+        L.lfo_gravity_vertex_jacobian.argtypes = [MP, i64, v3, v3, ctypes.c_int]
+        L.lfo_magnetic_vertex_jacobian.argtypes = [MP, v3, i64, v3, v3, ctypes.c_int]
+        L.lfo_gravity_vertex_jtw.argtypes = [MP, i64, v3, v3, v3, ctypes.c_int]
+        L.lfo_magnetic_vertex_jtw.argtypes = [MP, v3, i64, v3, v3, v3, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_D)
+
+
+def _v(x):
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+def solid_angle(a, b, c, o):
+    a, b, c, o = map(_v, (a, b, c, o))
+    return lib().lfo_solid_angle(_p(a), _p(b), _p(c), _p(o))
+
+
+def line_integral(a, b, o):
+    a, b, o = map(_v, (a, b, o))
+    return lib().lfo_line_integral(_p(a), _p(b), _p(o))
+
+
+def face_nagy(o, a, b, c):
+    o, a, b, c = map(_v, (o, a, b, c))
+    return lib().lfo_face_nagy(_p(o), _p(a), _p(b), _p(c))
+
+
+def magnetic_facet(o, a, b, c, mdir, fdir, textbook=False):
+    args = list(map(_v, (o, a, b, c, mdir, fdir)))
+    f = lib().lfo_magnetic_facet_textbook if textbook else lib().lfo_magnetic_facet
+    return f(*[_p(x) for x in args])
+
+
+def field_unit_vector(inc, dec):
+    out = np.zeros(3)
+    lib().lfo_field_unit_vector(inc, dec, _p(out))
+    return out
+
+
+class OracleModel:
+    """Holds the reference layout (one closed mesh per group) for the C oracle."""
+
+    def __init__(self, vertices, group_tris, density, mag_amp=None, mag_dir=None):
+        self.vertices = _v(vertices).reshape(-1, 3)
+        tris = [np.ascontiguousarray(t, dtype=np.uint32).reshape(-1, 3) for t in group_tris]
+        self.n_groups = len(tris)
+        self.tri = np.ascontiguousarray(np.concatenate(tris, 0) if tris else np.zeros((0, 3)),
+                                        dtype=np.uint32)
+        self.offs = np.zeros(self.n_groups + 1, np.int64)
+        self.offs[1:] = np.cumsum([len(t) for t in tris])
+        self.density = _v(density).reshape(-1)
+        self.mag_amp = _v(mag_amp if mag_amp is not None else np.zeros(self.n_groups)).reshape(-1)
+        md = mag_dir if mag_dir is not None else np.tile([0.0, 0.0, 1.0], (self.n_groups, 1))
+        self.mag_dir = _v(md).reshape(-1, 3)
+        self._s = _Model(self.n_groups, len(self.vertices), _p(self.vertices),
+                         self.tri.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                         self.offs.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                         _p(self.density), _p(self.mag_amp), _p(self.mag_dir))
+
+    @classmethod
+    def from_litho(cls, m, field_nT=50000.0, fhat=None):
+        tris = [m.group_triangles(g) for g in range(m.n_units)]
+        if fhat is None:
+            fhat = np.array([0.0, 0.0, 1.0])
+        amp, mdir = m.magnetization(field_nT, fhat)
+        return cls(m.vertices, tris, m.density, amp, mdir)
+
+    def gravity(self, stations, threads=0):
+        st = _v(stations).reshape(-1, 3)
+        out = np.zeros(len(st))
+        lib().lfo_gravity_forward(ctypes.byref(self._s), len(st), _p(st), _p(out), threads)
+        return out
+
+    def magnetic(self, fhat, stations, textbook=False, threads=0):
+        st = _v(stations).reshape(-1, 3)
+        fh = _v(fhat)
+        out = np.zeros(len(st))
+        lib().lfo_magnetic_forward(ctypes.byref(self._s), _p(fh), len(st), _p(st), _p(out),
+                                   int(bool(textbook)), threads)
+        return out
+
+    def gravity_vertex_jacobian(self, stations, threads=0):
+        st = _v(stations).reshape(-1, 3)
+        J = np.zeros((len(st), len(self.vertices), 3))
+        lib().lfo_gravity_vertex_jacobian(ctypes.byref(self._s), len(st), _p(st), _p(J), threads)
+        return J
+
+    def magnetic_vertex_jacobian(self, fhat, stations, threads=0):
+        st = _v(stations).reshape(-1, 3)
+        fh = _v(fhat)
+        J = np.zeros((len(st), len(self.vertices), 3))
+        lib().lfo_magnetic_vertex_jacobian(ctypes.byref(self._s), _p(fh), len(st), _p(st), _p(J),
+                                           threads)
+        return J
+
+    def gravity_vertex_jtw(self, stations, w, threads=0):
+        st = _v(stations).reshape(-1, 3)
+        w = _v(w)
+        g = np.zeros((len(self.vertices), 3))
+        lib().lfo_gravity_vertex_jtw(ctypes.byref(self._s), len(st), _p(st), _p(w), _p(g), threads)
+        return g
+
+    def magnetic_vertex_jtw(self, fhat, stations, w, threads=0):
+        st = _v(stations).reshape(-1, 3)
+        fh = _v(fhat)
+        w = _v(w)
+        g = np.zeros((len(self.vertices), 3))
+        lib().lfo_magnetic_vertex_jtw(ctypes.byref(self._s), _p(fh), len(st), _p(st), _p(w),
+                                      _p(g), threads)
+        return g

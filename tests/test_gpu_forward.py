@@ -1,0 +1,122 @@
+"""GPU parity of the forward (gz, TMI) against the FP64 CPU oracle.
+
+Tolerances (written per test):
+  FP64 path  : |gz - oracle| <= 1e-4 mGal, |TMI - oracle| <= 1e-3 nT
+  FP32C path : |gz - oracle| <= 1e-3 mGal, |TMI - oracle| <= 1e-2 nT
+"""
+import numpy as np
+import pytest
+
+from lithoinv3d_amd import synth
+from lithoinv3d_amd.lithofwd import ForwardContext
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = (1e-4, 1e-3)
+FP32C_TOL = (1e-3, 1e-2)
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def oracle_forward(O, m, st, fhat, field=50000.0, textbook=False):
+    om = O.OracleModel.from_litho(m, field, fhat)
+    return om.gravity(st), om.magnetic(fhat, st, textbook=textbook)
+
+
+@pytest.fixture(scope="module")
+def c0(oracle):
+    c = synth.config0()
+    fh = synth.field_unit_vector(c["inc"], c["dec"])
+    gz, tm = oracle_forward(oracle, c["model"], c["stations"], fh)
+    return c, fh, gz, tm
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", FP64_TOL), ("fp32c", FP32C_TOL)])
+def test_config0_forward(c0, precision, tol):
+    c, fh, gz_o, tm_o = c0
+    say(f"config0 forward {precision}")
+    with ForwardContext(c["model"], c["stations"], fh, c["field"], precision=precision) as ctx:
+        gz, tm = ctx.forward()
+    eg = np.abs(gz - gz_o).max()
+    et = np.abs(tm - tm_o).max()
+    print(f"config0 {precision}: max|dgz|={eg:.3e} mGal max|dTMI|={et:.3e} nT "
+          f"(gz max {np.abs(gz_o).max():.3f}, TMI max {np.abs(tm_o).max():.3f})")
+    assert eg <= tol[0]
+    assert et <= tol[1]
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", FP64_TOL), ("fp32c", FP32C_TOL)])
+def test_config0_textbook(c0, oracle, precision, tol):
+    c, fh, gz_o, _ = c0
+    say(f"config0 textbook {precision}")
+    tm_o = oracle.OracleModel.from_litho(c["model"], c["field"], fh).magnetic(
+        fh, c["stations"], textbook=True)
+    with ForwardContext(c["model"], c["stations"], fh, c["field"], precision=precision,
+                        tmi_mode="textbook", want=("tmi",)) as ctx:
+        gz, tm = ctx.forward()
+    assert gz is None
+    assert np.abs(tm - tm_o).max() <= tol[1]
+
+
+def small_contact(seed=7, shape=(9, 8, 5), n_units=4):
+    rng = np.random.default_rng(seed)
+    labels = rng.integers(0, n_units, size=shape).astype(np.int32)
+    labels[rng.random(shape) < 0.1] = -1  # holes: faces against the outside inside the grid
+    dens = rng.uniform(-0.3, 0.5, n_units)
+    sus = np.exp(rng.uniform(np.log(1e-4), np.log(5e-2), n_units))
+    m = synth.label_grid_model(labels, (-112.5, -100.0, -31.0), (25.0, 25.0, 25.0), dens, sus)
+    st = synth.topo_stations(12, 300.0)
+    return m, st
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", FP64_TOL), ("fp32c", FP32C_TOL)])
+def test_contact_model_forward(oracle, precision, tol):
+    m, st = small_contact()
+    fh = synth.field_unit_vector(-60.0, 10.0)
+    gz_o, tm_o = oracle_forward(oracle, m, st, fh)
+    say(f"contact model forward {precision} faces={m.n_faces}")
+    with ForwardContext(m, st, fh, 50000.0, precision=precision) as ctx:
+        gz, tm = ctx.forward()
+    print(f"contact {precision}: max|dgz|={np.abs(gz - gz_o).max():.3e} "
+          f"max|dTMI|={np.abs(tm - tm_o).max():.3e}")
+    assert np.abs(gz - gz_o).max() <= tol[0]
+    assert np.abs(tm - tm_o).max() <= tol[1]
+
+
+def test_set_vertices_and_properties(c0, oracle):
+    c, fh, _, _ = c0
+    m = c["model"]
+    rng = np.random.default_rng(3)
+    v2 = m.vertices + rng.normal(0, 2.0, m.vertices.shape)
+    m2 = synth.LithoModel(v2, m.faces, m.front, m.back, np.array([0.45]), np.array([0.01]))
+    gz_o, tm_o = oracle_forward(oracle, m2, c["stations"], fh)
+    say("set_vertices / set_properties fp64")
+    with ForwardContext(m, c["stations"], fh, c["field"], precision="fp64") as ctx:
+        ctx.forward()
+        ctx.set_vertices(v2)
+        amp = 0.01 * c["field"] / 100.0
+        ctx.set_properties(np.array([0.45]), (amp * fh)[None, :])
+        gz, tm = ctx.forward()
+    assert np.abs(gz - gz_o).max() <= 1e-4
+    assert np.abs(tm - tm_o).max() <= 1e-3
+
+
+def test_empty_and_ragged(oracle):
+    """No faces -> zeros; a station count that is not a multiple of the block size."""
+    m, st = small_contact(seed=11, shape=(3, 3, 2))
+    fh = synth.field_unit_vector(-60.0, 0.0)
+    st = st[:37]
+    gz_o, tm_o = oracle_forward(oracle, m, st, fh)
+    say("ragged stations fp32c")
+    with ForwardContext(m, st, fh, 50000.0, precision="fp32c") as ctx:
+        gz, tm = ctx.forward()
+    assert np.abs(gz - gz_o).max() <= 1e-3 and np.abs(tm - tm_o).max() <= 1e-2
+    empty = synth.LithoModel(m.vertices, m.faces[:0], m.front[:0], m.back[:0], m.density,
+                             m.susceptibility)
+    say("no faces")
+    with ForwardContext(empty, st, fh, 50000.0, precision="fp64") as ctx:
+        gz, tm = ctx.forward()
+        g = ctx.adjoint(np.ones(len(st)), np.ones(len(st)))
+    assert np.all(gz == 0) and np.all(tm == 0) and np.all(g == 0)
