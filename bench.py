@@ -1,0 +1,250 @@
+#!/usr/bin/env python3
+"""Throughput of the FP32C polyhedral forward (gz + TMI) plus the matrix-free adjoint
+gradient (d/dx, d/dy, d/dz of every vertex) on one MI355X.
+
+One step = new vertex positions (facet setup) -> forward over every station x facet
+pair -> adjoint over every pair, on the contact model with 192x192x64 cells (about 2.2M
+facets, contact facets stored once) and 256x256 stations; inputs are resident in HBM
+before the timed region.  Prints one JSON line; with --gpus N > 1 it exits with a
+message (this build runs on one GPU).
+
+value = station x facet pairs per second for forward + adjoint
+      = n_stations * n_facets / (seconds per step)
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from lithoinv3d_amd import synth, workload  # noqa: E402
+from lithoinv3d_amd.lithofwd import ForwardContext  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline(model, st, fh, field, seconds):
+    """The oracle (FP64 restatement of the reference forward and its per-face FD
+    Jacobian contracted with weights), single-threaded like the reference, on a sample
+    of stations x every facet of the reference layout (contact facets twice)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # the checker; timed here as the CPU baseline only
+
+    om = O.OracleModel.from_litho(model, field, fh)
+    ref_faces = model.reference_face_count()
+    rng = np.random.default_rng(11)
+
+    def run(idx):
+        s = st[idx]
+        w = rng.normal(size=len(idx))
+        t0 = time.perf_counter()
+        om.gravity(s, threads=1)
+        om.magnetic(fh, s, threads=1)
+        t1 = time.perf_counter()
+        om.gravity_vertex_jtw(s, w, threads=1)
+        om.magnetic_vertex_jtw(fh, s, w, threads=1)
+        t2 = time.perf_counter()
+        return t1 - t0, t2 - t1
+
+    probe = run(np.array([len(st) // 2]))
+    per_station = sum(probe)
+    n = int(max(1, min(64, seconds // max(per_station, 1e-9))))
+    idx = rng.choice(len(st), n, replace=False)
+    tf, tg = run(idx)
+    pairs = n * ref_faces
+    return {
+        "value": pairs / (tf + tg),
+        "unit": "station x facet pairs/s (fwd+grad, reference facet layout)",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{n} stations x {ref_faces} facets (all units' meshes, contacts twice): "
+                  f"forward gz+TMI {tf:.2f} s, per-face FD Jacobian J^T w gz+TMI {tg:.2f} s; "
+                  f"host {cpu_model()}",
+        "forward_pairs_per_s": pairs / tf,
+        "grad_pairs_per_s": pairs / tg,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="contact192x192x64_st256",
+                    choices=sorted(workload.WORKLOADS))
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU design is facet-sharded with one all-reduce per forward, "
+              "not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    dev = torch.device("cuda:0")
+
+    t0 = time.time()
+    cfg = workload.build(args.workload)
+    m, st = cfg["model"], cfg["stations"]
+    fh = synth.field_unit_vector(cfg["inc"], cfg["dec"])
+    field = cfg["field"]
+    n_s, n_f = len(st), m.n_faces
+    pairs = n_s * n_f
+    log(f"workload {args.workload}: {m.n_vertices} vertices, {n_f} facets "
+        f"({m.reference_face_count()} in the reference layout), {n_s} stations, "
+        f"{pairs:.3e} pairs; generated in {time.time() - t0:.1f}s")
+
+    ctx = ForwardContext(m, st, fh, field, precision="fp32c")
+    rng = np.random.default_rng(7)
+    wg = torch.tensor(rng.normal(size=n_s), dtype=torch.float64, device=dev)
+    wt = torch.tensor(rng.normal(size=n_s), dtype=torch.float64, device=dev)
+    gz = torch.zeros(n_s, dtype=torch.float64, device=dev)
+    tm = torch.zeros(n_s, dtype=torch.float64, device=dev)
+    grad = torch.zeros(m.n_vertices * 3, dtype=torch.float64, device=dev)
+    verts = torch.tensor(m.vertices.reshape(-1), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    def step():
+        ctx.set_vertices_device(verts.data_ptr())
+        ctx.forward_device(gz.data_ptr(), tm.data_ptr())
+        ctx.adjoint_device(wg.data_ptr(), wt.data_ptr(), grad.data_ptr())
+
+    for i in range(args.warmup):
+        step()
+        ctx.sync()
+        log(f"warmup step {i + 1}/{args.warmup} done")
+    ctx.timing()  # reset
+    ctx.set_timing(True)
+    torch.cuda.synchronize()
+    ctx.sync()
+    t1 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+    ctx.sync()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    ctx.set_timing(False)
+    tim = ctx.timing()
+    sec = (t2 - t1) / args.steps
+    log(f"{args.steps} steps in {t2 - t1:.3f}s; kernels: " +
+        ", ".join(f"{k} {v[0] / max(v[1], 1):.2f} ms x{v[1]}" for k, v in tim.items() if v[1]))
+    fb = ctx.fp64_edges()
+
+    fwd_ms = tim["forward"][0] / max(tim["forward"][1], 1)
+    adj_ms = tim["adjoint_pairs"][0] / max(tim["adjoint_pairs"][1], 1)
+    kern = {
+        "forward_f32c": fwd_ms,
+        "adjoint_pairs_f32": adj_ms,
+    }
+    dom = max(kern, key=kern.get)
+    flops = workload.flop_eq(dom) * pairs
+    achieved = flops / (kern[dom] * 1e-3) / 1e12
+    per_kernel = {
+        k: {
+            "avg_ms": v,
+            "tflops": workload.flop_eq(k) * pairs / (v * 1e-3) / 1e12,
+            "arith_tflops": workload.OPS[k]["arith"] * pairs / (v * 1e-3) / 1e12,
+            "pairs_per_s": pairs / (v * 1e-3),
+        }
+        for k, v in kern.items()
+    }
+
+    acc = None
+    if not args.no_accuracy:
+        # FP32C against the FP64 GPU path (itself held to the oracle at 1e-4 mGal /
+        # 1e-3 nT by the parity tests) over every station
+        g32 = gz.cpu().numpy().copy()
+        t32 = tm.cpu().numpy().copy()
+        with ForwardContext(m, st, fh, field, precision="fp64") as c64:
+            ta = time.perf_counter()
+            g64, t64 = c64.forward()
+            tb = time.perf_counter()
+        acc = {
+            "vs": "FP64 GPU path, all stations",
+            "max_abs_dgz_mGal": float(np.abs(g32 - g64).max()),
+            "max_abs_dtmi_nT": float(np.abs(t32 - t64).max()),
+            "gz_range_mGal": [float(g64.min()), float(g64.max())],
+            "tmi_range_nT": [float(t64.min()), float(t64.max())],
+            "fp64_forward_s": tb - ta,
+            "fp32c_edges_on_fp64_branch": fb,
+        }
+        log(f"accuracy: {acc}")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("timing the CPU baseline")
+        cpu = cpu_baseline(m, st, fh, field, args.cpu_seconds)
+        log(f"cpu baseline: {cpu}")
+
+    out = {
+        "metric": "station x facet evals/s (fwd+grad)",
+        "value": pairs / sec,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded Gaussian-field label grid, voxel-face facets)",
+        "config": {
+            "workload": args.workload,
+            "n_stations": n_s,
+            "n_facets": n_f,
+            "n_facets_reference_layout": m.reference_face_count(),
+            "n_vertices": m.n_vertices,
+            "precision": "fp32c (FP32 facet frames, Kahan facet sums, FP64 near-edge branch)",
+            "step": "facet setup + forward gz,TMI + adjoint dPhi/dxyz",
+            "reference_layout_pairs_per_s": n_s * m.reference_face_count() / sec,
+        },
+        "roofline": {
+            "bound": "valu",
+            "kernel": dom,
+            "achieved": achieved,
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP32_PEAK_TFLOPS,
+            "traffic": None,
+            "flop_eq_per_pair": workload.flop_eq(dom),
+        },
+        "kernels": per_kernel,
+        "accuracy": acc,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

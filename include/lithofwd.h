@@ -108,6 +108,18 @@ int lf_adjoint(lf_ctx* ctx, const double* w_gz, const double* w_tmi, double* gra
 int lf_adjoint_device(lf_ctx* ctx, const double* d_w_gz, const double* d_w_tmi,
                       double* d_grad);
 
+/* Squared Jacobian column norms for parameters that are linear combinations of one
+ * vertex coordinate axis: parameter j moves coordinate `axis` of vertex
+ * dof_vertex[e] by dof_weight[e] per unit, for e in [dof_off[j], dof_off[j+1]).
+ *   out_gz[j]  = sum_i (d gz_i  / d p_j)^2,   out_tmi[j] = sum_i (d TMI_i / d p_j)^2
+ * i.e. the columns of the reference's Jacobian (gravity_forward.cpp:184-292,
+ * magnetic_forward.cpp:196-306) including control-point interpolation weights, without
+ * forming it.  Used as the exact targets of the learned L-BFGS preconditioner (the
+ * Gauss-Newton diagonal).  Either output may be NULL. */
+int lf_colnorm2(lf_ctx* ctx, int axis, int64_t n_dof, const int64_t* dof_off,
+                const int32_t* dof_vertex, const double* dof_weight, double* out_gz,
+                double* out_tmi);
+
 /* Waits for the context stream. */
 int lf_sync(lf_ctx* ctx);
 

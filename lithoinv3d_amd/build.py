@@ -31,8 +31,27 @@ HIP_FLAGS = [
 ]
 
 
+HOST = os.path.join(HERE, "host")
+HOST_LIB = os.path.join(HERE, "liblithohost.so")
+CXX = os.environ.get("CXX", "g++")
+
+
 def sources():
     return [os.path.join(CSRC, "lithofwd.hip")]
+
+
+def build_host(verbose: bool = False) -> str:
+    """The C++ host layer (ForwardModel / objectives / L-BFGS-B) over the C ABI,
+    linked against liblithofwd.so."""
+    srcs = [os.path.join(HOST, f) for f in ("litho_host.cpp", "litho_capi.cpp")]
+    cmd = [CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra",
+           "-Wno-unused-parameter"] + srcs + [
+        "-L" + HERE, "-llithofwd", "-Wl,-rpath,$ORIGIN", "-o", HOST_LIB + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd, cwd=HOST)
+    os.replace(HOST_LIB + ".tmp", HOST_LIB)
+    return HOST_LIB
 
 
 def build(verbose: bool = False, extra=None, out: str = LIB) -> str:
@@ -41,6 +60,8 @@ def build(verbose: bool = False, extra=None, out: str = LIB) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd, cwd=CSRC)
     os.replace(out + ".tmp", out)
+    if out == LIB:
+        build_host(verbose)
     return out
 
 

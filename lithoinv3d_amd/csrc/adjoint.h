@@ -95,6 +95,172 @@ __device__ __forceinline__ T edge_log2_grad(T s1, T s2, T d1, T d2, T rho2, T& g
     return fast_log2(num * fast_rcp(den));
 }
 
+// One (station, facet) pair: frame-form forward and its reverse sweep for the weights
+// (lam on gz, mu on TMI), adding the pair's 22 facet seeds into acc (see the layout in
+// SeedIdx).  Linear in (lam, mu).
+template <class T>
+__device__ __forceinline__ void pair_seeds(const FaceFrame<T>& F, double ox, double oy,
+                                           double oz, T lam, T mu, T* acc)
+{
+    const T Px = T(ox - F.ax), Py = T(oy - F.ay), Pz = T(oz - F.az);
+    const T px = Px * F.ux + Py * F.uy + Pz * F.uz;
+    const T py = Px * F.vx + Py * F.vy + Pz * F.vz;
+    const T pz = Px * F.nx + Py * F.ny + Pz * F.nz;
+    // ---- forward
+    const T pz2 = pz * pz;
+    const T rho0 = py * py + pz2;
+    const T da2 = px * px + rho0;
+    const T t = F.l0 - px;
+    const T db2 = t * t + rho0;
+    const T dcx = F.cu - px, dcy = F.cv - py;
+    const T dc2 = dcx * dcx + dcy * dcy + pz2;
+    const T da = fast_sqrt(da2), db = fast_sqrt(db2), dc = fast_sqrt(dc2);
+    const T rab = da2 - F.l0 * px;
+    const T rac = da2 - (F.cu * px + F.cv * py);
+    const T rbc = t * dcx - py * dcy + pz2;
+    const T D = da * db * dc + rab * dc + rac * db + rbc * da;
+    const T N = -F.twoA * pz;
+    const T th = atan2_t(N, D);
+    const T dp1 = t * F.e1v + py * F.e1u;
+    const T dp2 = dcx * F.e2v - dcy * F.e2u;
+    const T s1b = t * F.e1u - py * F.e1v;
+    const T s1c = dcx * F.e2u + dcy * F.e2v;
+    const T rho1 = dp1 * dp1 + pz2;
+    const T rho2 = dp2 * dp2 + pz2;
+    T g0s1, g0s2, g0d1, g0d2, g0r, g1s1, g1s2, g1d1, g1d2, g1r, g2s1, g2s2, g2d1, g2d2, g2r;
+    const T lam0 = edge_log2_grad(-px, t, da, db, rho0, g0s1, g0s2, g0d1, g0d2, g0r);
+    const T lam1 = edge_log2_grad(s1b, s1b + F.l1, db, dc, rho1, g1s1, g1s2, g1d1, g1d2, g1r);
+    const T lam2 = edge_log2_grad(s1c, s1c + F.l2, dc, da, rho2, g2s1, g2s2, g2d1, g2d2, g2r);
+    const T es = py * lam0 + dp1 * lam1 + dp2 * lam2;
+    // ---- coefficient seeds
+    acc[S_K1] += lam * es;
+    acc[S_K2] += lam * (pz * th);
+    acc[S_W0] += mu * th;
+    acc[S_W1] += mu * lam0;
+    acc[S_W2] += mu * lam1;
+    acc[S_W3] += mu * lam2;
+    // ---- reverse
+    const T ses = lam * F.K1;
+    const T sth = lam * F.K2 * pz + mu * F.W0;
+    T spz = lam * F.K2 * th;
+    const T sl0 = ses * py + mu * F.W1;
+    const T sl1 = ses * dp1 + mu * F.W2;
+    const T sl2 = ses * dp2 + mu * F.W3;
+    T spy = ses * lam0;
+    T sdp1 = ses * lam1;
+    T sdp2 = ses * lam2;
+    // edge 0: s1 = -px, s2 = t, d1 = da, d2 = db
+    T spx = -sl0 * g0s1;
+    T st_ = sl0 * g0s2;
+    T sda = sl0 * g0d1;
+    T sdb = sl0 * g0d2;
+    T srho0 = sl0 * g0r;
+    // edge 1: s1 = s1b, s2 = s1b + l1, d1 = db, d2 = dc
+    const T ss1b = sl1 * (g1s1 + g1s2);
+    const T sL1 = sl1 * g1s2;
+    sdb += sl1 * g1d1;
+    T sdc = sl1 * g1d2;
+    const T srho1 = sl1 * g1r;
+    // edge 2: s1 = s1c, s2 = s1c + l2, d1 = dc, d2 = da
+    const T ss1c = sl2 * (g2s1 + g2s2);
+    const T sL2 = sl2 * g2s2;
+    sdc += sl2 * g2d1;
+    sda += sl2 * g2d2;
+    const T srho2 = sl2 * g2r;
+    // rho1, rho2
+    sdp1 += T(2) * dp1 * srho1;
+    sdp2 += T(2) * dp2 * srho2;
+    T spz2 = srho1 + srho2;
+    // s1b = t e1u - py e1v
+    st_ += ss1b * F.e1u;
+    spy -= ss1b * F.e1v;
+    T se1u = ss1b * t;
+    T se1v = -ss1b * py;
+    // s1c = dcx e2u + dcy e2v
+    T sdcx = ss1c * F.e2u;
+    T sdcy = ss1c * F.e2v;
+    T se2u = ss1c * dcx;
+    T se2v = ss1c * dcy;
+    // dp1 = t e1v + py e1u
+    st_ += sdp1 * F.e1v;
+    spy += sdp1 * F.e1u;
+    se1v += sdp1 * t;
+    se1u += sdp1 * py;
+    // dp2 = dcx e2v - dcy e2u
+    sdcx += sdp2 * F.e2v;
+    sdcy -= sdp2 * F.e2u;
+    se2v += sdp2 * dcx;
+    se2u -= sdp2 * dcy;
+    // th = atan2(N, D)
+    const T q = fast_rcp(N * N + D * D);
+    const T sN = sth * D * q;
+    const T sD = -sth * N * q;
+    const T stwoA = -sN * pz;
+    spz -= sN * F.twoA;
+    // D = da db dc + rab dc + rac db + rbc da
+    sda += sD * (db * dc + rbc);
+    sdb += sD * (da * dc + rac);
+    sdc += sD * (da * db + rab);
+    const T srab = sD * dc;
+    const T srac = sD * db;
+    const T srbc = sD * da;
+    // rbc = t dcx - py dcy + pz2
+    st_ += srbc * dcx;
+    sdcx += srbc * t;
+    spy -= srbc * dcy;
+    sdcy -= srbc * py;
+    spz2 += srbc;
+    // rac = da2 - cu px - cv py ; rab = da2 - l0 px
+    T sda2 = srac + srab;
+    T scu = -srac * px;
+    T scv = -srac * py;
+    spx -= srac * F.cu + srab * F.l0;
+    spy -= srac * F.cv;
+    T sl0c = -srab * px;
+    // square roots
+    sda2 += T(0.5) * sda * fast_rcp(da);
+    const T sdb2 = T(0.5) * sdb * fast_rcp(db);
+    const T sdc2 = T(0.5) * sdc * fast_rcp(dc);
+    // dc2 = dcx^2 + dcy^2 + pz2
+    sdcx += T(2) * dcx * sdc2;
+    sdcy += T(2) * dcy * sdc2;
+    spz2 += sdc2;
+    // dcx = cu - px, dcy = cv - py
+    scu += sdcx;
+    spx -= sdcx;
+    scv += sdcy;
+    spy -= sdcy;
+    // db2 = t^2 + rho0
+    st_ += T(2) * t * sdb2;
+    srho0 += sdb2;
+    // t = l0 - px
+    sl0c += st_;
+    spx -= st_;
+    // da2 = px^2 + rho0 ; rho0 = py^2 + pz2 ; pz2 = pz^2
+    spx += T(2) * px * sda2;
+    srho0 += sda2;
+    spy += T(2) * py * srho0;
+    spz2 += srho0;
+    spz += T(2) * pz * spz2;
+    // ---- accumulate
+    acc[S_GPX] += spx;
+    acc[S_GPY] += spy;
+    acc[S_GPZ] += spz;
+    acc[S_AXY] += spx * py - spy * px;
+    acc[S_AXZ] += spx * pz - spz * px;
+    acc[S_AYZ] += spy * pz - spz * py;
+    acc[S_L0] += sl0c;
+    acc[S_CU] += scu;
+    acc[S_CV] += scv;
+    acc[S_E1U] += se1u;
+    acc[S_E1V] += se1v;
+    acc[S_L1] += sL1;
+    acc[S_E2U] += se2u;
+    acc[S_E2V] += se2v;
+    acc[S_L2] += sL2;
+    acc[S_TWOA] += stwoA;
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void k_adjoint_pairs(const FaceFrame<T>* __restrict__ faces,
                                                        int64_t n_f,
@@ -114,164 +280,7 @@ __global__ __launch_bounds__(256) void k_adjoint_pairs(const FaceFrame<T>* __res
     for (int64_t i = i0; i < i1; ++i) {
         const StationW S = st[i];
         if (S.wg == 0.0 && S.wt == 0.0) continue;
-        const T lam = T(S.wg), mu = T(S.wt);
-        const T Px = T(S.x - F.ax), Py = T(S.y - F.ay), Pz = T(S.z - F.az);
-        const T px = Px * F.ux + Py * F.uy + Pz * F.uz;
-        const T py = Px * F.vx + Py * F.vy + Pz * F.vz;
-        const T pz = Px * F.nx + Py * F.ny + Pz * F.nz;
-        // ---- forward
-        const T pz2 = pz * pz;
-        const T rho0 = py * py + pz2;
-        const T da2 = px * px + rho0;
-        const T t = F.l0 - px;
-        const T db2 = t * t + rho0;
-        const T dcx = F.cu - px, dcy = F.cv - py;
-        const T dc2 = dcx * dcx + dcy * dcy + pz2;
-        const T da = fast_sqrt(da2), db = fast_sqrt(db2), dc = fast_sqrt(dc2);
-        const T rab = da2 - F.l0 * px;
-        const T rac = da2 - (F.cu * px + F.cv * py);
-        const T rbc = t * dcx - py * dcy + pz2;
-        const T D = da * db * dc + rab * dc + rac * db + rbc * da;
-        const T N = -F.twoA * pz;
-        const T th = atan2_t(N, D);
-        const T dp1 = t * F.e1v + py * F.e1u;
-        const T dp2 = dcx * F.e2v - dcy * F.e2u;
-        const T s1b = t * F.e1u - py * F.e1v;
-        const T s1c = dcx * F.e2u + dcy * F.e2v;
-        const T rho1 = dp1 * dp1 + pz2;
-        const T rho2 = dp2 * dp2 + pz2;
-        T g0s1, g0s2, g0d1, g0d2, g0r, g1s1, g1s2, g1d1, g1d2, g1r, g2s1, g2s2, g2d1, g2d2, g2r;
-        const T lam0 = edge_log2_grad(-px, t, da, db, rho0, g0s1, g0s2, g0d1, g0d2, g0r);
-        const T lam1 = edge_log2_grad(s1b, s1b + F.l1, db, dc, rho1, g1s1, g1s2, g1d1, g1d2, g1r);
-        const T lam2 = edge_log2_grad(s1c, s1c + F.l2, dc, da, rho2, g2s1, g2s2, g2d1, g2d2, g2r);
-        const T es = py * lam0 + dp1 * lam1 + dp2 * lam2;
-        // ---- coefficient seeds
-        acc[S_K1] += lam * es;
-        acc[S_K2] += lam * (pz * th);
-        acc[S_W0] += mu * th;
-        acc[S_W1] += mu * lam0;
-        acc[S_W2] += mu * lam1;
-        acc[S_W3] += mu * lam2;
-        // ---- reverse
-        const T ses = lam * F.K1;
-        const T sth = lam * F.K2 * pz + mu * F.W0;
-        T spz = lam * F.K2 * th;
-        const T sl0 = ses * py + mu * F.W1;
-        const T sl1 = ses * dp1 + mu * F.W2;
-        const T sl2 = ses * dp2 + mu * F.W3;
-        T spy = ses * lam0;
-        T sdp1 = ses * lam1;
-        T sdp2 = ses * lam2;
-        // edge 0: s1 = -px, s2 = t, d1 = da, d2 = db
-        T spx = -sl0 * g0s1;
-        T st_ = sl0 * g0s2;
-        T sda = sl0 * g0d1;
-        T sdb = sl0 * g0d2;
-        T srho0 = sl0 * g0r;
-        // edge 1: s1 = s1b, s2 = s1b + l1, d1 = db, d2 = dc
-        const T ss1b = sl1 * (g1s1 + g1s2);
-        const T sL1 = sl1 * g1s2;
-        sdb += sl1 * g1d1;
-        T sdc = sl1 * g1d2;
-        const T srho1 = sl1 * g1r;
-        // edge 2: s1 = s1c, s2 = s1c + l2, d1 = dc, d2 = da
-        const T ss1c = sl2 * (g2s1 + g2s2);
-        const T sL2 = sl2 * g2s2;
-        sdc += sl2 * g2d1;
-        sda += sl2 * g2d2;
-        const T srho2 = sl2 * g2r;
-        // rho1, rho2
-        sdp1 += T(2) * dp1 * srho1;
-        sdp2 += T(2) * dp2 * srho2;
-        T spz2 = srho1 + srho2;
-        // s1b = t e1u - py e1v
-        st_ += ss1b * F.e1u;
-        spy -= ss1b * F.e1v;
-        T se1u = ss1b * t;
-        T se1v = -ss1b * py;
-        // s1c = dcx e2u + dcy e2v
-        T sdcx = ss1c * F.e2u;
-        T sdcy = ss1c * F.e2v;
-        T se2u = ss1c * dcx;
-        T se2v = ss1c * dcy;
-        // dp1 = t e1v + py e1u
-        st_ += sdp1 * F.e1v;
-        spy += sdp1 * F.e1u;
-        se1v += sdp1 * t;
-        se1u += sdp1 * py;
-        // dp2 = dcx e2v - dcy e2u
-        sdcx += sdp2 * F.e2v;
-        sdcy -= sdp2 * F.e2u;
-        se2v += sdp2 * dcx;
-        se2u -= sdp2 * dcy;
-        // th = atan2(N, D)
-        const T q = fast_rcp(N * N + D * D);
-        const T sN = sth * D * q;
-        const T sD = -sth * N * q;
-        const T stwoA = -sN * pz;
-        spz -= sN * F.twoA;
-        // D = da db dc + rab dc + rac db + rbc da
-        sda += sD * (db * dc + rbc);
-        sdb += sD * (da * dc + rac);
-        sdc += sD * (da * db + rab);
-        const T srab = sD * dc;
-        const T srac = sD * db;
-        const T srbc = sD * da;
-        // rbc = t dcx - py dcy + pz2
-        st_ += srbc * dcx;
-        sdcx += srbc * t;
-        spy -= srbc * dcy;
-        sdcy -= srbc * py;
-        spz2 += srbc;
-        // rac = da2 - cu px - cv py ; rab = da2 - l0 px
-        T sda2 = srac + srab;
-        T scu = -srac * px;
-        T scv = -srac * py;
-        spx -= srac * F.cu + srab * F.l0;
-        spy -= srac * F.cv;
-        T sl0c = -srab * px;
-        // square roots
-        sda2 += T(0.5) * sda * fast_rcp(da);
-        const T sdb2 = T(0.5) * sdb * fast_rcp(db);
-        const T sdc2 = T(0.5) * sdc * fast_rcp(dc);
-        // dc2 = dcx^2 + dcy^2 + pz2
-        sdcx += T(2) * dcx * sdc2;
-        sdcy += T(2) * dcy * sdc2;
-        spz2 += sdc2;
-        // dcx = cu - px, dcy = cv - py
-        scu += sdcx;
-        spx -= sdcx;
-        scv += sdcy;
-        spy -= sdcy;
-        // db2 = t^2 + rho0
-        st_ += T(2) * t * sdb2;
-        srho0 += sdb2;
-        // t = l0 - px
-        sl0c += st_;
-        spx -= st_;
-        // da2 = px^2 + rho0 ; rho0 = py^2 + pz2 ; pz2 = pz^2
-        spx += T(2) * px * sda2;
-        srho0 += sda2;
-        spy += T(2) * py * srho0;
-        spz2 += srho0;
-        spz += T(2) * pz * spz2;
-        // ---- accumulate
-        acc[S_GPX] += spx;
-        acc[S_GPY] += spy;
-        acc[S_GPZ] += spz;
-        acc[S_AXY] += spx * py - spy * px;
-        acc[S_AXZ] += spx * pz - spz * px;
-        acc[S_AYZ] += spy * pz - spz * py;
-        acc[S_L0] += sl0c;
-        acc[S_CU] += scu;
-        acc[S_CV] += scv;
-        acc[S_E1U] += se1u;
-        acc[S_E1V] += se1v;
-        acc[S_L1] += sL1;
-        acc[S_E2U] += se2u;
-        acc[S_E2V] += se2v;
-        acc[S_L2] += sL2;
-        acc[S_TWOA] += stwoA;
+        pair_seeds(F, S.x, S.y, S.z, T(S.wg), T(S.wt), acc);
     }
     if (active) {
 #pragma unroll
@@ -280,8 +289,83 @@ __global__ __launch_bounds__(256) void k_adjoint_pairs(const FaceFrame<T>* __res
     }
 }
 
-// Per facet: sum the chunk seeds in FP64, then push them through the facet frame and
-// constants along the nine vertex-coordinate directions.
+// Per-facet derivative matrix T (22 x 9): the derivative of the facet's contribution
+// with respect to vertex coordinate k (a.x, a.y, a.z, b.x, ..., c.z) is
+// sum_m seed_m * T[m][k] for the per-pair (or accumulated) seeds of SeedIdx.  Built
+// from the facet constants with forward-mode duals along the nine directions.
+// Returns false for a degenerate facet (T = 0).
+__device__ __forceinline__ bool facet_T(const double P[3][3], double drho, const double Md[3],
+                                        const double Ms[3], const double Fd[3], bool textbook,
+                                        double Tm[kSeeds][9])
+{
+    FrameVals<double> fv;
+    facet_frame(P, drho, Md, Ms, Fd, textbook, fv);
+    for (int m = 0; m < kSeeds; ++m)
+        for (int k = 0; k < 9; ++k) Tm[m][k] = 0.0;
+    if (fv.bad) return false;
+    for (int k = 0; k < 9; ++k) {
+        Dual Q[3][3];
+        for (int c = 0; c < 3; ++c)
+            for (int j = 0; j < 3; ++j) Q[c][j] = Dual(P[c][j], (c * 3 + j == k) ? 1.0 : 0.0);
+        FrameVals<Dual> fd;
+        facet_frame(Q, drho, Md, Ms, Fd, textbook, fd);
+        double wxy = 0, wxz = 0, wyz = 0;
+        for (int j = 0; j < 3; ++j) {
+            wxy += fd.u[j].d * fv.v[j];
+            wxz += fd.u[j].d * fv.n[j];
+            wyz += fd.v[j].d * fv.n[j];
+        }
+        if (k < 3) {
+            // p = R (o - a): moving a moves p by -R da
+            Tm[S_GPX][k] = -fv.u[k];
+            Tm[S_GPY][k] = -fv.v[k];
+            Tm[S_GPZ][k] = -fv.n[k];
+        }
+        Tm[S_AXY][k] = wxy;
+        Tm[S_AXZ][k] = wxz;
+        Tm[S_AYZ][k] = wyz;
+        Tm[S_L0][k] = fd.l0.d;
+        Tm[S_CU][k] = fd.cu.d;
+        Tm[S_CV][k] = fd.cv.d;
+        Tm[S_E1U][k] = fd.e1u.d;
+        Tm[S_E1V][k] = fd.e1v.d;
+        Tm[S_L1][k] = fd.l1.d;
+        Tm[S_E2U][k] = fd.e2u.d;
+        Tm[S_E2V][k] = fd.e2v.d;
+        Tm[S_L2][k] = fd.l2.d;
+        Tm[S_TWOA][k] = fd.twoA.d;
+        Tm[S_K1][k] = fd.K1.d;
+        Tm[S_K2][k] = fd.K2.d;
+        Tm[S_W0][k] = fd.W0.d;
+        Tm[S_W1][k] = fd.W1.d;
+        Tm[S_W2][k] = fd.W2.d;
+        Tm[S_W3][k] = fd.W3.d;
+    }
+    return true;
+}
+
+__device__ __forceinline__ void load_facet(int64_t f, const double* __restrict__ verts,
+                                           const uint32_t* __restrict__ tri,
+                                           const int32_t* __restrict__ front,
+                                           const int32_t* __restrict__ back,
+                                           const double* __restrict__ density,
+                                           const double* __restrict__ mag, double P[3][3],
+                                           double& drho, double Md[3], double Ms[3])
+{
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t vi = tri[3 * f + k];
+        for (int j = 0; j < 3; ++j) P[k][j] = verts[3 * (int64_t)vi + j];
+    }
+    const int uf = front[f], ub = back[f];
+    drho = density[uf] - (ub >= 0 ? density[ub] : 0.0);
+    for (int j = 0; j < 3; ++j) {
+        const double mf = mag[3 * uf + j], mb = ub >= 0 ? mag[3 * ub + j] : 0.0;
+        Md[j] = mf - mb;
+        Ms[j] = mf + mb;
+    }
+}
+
+// Per facet: sum the chunk seeds in FP64, then apply the facet's derivative matrix.
 template <class T>
 __global__ __launch_bounds__(128) void k_adjoint_finalize(
     const T* __restrict__ seeds, int n_chunks, int64_t n_f, const double* __restrict__ verts,
@@ -298,53 +382,107 @@ __global__ __launch_bounds__(128) void k_adjoint_finalize(
         for (int c = 0; c < n_chunks; ++c) a += (double)seeds[((int64_t)c * kSeeds + k) * n_f + f];
         s[k] = a;
     }
-    double P[3][3];
-    for (int k = 0; k < 3; ++k) {
-        const uint32_t vi = tri[3 * f + k];
-        for (int j = 0; j < 3; ++j) P[k][j] = verts[3 * (int64_t)vi + j];
-    }
-    const int uf = front[f], ub = back[f];
-    const double drho = density[uf] - (ub >= 0 ? density[ub] : 0.0);
-    double Md[3], Ms[3];
-    for (int j = 0; j < 3; ++j) {
-        const double mf = mag[3 * uf + j], mb = ub >= 0 ? mag[3 * ub + j] : 0.0;
-        Md[j] = mf - mb;
-        Ms[j] = mf + mb;
-    }
+    double P[3][3], drho, Md[3], Ms[3];
+    load_facet(f, verts, tri, front, back, density, mag, P, drho, Md, Ms);
     const double Fd[3] = {fx, fy, fz};
-    const bool textbook = tmi_mode == 1;
-    FrameVals<double> fv;
-    facet_frame(P, drho, Md, Ms, Fd, textbook, fv);
+    double Tm[kSeeds][9];
+    facet_T(P, drho, Md, Ms, Fd, tmi_mode == 1, Tm);
     double* out = face_grad + 9 * f;
-    if (fv.bad) {
-        for (int k = 0; k < 9; ++k) out[k] = 0.0;
-        return;
-    }
-    double RtG[3];
-    for (int j = 0; j < 3; ++j)
-        RtG[j] = fv.u[j] * s[S_GPX] + fv.v[j] * s[S_GPY] + fv.n[j] * s[S_GPZ];
     for (int k = 0; k < 9; ++k) {
-        Dual Q[3][3];
-        for (int c = 0; c < 3; ++c)
-            for (int j = 0; j < 3; ++j) Q[c][j] = Dual(P[c][j], (c * 3 + j == k) ? 1.0 : 0.0);
-        FrameVals<Dual> fd;
-        facet_frame(Q, drho, Md, Ms, Fd, textbook, fd);
-        double wxy = 0, wxz = 0, wyz = 0;
-        for (int j = 0; j < 3; ++j) {
-            wxy += fd.u[j].d * fv.v[j];
-            wxz += fd.u[j].d * fv.n[j];
-            wyz += fd.v[j].d * fv.n[j];
-        }
-        double g = wxy * s[S_AXY] + wxz * s[S_AXZ] + wyz * s[S_AYZ];
-        g += s[S_L0] * fd.l0.d + s[S_CU] * fd.cu.d + s[S_CV] * fd.cv.d;
-        g += s[S_E1U] * fd.e1u.d + s[S_E1V] * fd.e1v.d + s[S_L1] * fd.l1.d;
-        g += s[S_E2U] * fd.e2u.d + s[S_E2V] * fd.e2v.d + s[S_L2] * fd.l2.d;
-        g += s[S_TWOA] * fd.twoA.d;
-        g += s[S_K1] * fd.K1.d + s[S_K2] * fd.K2.d;
-        g += s[S_W0] * fd.W0.d + s[S_W1] * fd.W1.d + s[S_W2] * fd.W2.d + s[S_W3] * fd.W3.d;
-        if (k < 3) g -= RtG[k];
+        double g = 0.0;
+        for (int m = 0; m < kSeeds; ++m) g += s[m] * Tm[m][k];
         out[k] = g;
     }
+}
+
+// ------------------------------------------------- Jacobian column norms (per DOF)
+
+// T restricted to one axis: Tz[f][m][corner] = T[m][3*corner + axis].
+template <class T>
+__global__ __launch_bounds__(128) void k_facet_T_axis(
+    int64_t n_f, const double* __restrict__ verts, const uint32_t* __restrict__ tri,
+    const int32_t* __restrict__ front, const int32_t* __restrict__ back,
+    const double* __restrict__ density, const double* __restrict__ mag, double fx, double fy,
+    double fz, int tmi_mode, int axis, T* __restrict__ Tax)
+{
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n_f) return;
+    double P[3][3], drho, Md[3], Ms[3];
+    load_facet(f, verts, tri, front, back, density, mag, P, drho, Md, Ms);
+    const double Fd[3] = {fx, fy, fz};
+    double Tm[kSeeds][9];
+    facet_T(P, drho, Md, Ms, Fd, tmi_mode == 1, Tm);
+    T* out = Tax + (int64_t)f * kSeeds * 3;
+    for (int m = 0; m < kSeeds; ++m)
+        for (int c = 0; c < 3; ++c) out[m * 3 + c] = (T)Tm[m][3 * c + axis];
+}
+
+// lane = facet; for each station of the block (wave-uniform) the per-pair seeds for a
+// unit weight on gz and on TMI, contracted with T: the derivative of gz_s and TMI_s with
+// respect to the chosen coordinate of each facet corner, G[s][3 f + corner].
+template <class T>
+__global__ __launch_bounds__(256) void k_station_facet_grad(
+    const FaceFrame<T>* __restrict__ faces, const T* __restrict__ Tax, int64_t n_f,
+    const StationW* __restrict__ st, int n_blk, float* __restrict__ Gg, float* __restrict__ Gt)
+{
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n_f) return;
+    const FaceFrame<T> F = faces[f];
+    const T* tf = Tax + (int64_t)f * kSeeds * 3;
+    for (int s = 0; s < n_blk; ++s) {
+        const StationW S = st[s];
+        T ag[kSeeds], at[kSeeds];
+#pragma unroll
+        for (int k = 0; k < kSeeds; ++k) ag[k] = at[k] = T(0);
+        pair_seeds(F, S.x, S.y, S.z, T(1), T(0), ag);
+        pair_seeds(F, S.x, S.y, S.z, T(0), T(1), at);
+        for (int c = 0; c < 3; ++c) {
+            T dg = 0, dt = 0;
+#pragma unroll
+            for (int m = 0; m < kSeeds; ++m) {
+                dg += ag[m] * tf[m * 3 + c];
+                dt += at[m] * tf[m * 3 + c];
+            }
+            Gg[(int64_t)s * 3 * n_f + 3 * f + c] = (float)dg;
+            Gt[(int64_t)s * 3 * n_f + 3 * f + c] = (float)dt;
+        }
+    }
+}
+
+// D[s][v] = sum of G[s][corner entries of v]
+__global__ __launch_bounds__(256) void k_gather_block(const float* __restrict__ G,
+                                                      const int64_t* __restrict__ off,
+                                                      const int32_t* __restrict__ ent,
+                                                      int64_t n_v, int64_t n_f,
+                                                      float* __restrict__ D)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (v >= n_v) return;
+    const float* g = G + (int64_t)s * 3 * n_f;
+    float acc = 0.f;
+    for (int64_t e = off[v]; e < off[v + 1]; ++e) acc += g[ent[e]];
+    D[(int64_t)s * n_v + v] = acc;
+}
+
+// out[j] += sum_s (sum_e w_e D[s][vertex_e])^2 over the entries of DOF j
+__global__ __launch_bounds__(256) void k_dof_colnorm(const float* __restrict__ D, int n_blk,
+                                                     int64_t n_v, int64_t n_dof,
+                                                     const int64_t* __restrict__ off,
+                                                     const int32_t* __restrict__ vert,
+                                                     const double* __restrict__ w,
+                                                     double* __restrict__ out)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_dof) return;
+    double acc = 0.0;
+    for (int s = 0; s < n_blk; ++s) {
+        const float* d = D + (int64_t)s * n_v;
+        double val = 0.0;
+        for (int64_t e = off[j]; e < off[j + 1]; ++e) val += w[e] * (double)d[vert[e]];
+        acc += val * val;
+    }
+    out[j] += acc;
 }
 
 __global__ __launch_bounds__(256) void k_vertex_gather(const double* __restrict__ face_grad,

@@ -723,6 +723,108 @@ int lf_adjoint(lf_ctx* c, const double* w_gz, const double* w_tmi, double* grad)
     return LF_OK;
 }
 
+int lf_colnorm2(lf_ctx* c, int axis, int64_t n_dof, const int64_t* dof_off,
+                const int32_t* dof_vertex, const double* dof_weight, double* out_gz,
+                double* out_tmi)
+{
+    if (check_ctx(c)) return LF_EINVAL;
+    if (axis < 0 || axis > 2 || n_dof < 0 || (n_dof > 0 && (!dof_off || !dof_vertex || !dof_weight)))
+        return set_err(c, "lf_colnorm2: invalid arguments", LF_EINVAL);
+    if (n_dof > 0 && dof_off[0] != 0) return set_err(c, "lf_colnorm2: dof_off[0] != 0", LF_EINVAL);
+    for (int64_t j = 0; j < n_dof; ++j)
+        if (dof_off[j + 1] < dof_off[j]) return set_err(c, "lf_colnorm2: dof_off not monotone", LF_EINVAL);
+    const int64_t n_ent = n_dof > 0 ? dof_off[n_dof] : 0;
+    for (int64_t e = 0; e < n_ent; ++e)
+        if (dof_vertex[e] < 0 || dof_vertex[e] >= c->n_v)
+            return set_err(c, "lf_colnorm2: DOF entry references a vertex out of range", LF_EINVAL);
+    LF_TRY(c, hipSetDevice(c->device));
+    int rc = ensure_faces(c);
+    if (rc) return rc;
+    const int kBlk = 64;
+    const size_t tsz = c->precision == LF_FP32C ? sizeof(float) : sizeof(double);
+    void* dT = nullptr;
+    StationW* dst = nullptr;
+    float *Gg = nullptr, *Gt = nullptr, *Dg = nullptr, *Dt = nullptr;
+    int64_t* doff = nullptr;
+    int32_t* dvert = nullptr;
+    double *dw = nullptr, *og = nullptr, *ot = nullptr;
+    auto cleanup = [&] {
+        void* b[] = {dT, dst, Gg, Gt, Dg, Dt, doff, dvert, dw, og, ot};
+        for (void* p : b)
+            if (p) hipFree(p);
+    };
+    auto fail = [&](hipError_t e, const char* what) {
+        cleanup();
+        return set_err(c, std::string("lf_colnorm2 ") + what + ": " + hipGetErrorString(e), LF_EHIP);
+    };
+    hipError_t e;
+#define CN(expr, what) do { e = (expr); if (e != hipSuccess) return fail(e, what); } while (0)
+    CN(hipMalloc(&dT, tsz * kSeeds * 3 * std::max<int64_t>(c->n_f, 1)), "alloc");
+    CN(hipMalloc((void**)&dst, sizeof(StationW) * kBlk), "alloc");
+    CN(hipMalloc((void**)&Gg, sizeof(float) * kBlk * 3 * std::max<int64_t>(c->n_f, 1)), "alloc");
+    CN(hipMalloc((void**)&Gt, sizeof(float) * kBlk * 3 * std::max<int64_t>(c->n_f, 1)), "alloc");
+    CN(hipMalloc((void**)&Dg, sizeof(float) * kBlk * std::max<int64_t>(c->n_v, 1)), "alloc");
+    CN(hipMalloc((void**)&Dt, sizeof(float) * kBlk * std::max<int64_t>(c->n_v, 1)), "alloc");
+    CN(hipMalloc((void**)&doff, sizeof(int64_t) * (n_dof + 1)), "alloc");
+    CN(hipMalloc((void**)&dvert, sizeof(int32_t) * std::max<int64_t>(n_ent, 1)), "alloc");
+    CN(hipMalloc((void**)&dw, sizeof(double) * std::max<int64_t>(n_ent, 1)), "alloc");
+    CN(hipMalloc((void**)&og, sizeof(double) * std::max<int64_t>(n_dof, 1)), "alloc");
+    CN(hipMalloc((void**)&ot, sizeof(double) * std::max<int64_t>(n_dof, 1)), "alloc");
+    if (n_dof > 0) {
+        CN(hipMemcpyAsync(doff, dof_off, sizeof(int64_t) * (n_dof + 1), hipMemcpyHostToDevice, c->stream), "copy");
+        if (n_ent) {
+            CN(hipMemcpyAsync(dvert, dof_vertex, sizeof(int32_t) * n_ent, hipMemcpyHostToDevice, c->stream), "copy");
+            CN(hipMemcpyAsync(dw, dof_weight, sizeof(double) * n_ent, hipMemcpyHostToDevice, c->stream), "copy");
+        }
+        const unsigned jb = (unsigned)((n_dof + 255) / 256);
+        hipLaunchKernelGGL(k_fill_zero, dim3(jb), dim3(256), 0, c->stream, og, n_dof);
+        hipLaunchKernelGGL(k_fill_zero, dim3(jb), dim3(256), 0, c->stream, ot, n_dof);
+        if (c->n_f > 0 && c->n_s > 0) {
+            const unsigned fb = (unsigned)((c->n_f + 127) / 128);
+            if (c->precision == LF_FP32C)
+                hipLaunchKernelGGL(k_facet_T_axis<float>, dim3(fb), dim3(128), 0, c->stream, c->n_f,
+                                   c->d_verts, c->d_tri, c->d_front, c->d_back, c->d_density,
+                                   c->d_mag, c->fdir[0], c->fdir[1], c->fdir[2], c->tmi_mode, axis,
+                                   (float*)dT);
+            else
+                hipLaunchKernelGGL(k_facet_T_axis<double>, dim3(fb), dim3(128), 0, c->stream, c->n_f,
+                                   c->d_verts, c->d_tri, c->d_front, c->d_back, c->d_density,
+                                   c->d_mag, c->fdir[0], c->fdir[1], c->fdir[2], c->tmi_mode, axis,
+                                   (double*)dT);
+            CN(hipGetLastError(), "facet T");
+            for (int64_t b0 = 0; b0 < c->n_s; b0 += kBlk) {
+                const int nb = (int)std::min<int64_t>(kBlk, c->n_s - b0);
+                hipLaunchKernelGGL(k_pack_stations, dim3(1), dim3(256), 0, c->stream, c->d_sx + b0,
+                                   c->d_sy + b0, c->d_sz + b0, c->adj.d_zero, c->adj.d_zero,
+                                   (int64_t)nb, dst);
+                const unsigned gb = (unsigned)((c->n_f + 255) / 256);
+                if (c->precision == LF_FP32C)
+                    hipLaunchKernelGGL(k_station_facet_grad<float>, dim3(gb), dim3(256), 0, c->stream,
+                                       c->d_f32, (const float*)dT, c->n_f, dst, nb, Gg, Gt);
+                else
+                    hipLaunchKernelGGL(k_station_facet_grad<double>, dim3(gb), dim3(256), 0, c->stream,
+                                       c->d_fr64, (const double*)dT, c->n_f, dst, nb, Gg, Gt);
+                const dim3 vg((unsigned)((c->n_v + 255) / 256), (unsigned)nb);
+                hipLaunchKernelGGL(k_gather_block, vg, dim3(256), 0, c->stream, Gg, c->adj.d_csr_off,
+                                   c->adj.d_csr_ent, c->n_v, c->n_f, Dg);
+                hipLaunchKernelGGL(k_gather_block, vg, dim3(256), 0, c->stream, Gt, c->adj.d_csr_off,
+                                   c->adj.d_csr_ent, c->n_v, c->n_f, Dt);
+                hipLaunchKernelGGL(k_dof_colnorm, dim3(jb), dim3(256), 0, c->stream, Dg, nb, c->n_v,
+                                   n_dof, doff, dvert, dw, og);
+                hipLaunchKernelGGL(k_dof_colnorm, dim3(jb), dim3(256), 0, c->stream, Dt, nb, c->n_v,
+                                   n_dof, doff, dvert, dw, ot);
+                CN(hipGetLastError(), "column-norm kernels");
+            }
+        }
+        if (out_gz) CN(hipMemcpyAsync(out_gz, og, sizeof(double) * n_dof, hipMemcpyDeviceToHost, c->stream), "copy");
+        if (out_tmi) CN(hipMemcpyAsync(out_tmi, ot, sizeof(double) * n_dof, hipMemcpyDeviceToHost, c->stream), "copy");
+    }
+    CN(hipStreamSynchronize(c->stream), "sync");
+#undef CN
+    cleanup();
+    return LF_OK;
+}
+
 int lf_sync(lf_ctx* c)
 {
     if (check_ctx(c)) return LF_EINVAL;

@@ -1,0 +1,344 @@
+// litho_capi.cpp -- C entry points over the C++ host layer, for the Python drivers
+// and tests (ctypes).  Handles are opaque; every call returns 0 on success and a
+// negative code on failure with li_last_error() describing it.
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "litho_host.hpp"
+
+using namespace lithofwd;
+
+namespace {
+thread_local std::string g_err;
+
+template <class F>
+int guard(F&& f)
+{
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+struct Problem {
+    std::shared_ptr<LithologyModel> model;
+    GravityData gdata;
+    MagneticData mdata;
+    std::shared_ptr<GravityForward> grav;
+    std::shared_ptr<MagneticForward> mag;
+    std::shared_ptr<ObjectiveFunction> gobj;
+    std::shared_ptr<ReferenceModelRegularization> ref;
+    std::unique_ptr<JointObjective> joint;
+};
+
+struct ModelHandle {
+    std::shared_ptr<LithologyModel> m;
+};
+}  // namespace
+
+extern "C" {
+
+const char* li_last_error(void) { return g_err.c_str(); }
+
+int li_model_create(void** out, int64_t n_v, const double* verts, int64_t n_f,
+                    const uint32_t* faces, const int32_t* front, const int32_t* back,
+                    int32_t n_u, const double* density, const double* susceptibility)
+{
+    return guard([&] {
+        std::vector<LithoGroup> groups(n_u);
+        for (int g = 0; g < n_u; ++g) {
+            groups[g].id = g;
+            groups[g].name = "unit" + std::to_string(g);
+            groups[g].density = density[g];
+            groups[g].susceptibility = susceptibility[g];
+        }
+        auto* h = new ModelHandle;
+        h->m = std::make_shared<LithologyModel>(
+            std::vector<double>(verts, verts + 3 * n_v),
+            std::vector<uint32_t>(faces, faces + 3 * n_f), std::vector<int32_t>(front, front + n_f),
+            std::vector<int32_t>(back, back + n_f), std::move(groups));
+        *out = h;
+    });
+}
+
+void li_model_destroy(void* h) { delete (ModelHandle*)h; }
+
+int li_model_fix_exterior(void* h, int interior_freedom)
+{
+    return guard([&] { ((ModelHandle*)h)->m->fixExteriorVertices((VertexFreedom)interior_freedom); });
+}
+
+int li_model_set_freedom(void* h, const uint8_t* freedom)
+{
+    return guard([&] {
+        auto& m = *((ModelHandle*)h)->m;
+        for (size_t v = 0; v < m.vertexCount(); ++v) m.setVertexFreedom(v, (VertexFreedom)freedom[v]);
+    });
+}
+
+int li_model_set_axis_scale(void* h, int axis, double s)
+{
+    return guard([&] { ((ModelHandle*)h)->m->setAxisScale(axis, s); });
+}
+
+int li_model_set_cp_stride(void* h, int stride, double cell, int64_t* n_dof)
+{
+    return guard([&] { *n_dof = ((ModelHandle*)h)->m->setControlPointStride(stride, cell); });
+}
+
+int64_t li_model_dof_count(void* h) { return ((ModelHandle*)h)->m->totalDofCount(); }
+
+int li_model_dofs(void* h, int32_t* vertex, int32_t* axis, uint8_t* is_cp)
+{
+    return guard([&] {
+        auto& m = *((ModelHandle*)h)->m;
+        for (uint32_t d = 0; d < m.totalDofCount(); ++d) {
+            vertex[d] = m.dofVertex(d);
+            axis[d] = m.dofAxis(d);
+        }
+        if (is_cp)
+            for (size_t v = 0; v < m.vertexCount(); ++v) is_cp[v] = m.isControlPoint(v) ? 1 : 0;
+    });
+}
+
+int li_model_params(void* h, double* out)
+{
+    return guard([&] {
+        const Vec p = ((ModelHandle*)h)->m->assembleParameterVector();
+        std::memcpy(out, p.data(), sizeof(double) * p.size());
+    });
+}
+
+int li_model_apply(void* h, const double* params, double* vertices_out)
+{
+    return guard([&] {
+        auto& m = *((ModelHandle*)h)->m;
+        m.applyParameterVector(Vec(params, params + m.totalDofCount()));
+        if (vertices_out)
+            std::memcpy(vertices_out, m.vertices().data(), sizeof(double) * m.vertices().size());
+    });
+}
+
+int li_model_pullback(void* h, const double* vgrad, double* out)
+{
+    return guard([&] {
+        auto& m = *((ModelHandle*)h)->m;
+        const Vec g = m.pullBackVertexGradient(Vec(vgrad, vgrad + 3 * m.vertexCount()));
+        std::memcpy(out, g.data(), sizeof(double) * g.size());
+    });
+}
+
+// parameter -> vertex map as CSR; call with vert/w NULL to get the entry count first
+int li_model_dof_map(void* h, int64_t* off, int32_t* vert, double* w, int64_t* n_entries)
+{
+    return guard([&] {
+        std::vector<int64_t> o;
+        std::vector<int32_t> v;
+        std::vector<double> ww;
+        ((ModelHandle*)h)->m->dofVertexMap(o, v, ww);
+        *n_entries = (int64_t)v.size();
+        if (off) std::memcpy(off, o.data(), sizeof(int64_t) * o.size());
+        if (vert) std::memcpy(vert, v.data(), sizeof(int32_t) * v.size());
+        if (w) std::memcpy(w, ww.data(), sizeof(double) * ww.size());
+    });
+}
+
+int li_model_bounds(void* h, const double* params, double margin_m, double* lo, double* hi)
+{
+    return guard([&] {
+        auto& m = *((ModelHandle*)h)->m;
+        Vec l, u;
+        m.getBounds(Vec(params, params + m.totalDofCount()), margin_m, l, u);
+        std::memcpy(lo, l.data(), sizeof(double) * l.size());
+        std::memcpy(hi, u.data(), sizeof(double) * u.size());
+    });
+}
+
+// Joint gravity + magnetic problem on one station set.  *_std <= 0 means unweighted.
+int li_problem_create(void** out, void* model, int64_t n_s, const double* stations,
+                      const double* gz_obs, const double* gz_std, const double* tmi_obs,
+                      const double* tmi_std, double inc, double dec, double field,
+                      double alpha_mag, int precision, double ref_weight, int device)
+{
+    return guard([&] {
+        auto P = std::make_unique<Problem>();
+        P->model = ((ModelHandle*)model)->m;
+        P->gdata.resize(n_s);
+        P->mdata.resize(n_s);
+        for (int64_t i = 0; i < n_s; ++i) {
+            for (int k = 0; k < 3; ++k) {
+                P->gdata[i].position[k] = stations[3 * i + k];
+                P->mdata[i].position[k] = stations[3 * i + k];
+            }
+            P->gdata[i].g_obs = gz_obs ? gz_obs[i] : 0.0;
+            P->gdata[i].g_std = gz_std ? gz_std[i] : 0.0;
+            P->mdata[i].t_obs = tmi_obs ? tmi_obs[i] : 0.0;
+            P->mdata[i].t_std = tmi_std ? tmi_std[i] : 0.0;
+        }
+        const Precision prec = precision == 1 ? Precision::FP32C : Precision::FP64;
+        P->grav = std::make_shared<GravityForward>(P->model, P->gdata, prec, device);
+        P->gobj = std::make_shared<ObjectiveFunction>(P->grav, P->gdata);
+        if (ref_weight > 0.0) {
+            P->ref = std::make_shared<ReferenceModelRegularization>();
+            P->ref->setWeight(ref_weight);
+            P->ref->captureReference(P->model->assembleParameterVector());
+            P->gobj->addRegularization(P->ref);
+        }
+        P->joint = std::make_unique<JointObjective>(P->gobj);
+        if (tmi_obs && alpha_mag > 0.0) {
+            P->mag = std::make_shared<MagneticForward>(P->model, P->mdata, inc, dec, field, prec);
+            P->joint->setMagnetic(P->mag, P->mdata, alpha_mag);
+        }
+        *out = P.release();
+    });
+}
+
+void li_problem_destroy(void* h) { delete (Problem*)h; }
+
+// components: gravity misfit, magnetic misfit, regularization, total
+int li_problem_evaluate(void* h, const double* params, double* comp)
+{
+    return guard([&] {
+        auto* P = (Problem*)h;
+        const auto c = P->joint->evaluateComponents(Vec(params, params + P->model->totalDofCount()));
+        comp[0] = c.gravityMisfit;
+        comp[1] = c.magneticMisfit;
+        comp[2] = c.regularization;
+        comp[3] = c.total;
+    });
+}
+
+int li_problem_gradient(void* h, const double* params, double* out)
+{
+    return guard([&] {
+        auto* P = (Problem*)h;
+        const Vec g = P->joint->gradient(Vec(params, params + P->model->totalDofCount()));
+        std::memcpy(out, g.data(), sizeof(double) * g.size());
+    });
+}
+
+int li_problem_forward(void* h, const double* params, double* gz, double* tmi)
+{
+    return guard([&] {
+        auto* P = (Problem*)h;
+        const Vec p(params, params + P->model->totalDofCount());
+        if (gz) {
+            const Vec g = P->grav->compute(p);
+            std::memcpy(gz, g.data(), sizeof(double) * g.size());
+        }
+        if (tmi) {
+            if (!P->mag) throw std::runtime_error("problem has no magnetic data");
+            const Vec t = P->mag->compute(p);
+            std::memcpy(tmi, t.data(), sizeof(double) * t.size());
+        }
+    });
+}
+
+// Dense Jacobians through the C++ ForwardModel surface (small problems).
+int li_problem_jacobian(void* h, const double* params, int which, double* J_colmajor)
+{
+    return guard([&] {
+        auto* P = (Problem*)h;
+        const Vec p(params, params + P->model->totalDofCount());
+        Mat J = which == 0 ? P->grav->computeJacobian(p) : P->mag->computeJacobian(p);
+        std::memcpy(J_colmajor, J.data.data(), sizeof(double) * J.data.size());
+    });
+}
+
+int li_problem_unit_response(void* h, int which, int group, double* out)
+{
+    return guard([&] {
+        auto* P = (Problem*)h;
+        const Vec r = which == 0 ? P->grav->computeGroupUnitResponse(group)
+                                 : P->mag->computeGroupUnitResponse(group);
+        std::memcpy(out, r.data(), sizeof(double) * r.size());
+    });
+}
+
+typedef double (*li_obj_cb)(const double* x, int64_t n, void* user);
+typedef void (*li_grad_cb)(const double* x, int64_t n, double* g, void* user);
+
+struct LiOptions {
+    int32_t max_iter;
+    int32_t history;
+    double ftol;
+    double armijo_c1;
+    double target;  // NaN: none
+    int32_t has_h0;
+    int32_t active_set;
+};
+
+// hist rows: iteration, f, |proj grad|, alpha, seconds, objective calls, gradient calls
+static int run_minimize(LBFGSBOptimizer::Objective obj, LBFGSBOptimizer::Gradient grad,
+                        int64_t n, const double* x0, const double* lb, const double* ub,
+                        const LiOptions* o, const double* h0, double* x_out, double* hist,
+                        int64_t hist_cap, int64_t* n_hist, int32_t* iterations, char* msg,
+                        int64_t msg_cap)
+{
+    return guard([&] {
+        LBFGSBOptimizer opt;
+        opt.setMaxIterations(o->max_iter);
+        opt.setHistorySize(o->history);
+        opt.setTolerance(o->ftol);
+        opt.setArmijoC1(o->armijo_c1);
+        if (!std::isnan(o->target)) opt.setTargetObjective(o->target);
+        if (o->has_h0 && h0) opt.setInitialHessianDiagonal(Vec(h0, h0 + n));
+        opt.setActiveSetAware(o->active_set != 0);
+        const auto r = opt.minimize(obj, grad, Vec(x0, x0 + n), Vec(lb, lb + n), Vec(ub, ub + n));
+        std::memcpy(x_out, r.params.data(), sizeof(double) * n);
+        const auto& H = opt.history();
+        const int64_t nh = std::min<int64_t>((int64_t)H.size(), hist_cap);
+        for (int64_t k = 0; k < nh; ++k) {
+            double* row = hist + 7 * k;
+            row[0] = H[k].iteration;
+            row[1] = H[k].f;
+            row[2] = H[k].pgNorm;
+            row[3] = H[k].alpha;
+            row[4] = H[k].seconds;
+            row[5] = H[k].objectiveCalls;
+            row[6] = H[k].gradientCalls;
+        }
+        *n_hist = nh;
+        *iterations = r.iterations;
+        if (msg && msg_cap > 0) {
+            std::strncpy(msg, r.message.c_str(), (size_t)msg_cap - 1);
+            msg[msg_cap - 1] = 0;
+        }
+    });
+}
+
+int li_minimize_callbacks(int64_t n, const double* x0, const double* lb, const double* ub,
+                          li_obj_cb f, li_grad_cb g, void* user, const LiOptions* o,
+                          const double* h0, double* x_out, double* hist, int64_t hist_cap,
+                          int64_t* n_hist, int32_t* iterations, char* msg, int64_t msg_cap)
+{
+    auto obj = [&](const Vec& x) { return f(x.data(), (int64_t)x.size(), user); };
+    auto grad = [&](const Vec& x) {
+        Vec out(x.size());
+        g(x.data(), (int64_t)x.size(), out.data(), user);
+        return out;
+    };
+    return run_minimize(obj, grad, n, x0, lb, ub, o, h0, x_out, hist, hist_cap, n_hist, iterations,
+                        msg, msg_cap);
+}
+
+int li_problem_minimize(void* h, const double* x0, const double* lb, const double* ub,
+                        const LiOptions* o, const double* h0, double* x_out, double* hist,
+                        int64_t hist_cap, int64_t* n_hist, int32_t* iterations, char* msg,
+                        int64_t msg_cap)
+{
+    auto* P = (Problem*)h;
+    const int64_t n = P->model->totalDofCount();
+    auto obj = [&](const Vec& x) { return P->joint->evaluate(x); };
+    auto grad = [&](const Vec& x) { return P->joint->gradient(x); };
+    return run_minimize(obj, grad, n, x0, lb, ub, o, h0, x_out, hist, hist_cap, n_hist, iterations,
+                        msg, msg_cap);
+}
+
+}  // extern "C"

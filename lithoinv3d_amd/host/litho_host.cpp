@@ -1,0 +1,845 @@
+// litho_host.cpp -- implementation of the C++ host layer (see litho_host.hpp).
+#include "litho_host.hpp"
+
+#include <algorithm>
+#include <array>
+#include <tuple>
+#include <chrono>
+#include <cmath>
+#include <limits>
+#include <map>
+#include <stdexcept>
+#include <unordered_map>
+
+#include "../../include/lithofwd.h"
+
+namespace lithofwd {
+
+namespace {
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kEps = 1e-15;
+
+double dot(const Vec& a, const Vec& b)
+{
+    double s = 0.0;
+    for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+    return s;
+}
+
+void fieldUnitVector(double inc_deg, double dec_deg, double out[3])
+{
+    // magnetic_forward.cpp:61-66
+    const double I = inc_deg * kPi / 180.0, D = dec_deg * kPi / 180.0;
+    out[0] = std::cos(I) * std::cos(D);
+    out[1] = std::cos(I) * std::sin(D);
+    out[2] = -std::sin(I);
+}
+
+void check(int rc, lf_ctx* c, const char* what)
+{
+    if (rc != 0)
+        throw std::runtime_error(std::string(what) + ": " + lf_last_error(c));
+}
+}  // namespace
+
+// =========================================================================== model
+
+LithologyModel::LithologyModel(std::vector<double> vertices, std::vector<uint32_t> faces,
+                               std::vector<int32_t> front, std::vector<int32_t> back,
+                               std::vector<LithoGroup> groups)
+    : m_vertices(std::move(vertices)), m_faces(std::move(faces)), m_front(std::move(front)),
+      m_back(std::move(back)), m_groups(std::move(groups))
+{
+    if (m_vertices.size() % 3 || m_faces.size() != 3 * m_front.size() ||
+        m_front.size() != m_back.size())
+        throw std::invalid_argument("LithologyModel: inconsistent array sizes");
+    m_rest = m_vertices;
+    m_freedom.assign(vertexCount(), VertexFreedom::FIXED);
+    m_isCP.assign(vertexCount(), 1);
+    rebuildDofs();
+}
+
+void LithologyModel::setVertexFreedom(size_t v, VertexFreedom f)
+{
+    m_freedom.at(v) = f;
+    rebuildDofs();
+}
+
+void LithologyModel::fixExteriorVertices(VertexFreedom interior)
+{
+    std::vector<uint8_t> ext(vertexCount(), 0);
+    for (size_t f = 0; f < m_front.size(); ++f)
+        if (m_back[f] < 0)
+            for (int k = 0; k < 3; ++k) ext[m_faces[3 * f + k]] = 1;
+    for (size_t v = 0; v < vertexCount(); ++v)
+        m_freedom[v] = ext[v] ? VertexFreedom::FIXED : interior;
+    rebuildDofs();
+}
+
+void LithologyModel::setAxisScale(int axis, double s)
+{
+    if (axis >= 0 && axis < 3 && s > 0) m_axisScale[axis] = s;
+}
+
+bool LithologyModel::isControlPoint(size_t v) const { return m_cpStride <= 0 || m_isCP[v]; }
+
+uint32_t LithologyModel::setControlPointStride(int stride, double cellSize)
+{
+    const size_t nv = vertexCount();
+    m_cpStride = stride;
+    m_wOff.clear();
+    m_wCp.clear();
+    m_w.clear();
+    m_isCP.assign(nv, 1);
+    if (stride <= 0 || nv < 4) {
+        m_cpStride = 0;
+        rebuildDofs();
+        return totalDofCount();
+    }
+    // quantized positions (1 um), lexicographic order like the reference's std::map
+    struct Q { int64_t x, y, z; };
+    std::vector<Q> q(nv);
+    for (size_t v = 0; v < nv; ++v)
+        q[v] = {(int64_t)std::llround(m_vertices[3 * v] * 1e6),
+                (int64_t)std::llround(m_vertices[3 * v + 1] * 1e6),
+                (int64_t)std::llround(m_vertices[3 * v + 2] * 1e6)};
+    std::vector<int32_t> order(nv);
+    for (size_t v = 0; v < nv; ++v) order[v] = (int32_t)v;
+    std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+        if (q[a].x != q[b].x) return q[a].x < q[b].x;
+        if (q[a].y != q[b].y) return q[a].y < q[b].y;
+        return q[a].z < q[b].z;
+    });
+    if (cellSize <= 0.0) {
+        // lithology_model.cpp:916-934
+        double mn[3] = {1e12, 1e12, 1e12};
+        const size_t ns = std::min<size_t>(nv, 300);
+        for (size_t i = 0; i < ns; ++i)
+            for (size_t j = i + 1; j < ns; ++j) {
+                const Q& a = q[order[i]];
+                const Q& b = q[order[j]];
+                const double d[3] = {std::abs((double)(a.x - b.x)) * 1e-6,
+                                     std::abs((double)(a.y - b.y)) * 1e-6,
+                                     std::abs((double)(a.z - b.z)) * 1e-6};
+                for (int k = 0; k < 3; ++k)
+                    if (d[k] > 0 && d[k] < mn[k]) mn[k] = d[k];
+            }
+        cellSize = std::max({mn[0], mn[1], mn[2], 1.0});
+    }
+    const int64_t strideQ = std::max<int64_t>(1, (int64_t)stride * (int64_t)(cellSize * 1e6));
+    auto cell = [strideQ](int64_t v) { return v >= 0 ? v / strideQ : (v - strideQ + 1) / strideQ; };
+    struct Best { int32_t v; int64_t d2; };
+    std::map<std::tuple<int64_t, int64_t, int64_t>, Best> best;
+    for (int32_t v : order) {
+        const int64_t cx = cell(q[v].x), cy = cell(q[v].y), cz = cell(q[v].z);
+        const int64_t ox = cx * strideQ + strideQ / 2, oy = cy * strideQ + strideQ / 2,
+                      oz = cz * strideQ + strideQ / 2;
+        const double dx = (double)(q[v].x - ox), dy = (double)(q[v].y - oy), dz = (double)(q[v].z - oz);
+        const int64_t d2 = (int64_t)(dx * dx + dy * dy + dz * dz);
+        auto key = std::make_tuple(cx, cy, cz);
+        auto it = best.find(key);
+        if (it == best.end() || d2 < it->second.d2) best[key] = {v, d2};
+    }
+    std::vector<int32_t> cps;
+    m_isCP.assign(nv, 0);
+    for (auto& kv : best) {
+        cps.push_back(kv.second.v);
+        m_isCP[kv.second.v] = 1;
+    }
+    // K = 4 nearest control points per non-control vertex, inverse-distance weights
+    // (surface_mesh.cpp:566-598); buckets of one grid cell keep the search local.
+    const double bucket = std::max(1.0, stride * cellSize);
+    std::map<std::array<int64_t, 3>, std::vector<int32_t>> buckets;
+    for (size_t i = 0; i < cps.size(); ++i) {
+        const double* p = &m_vertices[3 * (size_t)cps[i]];
+        std::array<int64_t, 3> c = {(int64_t)std::floor(p[0] / bucket),
+                                    (int64_t)std::floor(p[1] / bucket),
+                                    (int64_t)std::floor(p[2] / bucket)};
+        buckets[c].push_back(cps[i]);
+    }
+    m_wOff.assign(nv + 1, 0);
+    const int K = 4;
+    std::vector<std::pair<double, int32_t>> cand;
+    for (size_t v = 0; v < nv; ++v) {
+        if (m_isCP[v]) {
+            m_wOff[v + 1] = m_wOff[v];
+            continue;
+        }
+        const double* p = &m_vertices[3 * v];
+        const std::array<int64_t, 3> c = {(int64_t)std::floor(p[0] / bucket),
+                                          (int64_t)std::floor(p[1] / bucket),
+                                          (int64_t)std::floor(p[2] / bucket)};
+        cand.clear();
+        for (int r = 1; r < 64; ++r) {
+            cand.clear();
+            for (int64_t dx = -r; dx <= r; ++dx)
+                for (int64_t dy = -r; dy <= r; ++dy)
+                    for (int64_t dz = -r; dz <= r; ++dz) {
+                        auto it = buckets.find({c[0] + dx, c[1] + dy, c[2] + dz});
+                        if (it == buckets.end()) continue;
+                        for (int32_t cv : it->second) {
+                            const double* pc = &m_vertices[3 * (size_t)cv];
+                            const double d = std::sqrt((pc[0] - p[0]) * (pc[0] - p[0]) +
+                                                       (pc[1] - p[1]) * (pc[1] - p[1]) +
+                                                       (pc[2] - p[2]) * (pc[2] - p[2]));
+                            cand.push_back({d, cv});
+                        }
+                    }
+            // the K nearest are exact once the K-th distance is within the searched shell
+            if ((int)cand.size() >= K) {
+                std::partial_sort(cand.begin(), cand.begin() + K, cand.end());
+                if (cand[K - 1].first <= r * bucket) break;
+            }
+        }
+        const size_t keep = std::min<size_t>(K, cand.size());
+        std::partial_sort(cand.begin(), cand.begin() + keep, cand.end());
+        double tot = 0.0;
+        for (size_t j = 0; j < keep; ++j) tot += 1.0 / std::max(cand[j].first, 1e-12);
+        for (size_t j = 0; j < keep; ++j) {
+            m_wCp.push_back(cand[j].second);
+            m_w.push_back((1.0 / std::max(cand[j].first, 1e-12)) / tot);
+        }
+        m_wOff[v + 1] = m_wOff[v] + (int64_t)keep;
+    }
+    rebuildDofs();
+    return totalDofCount();
+}
+
+void LithologyModel::rebuildDofs()
+{
+    m_dofVertex.clear();
+    m_dofAxis.clear();
+    for (size_t v = 0; v < vertexCount(); ++v) {
+        if (m_cpStride > 0 && !m_isCP[v]) continue;
+        switch (m_freedom[v]) {
+            case VertexFreedom::FIXED: break;
+            case VertexFreedom::Z_ONLY: m_dofVertex.push_back((int)v); m_dofAxis.push_back(2); break;
+            case VertexFreedom::X_ONLY: m_dofVertex.push_back((int)v); m_dofAxis.push_back(0); break;
+            case VertexFreedom::Y_ONLY: m_dofVertex.push_back((int)v); m_dofAxis.push_back(1); break;
+            case VertexFreedom::XYZ_FREE:
+                for (int a = 0; a < 3; ++a) {
+                    m_dofVertex.push_back((int)v);
+                    m_dofAxis.push_back(a);
+                }
+                break;
+        }
+    }
+}
+
+Vec LithologyModel::assembleParameterVector() const
+{
+    Vec p(m_dofVertex.size());
+    for (size_t d = 0; d < p.size(); ++d)
+        p[d] = m_vertices[3 * (size_t)m_dofVertex[d] + m_dofAxis[d]] / m_axisScale[m_dofAxis[d]];
+    return p;
+}
+
+void LithologyModel::applyParameterVector(const Vec& params)
+{
+    if (params.size() != m_dofVertex.size())
+        throw std::invalid_argument("applyParameterVector: wrong parameter count");
+    for (size_t d = 0; d < params.size(); ++d)
+        m_vertices[3 * (size_t)m_dofVertex[d] + m_dofAxis[d]] = params[d] * m_axisScale[m_dofAxis[d]];
+    if (m_cpStride > 0) {
+        // displacement of each control point, spread to its dependants
+        for (size_t v = 0; v < vertexCount(); ++v) {
+            if (m_isCP[v]) continue;
+            for (int a = 0; a < 3; ++a) {
+                double disp = 0.0;
+                for (int64_t e = m_wOff[v]; e < m_wOff[v + 1]; ++e) {
+                    const size_t c = (size_t)m_wCp[e];
+                    disp += m_w[e] * (m_vertices[3 * c + a] - m_rest[3 * c + a]);
+                }
+                m_vertices[3 * v + a] = m_rest[3 * v + a] + disp;
+            }
+        }
+    }
+}
+
+Vec LithologyModel::pullBackVertexGradient(const Vec& vg) const
+{
+    // dPhi/dx_cp += sum over dependants w * dPhi/dx_v
+    std::vector<double> eff(vg.begin(), vg.end());
+    if (m_cpStride > 0) {
+        for (size_t v = 0; v < vertexCount(); ++v) {
+            if (m_isCP[v]) continue;
+            for (int64_t e = m_wOff[v]; e < m_wOff[v + 1]; ++e) {
+                const size_t c = (size_t)m_wCp[e];
+                for (int a = 0; a < 3; ++a) eff[3 * c + a] += m_w[e] * vg[3 * v + a];
+            }
+        }
+    }
+    Vec g(m_dofVertex.size());
+    for (size_t d = 0; d < g.size(); ++d)
+        g[d] = eff[3 * (size_t)m_dofVertex[d] + m_dofAxis[d]] * m_axisScale[m_dofAxis[d]];
+    return g;
+}
+
+void LithologyModel::dofVertexMap(std::vector<int64_t>& off, std::vector<int32_t>& vert,
+                                  std::vector<double>& w) const
+{
+    const size_t nd = m_dofVertex.size();
+    // dependants of each control-point vertex (transpose of the IDW lists)
+    std::vector<std::vector<std::pair<int32_t, double>>> dep;
+    if (m_cpStride > 0) {
+        dep.resize(vertexCount());
+        for (size_t v = 0; v < vertexCount(); ++v)
+            for (int64_t e = m_wOff[v]; e < m_wOff[v + 1]; ++e)
+                dep[(size_t)m_wCp[e]].push_back({(int32_t)v, m_w[e]});
+    }
+    off.assign(nd + 1, 0);
+    vert.clear();
+    w.clear();
+    for (size_t d = 0; d < nd; ++d) {
+        const double s = m_axisScale[m_dofAxis[d]];
+        vert.push_back(m_dofVertex[d]);
+        w.push_back(s);
+        if (m_cpStride > 0)
+            for (auto& pr : dep[(size_t)m_dofVertex[d]]) {
+                vert.push_back(pr.first);
+                w.push_back(pr.second * s);
+            }
+        off[d + 1] = (int64_t)vert.size();
+    }
+}
+
+void LithologyModel::getBounds(const Vec& params, double margin_m, Vec& lower, Vec& upper) const
+{
+    lower.resize(params.size());
+    upper.resize(params.size());
+    for (size_t d = 0; d < params.size(); ++d) {
+        const double m = margin_m / m_axisScale[m_dofAxis[d]];
+        lower[d] = params[d] - m;
+        upper[d] = params[d] + m;
+    }
+}
+
+// ================================================================== ForwardModel
+
+Mat ForwardModel::computeJacobian(const Vec& params)
+{
+    const size_t nd = dataCount(), np = parameterCount();
+    Mat J(nd, np);
+    for (size_t j = 0; j < np; ++j) {
+        Vec pp = params, pm = params;
+        pp[j] += m_fdStep;
+        pm[j] -= m_fdStep;
+        const Vec fp = compute(pp), fm = compute(pm);
+        for (size_t i = 0; i < nd; ++i) J(i, j) = (fp[i] - fm[i]) / (2.0 * m_fdStep);
+    }
+    compute(params);
+    return J;
+}
+
+void ForwardModel::computeBoth(const Vec& params, Vec& predicted, Mat& jacobian)
+{
+    predicted = compute(params);
+    jacobian = computeJacobian(params);
+}
+
+Vec ForwardModel::computeJTr(const Vec& params, const Vec& w)
+{
+    const Mat J = computeJacobian(params);
+    Vec g(J.cols, 0.0);
+    for (size_t j = 0; j < J.cols; ++j)
+        for (size_t i = 0; i < J.rows; ++i) g[j] += J(i, j) * w[i];
+    return g;
+}
+
+// ================================================================ GPU forwards
+
+GpuForwardBase::GpuForwardBase(std::shared_ptr<LithologyModel> model,
+                               std::vector<double> stations, int want, Precision prec,
+                               const double fdir[3], int device)
+    : m_model(std::move(model)), m_nData(stations.size() / 3), m_want(want)
+{
+    lf_desc d{};
+    d.n_vertices = (int64_t)m_model->vertexCount();
+    d.n_faces = (int64_t)m_model->faceCount();
+    d.n_stations = (int64_t)m_nData;
+    d.n_units = m_model->groupCount();
+    d.precision = prec == Precision::FP32C ? LF_FP32C : LF_FP64;
+    d.tmi_mode = LF_TMI_REFERENCE;
+    d.want = want;
+    d.device = device;
+    d.vertices = m_model->vertices().data();
+    d.faces = m_model->faces().data();
+    d.front = m_model->front().data();
+    d.back = m_model->back().data();
+    d.stations = stations.data();
+    std::vector<double> dens(m_model->groupCount(), 0.0), mag(3 * m_model->groupCount(), 0.0);
+    d.density = dens.data();
+    d.magnetization = mag.data();
+    for (int k = 0; k < 3; ++k) d.field_dir[k] = fdir[k];
+    const int rc = lf_create(&m_ctx, &d);
+    if (rc != 0) throw std::runtime_error(std::string("lf_create: ") + lf_last_error(nullptr));
+}
+
+GpuForwardBase::~GpuForwardBase()
+{
+    if (m_ctx) lf_destroy(m_ctx);
+}
+
+void GpuForwardBase::pushProperties()
+{
+    std::vector<double> dens, mag;
+    buildProperties(dens, mag);
+    check(lf_set_properties(m_ctx, dens.data(), mag.data()), m_ctx, "lf_set_properties");
+}
+
+void GpuForwardBase::upload(const Vec& params)
+{
+    m_model->applyParameterVector(params);
+    check(lf_set_vertices(m_ctx, m_model->vertices().data()), m_ctx, "lf_set_vertices");
+}
+
+Vec GpuForwardBase::forwardRaw(int which)
+{
+    Vec out(m_nData);
+    check(lf_forward(m_ctx, which == 0 ? out.data() : nullptr, which == 1 ? out.data() : nullptr),
+          m_ctx, "lf_forward");
+    return out;
+}
+
+Vec GpuForwardBase::computeJTr(const Vec& params, const Vec& w)
+{
+    upload(params);
+    Vec vg(3 * m_model->vertexCount());
+    const bool gz = (m_want & LF_WANT_GZ) != 0;
+    check(lf_adjoint(m_ctx, gz ? w.data() : nullptr, gz ? nullptr : w.data(), vg.data()), m_ctx,
+          "lf_adjoint");
+    return m_model->pullBackVertexGradient(vg);
+}
+
+Mat GpuForwardBase::computeJacobian(const Vec& params)
+{
+    const size_t nd = dataCount(), np = parameterCount();
+    Mat J(nd, np);
+    Vec e(nd, 0.0);
+    for (size_t i = 0; i < nd; ++i) {
+        e[i] = 1.0;
+        const Vec row = computeJTr(params, e);
+        e[i] = 0.0;
+        for (size_t j = 0; j < np; ++j) J(i, j) = row[j];
+    }
+    return J;
+}
+
+GravityForward::GravityForward(std::shared_ptr<LithologyModel> model, const GravityData& data,
+                               Precision prec, int device)
+    : GpuForwardBase(model,
+                     [&] {
+                         std::vector<double> s;
+                         for (auto& p : data) s.insert(s.end(), p.position, p.position + 3);
+                         return s;
+                     }(),
+                     LF_WANT_GZ, prec, (const double[3]){0.0, 0.0, 1.0}, device)
+{
+    pushProperties();
+}
+
+void GravityForward::buildProperties(std::vector<double>& dens, std::vector<double>& mag) const
+{
+    const int ng = m_model->groupCount();
+    dens.assign(ng, 0.0);
+    mag.assign(3 * ng, 0.0);
+    for (int g = 0; g < ng; ++g)
+        dens[g] = (m_unitOnly < 0) ? m_model->group(g).density : (g == m_unitOnly ? 1.0 : 0.0);
+}
+
+Vec GravityForward::compute(const Vec& params)
+{
+    upload(params);
+    return forwardRaw(0);
+}
+
+Vec GravityForward::computeGroupUnitResponse(int g)
+{
+    m_unitOnly = g;
+    pushProperties();
+    Vec r = forwardRaw(0);
+    m_unitOnly = -1;
+    pushProperties();
+    return r;
+}
+
+MagneticForward::MagneticForward(std::shared_ptr<LithologyModel> model, const MagneticData& data,
+                                 double inc_deg, double dec_deg, double field_nT, Precision prec,
+                                 bool textbook, int device)
+    : GpuForwardBase(model,
+                     [&] {
+                         std::vector<double> s;
+                         for (auto& p : data) s.insert(s.end(), p.position, p.position + 3);
+                         return s;
+                     }(),
+                     LF_WANT_TMI, prec,
+                     [&] {
+                         static thread_local double f[3];
+                         fieldUnitVector(inc_deg, dec_deg, f);
+                         return (const double*)f;
+                     }(),
+                     device),
+      m_field(field_nT)
+{
+    fieldUnitVector(inc_deg, dec_deg, m_fhat);
+    (void)textbook;
+    pushProperties();
+}
+
+void MagneticForward::buildGroupMagnetization(int g, double& amp, double dir[3]) const
+{
+    // magnetic_forward.cpp:100-134
+    const LithoGroup& G = m_model->group(g);
+    const double induced = G.susceptibility * m_field / 100.0;
+    double M[3] = {induced * m_fhat[0], induced * m_fhat[1], induced * m_fhat[2]};
+    if (m_mode == RemanentMagnetizationMode::FixedVectorPerGroup) {
+        double R[3];
+        fieldUnitVector(G.remanence_inclination, G.remanence_declination, R);
+        for (int k = 0; k < 3; ++k) M[k] += G.remanence_magnitude * R[k];
+    } else if (m_mode == RemanentMagnetizationMode::VectorPerGroup) {
+        for (int k = 0; k < 3; ++k) M[k] += G.magnetization[k];
+    }
+    amp = std::sqrt(M[0] * M[0] + M[1] * M[1] + M[2] * M[2]);
+    for (int k = 0; k < 3; ++k) dir[k] = amp < kEps ? m_fhat[k] : M[k] / amp;
+}
+
+void MagneticForward::buildProperties(std::vector<double>& dens, std::vector<double>& mag) const
+{
+    const int ng = m_model->groupCount();
+    dens.assign(ng, 0.0);
+    mag.assign(3 * ng, 0.0);
+    for (int g = 0; g < ng; ++g) {
+        double amp, dir[3];
+        if (m_unitOnly >= 0) {
+            // unit susceptibility, induced (magnetic_forward.cpp:148-162)
+            amp = g == m_unitOnly ? m_field / 100.0 : 0.0;
+            for (int k = 0; k < 3; ++k) dir[k] = m_fhat[k];
+        } else {
+            buildGroupMagnetization(g, amp, dir);
+        }
+        for (int k = 0; k < 3; ++k) mag[3 * g + k] = amp * dir[k];
+    }
+}
+
+Vec MagneticForward::compute(const Vec& params)
+{
+    upload(params);
+    return forwardRaw(1);
+}
+
+Vec MagneticForward::computeGroupUnitResponse(int g)
+{
+    m_unitOnly = g;
+    pushProperties();
+    Vec r = forwardRaw(1);
+    m_unitOnly = -1;
+    pushProperties();
+    return r;
+}
+
+// ============================================================= regularization
+
+double ReferenceModelRegularization::evaluate(const Vec& params)
+{
+    if (!m_captured) return 0.0;
+    double s = 0.0;
+    for (size_t i = 0; i < params.size(); ++i) s += (params[i] - m_ref[i]) * (params[i] - m_ref[i]);
+    return 0.5 * m_weight * s;
+}
+
+Vec ReferenceModelRegularization::gradient(const Vec& params)
+{
+    Vec g(params.size(), 0.0);
+    if (!m_captured) return g;
+    for (size_t i = 0; i < params.size(); ++i) g[i] = m_weight * (params[i] - m_ref[i]);
+    return g;
+}
+
+// ================================================================== objectives
+
+ObjectiveFunction::ObjectiveFunction(std::shared_ptr<ForwardModel> forward, const GravityData& data)
+    : m_forward(std::move(forward)), m_data(data)
+{
+}
+
+double ObjectiveFunction::dataMisfit(const Vec& params)
+{
+    const Vec pred = m_forward->compute(params);
+    double mis = 0.0;
+    for (size_t i = 0; i < m_data.size(); ++i) {
+        const double w = m_data[i].g_std > 0 ? 1.0 / m_data[i].g_std : 1.0;
+        const double r = w * (m_data[i].g_obs - pred[i]);
+        mis += 0.5 * r * r;
+    }
+    return mis;
+}
+
+Vec ObjectiveFunction::residuals(const Vec& params)
+{
+    const Vec pred = m_forward->compute(params);
+    Vec r(m_data.size());
+    for (size_t i = 0; i < m_data.size(); ++i) r[i] = m_data[i].g_obs - pred[i];
+    return r;
+}
+
+Vec ObjectiveFunction::dataMisfitGradient(const Vec& params)
+{
+    // -J^T W^2 r  (objective.cpp:42-55), matrix-free
+    const Vec pred = m_forward->compute(params);
+    Vec w(m_data.size());
+    for (size_t i = 0; i < m_data.size(); ++i) {
+        const double wi = m_data[i].g_std > 0 ? 1.0 / m_data[i].g_std : 1.0;
+        w[i] = -wi * wi * (m_data[i].g_obs - pred[i]);
+    }
+    return m_forward->computeJTr(params, w);
+}
+
+Vec ObjectiveFunction::gradient(const Vec& params)
+{
+    Vec g = dataMisfitGradient(params);
+    for (auto& r : m_regs) {
+        const Vec gr = r->gradient(params);
+        for (size_t i = 0; i < g.size(); ++i) g[i] += gr[i];
+    }
+    return g;
+}
+
+ObjectiveFunction::Components ObjectiveFunction::evaluateComponents(const Vec& params)
+{
+    Components c;
+    c.dataMisfit = dataMisfit(params);
+    for (auto& r : m_regs) c.regularization += r->evaluate(params);
+    c.total = c.dataMisfit + c.regularization + c.constraintPenalty;
+    return c;
+}
+
+void JointObjective::setMagnetic(std::shared_ptr<ForwardModel> fwd, const MagneticData& data,
+                                 double alpha)
+{
+    m_mag = std::move(fwd);
+    m_magData = &data;
+    m_alpha = alpha;
+}
+
+Vec JointObjective::gradient(const Vec& params)
+{
+    Vec g = m_grav->gradient(params);
+    if (m_mag && m_magData && !m_magData->empty() && m_alpha > 0.0) {
+        // -alpha J_mag^T W^2 r  (joint_objective.cpp:47-58)
+        const Vec pred = m_mag->compute(params);
+        Vec w(m_magData->size());
+        for (size_t i = 0; i < w.size(); ++i) {
+            const auto& d = (*m_magData)[i];
+            const double wi = d.t_std > 0 ? 1.0 / d.t_std : 1.0;
+            w[i] = -m_alpha * wi * wi * (d.t_obs - pred[i]);
+        }
+        const Vec gm = m_mag->computeJTr(params, w);
+        for (size_t i = 0; i < g.size(); ++i) g[i] += gm[i];
+    }
+    return g;
+}
+
+JointObjective::JointComponents JointObjective::evaluateComponents(const Vec& params)
+{
+    JointComponents c;
+    const auto gc = m_grav->evaluateComponents(params);
+    c.gravityMisfit = gc.dataMisfit;
+    c.regularization = gc.regularization;
+    c.constraintPenalty = gc.constraintPenalty;
+    if (m_mag && m_magData && !m_magData->empty() && m_alpha > 0.0) {
+        const Vec pred = m_mag->compute(params);
+        double mis = 0.0;
+        for (size_t i = 0; i < pred.size(); ++i) {
+            const auto& d = (*m_magData)[i];
+            const double wi = d.t_std > 0 ? 1.0 / d.t_std : 1.0;
+            const double r = wi * (d.t_obs - pred[i]);
+            mis += 0.5 * r * r;
+        }
+        c.magneticMisfit = mis;
+    }
+    c.total = c.gravityMisfit + m_alpha * c.magneticMisfit + c.regularization + c.constraintPenalty;
+    return c;
+}
+
+// =================================================================== L-BFGS-B
+
+namespace {
+Vec project(const Vec& x, const Vec& lb, const Vec& ub)
+{
+    Vec p(x.size());
+    for (size_t i = 0; i < x.size(); ++i) p[i] = std::max(lb[i], std::min(ub[i], x[i]));
+    return p;
+}
+}  // namespace
+
+Vec LBFGSBOptimizer::direction(const Vec& gin, const std::vector<uint8_t>& freeMask) const
+{
+    // lbfgsb_optimizer.cpp:33-72 with H0 = gamma * D
+    const size_t n = gin.size();
+    const bool diag = m_h0.size() == n;
+    Vec g = gin;
+    if (!freeMask.empty())
+        for (size_t i = 0; i < n; ++i)
+            if (!freeMask[i]) g[i] = 0.0;
+    if (m_s.empty()) {
+        Vec d(n);
+        for (size_t i = 0; i < n; ++i) d[i] = -(diag ? m_h0[i] : 1.0) * g[i];
+        return d;
+    }
+    const Vec& sl = m_s.back();
+    const Vec& yl = m_y.back();
+    double gamma;
+    if (diag) {
+        double yDy = 0.0;
+        for (size_t i = 0; i < n; ++i) yDy += yl[i] * m_h0[i] * yl[i];
+        gamma = yDy > std::numeric_limits<double>::epsilon() ? dot(sl, yl) / yDy : 1.0;
+    } else {
+        const double yy = dot(yl, yl);
+        gamma = yy > std::numeric_limits<double>::epsilon() ? dot(sl, yl) / yy : 1.0;
+    }
+    const int m = (int)m_s.size();
+    std::vector<double> alpha(m);
+    Vec q = g;
+    for (int i = m - 1; i >= 0; --i) {
+        alpha[i] = m_rho[i] * dot(m_s[i], q);
+        for (size_t k = 0; k < n; ++k) q[k] -= alpha[i] * m_y[i][k];
+    }
+    Vec r(n);
+    for (size_t k = 0; k < n; ++k) r[k] = gamma * (diag ? m_h0[k] : 1.0) * q[k];
+    for (int i = 0; i < m; ++i) {
+        const double beta = m_rho[i] * dot(m_y[i], r);
+        for (size_t k = 0; k < n; ++k) r[k] += m_s[i][k] * (alpha[i] - beta);
+    }
+    for (size_t k = 0; k < n; ++k) r[k] = (!freeMask.empty() && !freeMask[k]) ? 0.0 : -r[k];
+    return r;
+}
+
+OptimizerResult LBFGSBOptimizer::minimize(Objective objective, Gradient gradient, const Vec& x0,
+                                          const Vec& lb, const Vec& ub)
+{
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    OptimizerResult res;
+    m_iter = 0;
+    m_s.clear();
+    m_y.clear();
+    m_rho.clear();
+    m_hist.clear();
+    int nObj = 0, nGrad = 0;
+    auto fobj = [&](const Vec& x) { ++nObj; return objective(x); };
+    auto fgrad = [&](const Vec& x) { ++nGrad; return gradient(x); };
+    Vec x = project(x0, lb, ub);
+    Vec g = fgrad(x);
+    double f = fobj(x);
+    double pg0 = 0.0;
+    int stall = 0;
+    auto pgnorm = [&](const Vec& xx, const Vec& gg) {
+        Vec t(xx.size());
+        for (size_t i = 0; i < xx.size(); ++i) t[i] = xx[i] - gg[i];
+        const Vec p = project(t, lb, ub);
+        double s = 0.0;
+        for (size_t i = 0; i < xx.size(); ++i) s += (xx[i] - p[i]) * (xx[i] - p[i]);
+        return std::sqrt(s);
+    };
+    m_hist.push_back({0, f, pgnorm(x, g), 0.0,
+                      std::chrono::duration<double>(clk::now() - t0).count(), nObj, nGrad});
+    for (int k = 0; k < m_maxIter; ++k) {
+        m_iter = k + 1;
+        if (m_hasTarget && f <= m_target) {
+            res.converged = true;
+            res.message = "Reached target objective";
+            break;
+        }
+        const double pg = pgnorm(x, g);
+        if (k == 0) pg0 = pg;
+        if (k > 0 && pg < m_ftol * std::max(1.0, pg0)) {
+            res.converged = true;
+            res.message = "Converged: projected gradient below tolerance";
+            break;
+        }
+        std::vector<uint8_t> freeMask;
+        if (m_activeSet) {
+            freeMask.assign(x.size(), 1);
+            for (size_t i = 0; i < x.size(); ++i)
+                if ((x[i] <= lb[i] && g[i] > 0.0) || (x[i] >= ub[i] && g[i] < 0.0)) freeMask[i] = 0;
+        }
+        const Vec d = direction(g, freeMask);
+        // projected Armijo backtracking (lbfgsb_optimizer.cpp:85-144); f(x) is carried
+        // from the previous accepted step instead of being re-evaluated (:100), which
+        // saves one objective call per iteration and leaves the iterates unchanged
+        double alpha = 1.0;
+        const double gd = dot(g, d);
+        Vec xn, gn;
+        double fn = f;
+        if (gd >= 0.0) {
+            res.converged = true;
+            res.message = "Converged: no step taken";
+            break;
+        }
+        bool accepted = false;
+        for (int it = 0; it < m_lsMax && alpha >= 1e-16; ++it) {
+            Vec xt(x.size());
+            for (size_t i = 0; i < x.size(); ++i) xt[i] = x[i] + alpha * d[i];
+            xt = project(xt, lb, ub);
+            double dec = 0.0;
+            for (size_t i = 0; i < x.size(); ++i) dec += g[i] * (xt[i] - x[i]);
+            if (dec >= 0.0) {
+                alpha *= 0.5;
+                continue;
+            }
+            const double ft = fobj(xt);
+            if (ft <= f + m_c1 * dec) {
+                xn = xt;
+                fn = ft;
+                accepted = true;
+                break;
+            }
+            alpha *= 0.5;
+        }
+        if (!accepted) {
+            alpha = 1e-16;
+            xn.resize(x.size());
+            for (size_t i = 0; i < x.size(); ++i) xn[i] = x[i] + alpha * d[i];
+            xn = project(xn, lb, ub);
+            fn = fobj(xn);
+        }
+        gn = fgrad(xn);
+        if (alpha < 1e-8) {
+            if (++stall >= 3) {
+                res.converged = true;
+                res.message = "Converged: line search stalled";
+                break;
+            }
+        } else {
+            stall = 0;
+        }
+        Vec s(x.size()), y(x.size());
+        for (size_t i = 0; i < x.size(); ++i) {
+            s[i] = xn[i] - x[i];
+            y[i] = gn[i] - g[i];
+        }
+        const double sy = dot(s, y);
+        if (sy > 1e-16) {
+            if ((int)m_s.size() >= m_historySize) {
+                m_s.erase(m_s.begin());
+                m_y.erase(m_y.begin());
+                m_rho.erase(m_rho.begin());
+            }
+            m_s.push_back(s);
+            m_y.push_back(y);
+            m_rho.push_back(1.0 / sy);
+        }
+        x = xn;
+        g = gn;
+        f = fn;
+        m_hist.push_back({m_iter, f, pgnorm(x, g), alpha,
+                          std::chrono::duration<double>(clk::now() - t0).count(), nObj, nGrad});
+    }
+    if (res.message.empty()) res.message = "Maximum iterations reached";
+    res.params = x;
+    res.finalValue = f;
+    res.iterations = m_iter;
+    return res;
+}
+
+}  // namespace lithofwd

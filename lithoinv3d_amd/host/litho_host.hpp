@@ -1,0 +1,349 @@
+// litho_host.hpp -- C++ host layer over the lithofwd C ABI, keeping the call surface
+// that LithoInv3D's litho-inversion module uses (ForwardModel::compute /
+// computeJacobian / computeBoth / dataCount / parameterCount, GravityForward and
+// MagneticForward constructors, ObjectiveFunction, JointObjective, LBFGSBOptimizer,
+// ReferenceModelRegularization), so that the GPU path drops in under the same names.
+//
+// Vectors are std::vector<double> and matrices column-major (the layout of Eigen's
+// MatrixXd); INTEGRATION.md shows the Eigen adapter a reference build would add.
+//
+// Differences from the reference, by design:
+//   * the model keeps ONE shared vertex array (the reference keeps one per group mesh
+//     and re-synchronizes shared positions after every update,
+//     lithology_model.cpp:230-247), and stores a contact triangle once with the unit on
+//     each side;
+//   * control-point interpolation moves non-control vertices by the IDW-weighted
+//     DISPLACEMENT of their K = 4 nearest control points; the reference writes the
+//     IDW-weighted control-point POSITIONS (surface_mesh.cpp:610-623), which moves
+//     every non-control vertex even for an unchanged parameter vector;
+//   * gradients are exact: ForwardModel::computeJTr runs the GPU adjoint and maps vertex
+//     gradients to parameters through the transposed parameter -> vertex map, including
+//     the interpolation weights the reference's Jacobian drops
+//     (gravity_forward.cpp:269-288).
+#pragma once
+#include <cstdint>
+#include <functional>
+#include <memory>
+#include <string>
+#include <vector>
+
+struct lf_ctx;
+
+namespace lithofwd {
+
+using Vec = std::vector<double>;
+
+// Column-major dense matrix (Eigen::MatrixXd layout).
+struct Mat {
+    size_t rows = 0, cols = 0;
+    std::vector<double> data;
+    Mat() = default;
+    Mat(size_t r, size_t c) : rows(r), cols(c), data(r * c, 0.0) {}
+    double& operator()(size_t r, size_t c) { return data[c * rows + r]; }
+    double operator()(size_t r, size_t c) const { return data[c * rows + r]; }
+};
+
+// modules/litho-core/include/litho_invert/core/common.h:16-38
+struct GravityPoint {
+    double position[3] = {0, 0, 0};
+    double g_obs = 0.0;
+    double g_std = 0.0;
+};
+struct MagneticPoint {
+    double position[3] = {0, 0, 0};
+    double t_obs = 0.0;
+    double t_std = 0.0;
+};
+using GravityData = std::vector<GravityPoint>;
+using MagneticData = std::vector<MagneticPoint>;
+
+// modules/litho-surface/include/litho_invert/surface/surface_mesh.h (VertexFreedom)
+enum class VertexFreedom : uint8_t { FIXED = 0, Z_ONLY = 1, X_ONLY = 2, Y_ONLY = 3, XYZ_FREE = 8 };
+
+// modules/litho-model/include/litho_invert/litho/litho_group.h:7-34
+enum class RemanentMagnetizationMode { EffectiveSusceptibility = 0, FixedVectorPerGroup = 1, VectorPerGroup = 2 };
+struct LithoGroup {
+    int id = -1;
+    std::string name;
+    double density = 0.0;         // g/cc (signed contrasts allowed here)
+    double susceptibility = 0.0;  // SI
+    double remanence_magnitude = 0.0;
+    double remanence_inclination = 0.0;
+    double remanence_declination = 0.0;
+    double magnetization[3] = {0, 0, 0};
+};
+
+// Lithology model: shared vertices, per-facet (front, back) units, DOF map.
+class LithologyModel {
+public:
+    LithologyModel(std::vector<double> vertices, std::vector<uint32_t> faces,
+                   std::vector<int32_t> front, std::vector<int32_t> back,
+                   std::vector<LithoGroup> groups);
+
+    int groupCount() const { return (int)m_groups.size(); }
+    LithoGroup& group(int g) { return m_groups[g]; }
+    const LithoGroup& group(int g) const { return m_groups[g]; }
+    size_t vertexCount() const { return m_vertices.size() / 3; }
+    size_t faceCount() const { return m_front.size(); }
+    const std::vector<double>& vertices() const { return m_vertices; }
+    const std::vector<uint32_t>& faces() const { return m_faces; }
+    const std::vector<int32_t>& front() const { return m_front; }
+    const std::vector<int32_t>& back() const { return m_back; }
+
+    void setVertexFreedom(size_t v, VertexFreedom f);
+    VertexFreedom vertexFreedom(size_t v) const { return m_freedom[v]; }
+    // Every vertex of a facet that bounds the model (back == -1) becomes FIXED, the
+    // rest takes `interior` (the role of LithologyModel::fixExteriorFaces,
+    // lithology_model.cpp:369-579, for closed voxel models).
+    void fixExteriorVertices(VertexFreedom interior);
+    void setAxisScale(int axis, double s);
+    double axisScale(int axis) const { return m_axisScale[axis]; }
+
+    // Control points on a 3-D grid of stride*cellSize cells, one per populated cell,
+    // the vertex closest to the cell centre (lithology_model.cpp:864-1040); cellSize <= 0
+    // estimates it like the reference (smallest non-zero coordinate step among the
+    // first 300 positions in lexicographic order, at least 1 m).  Non-control vertices
+    // follow their K = 4 nearest control points with inverse-distance weights
+    // (surface_mesh.cpp:544-601).  stride <= 0 turns control points off.
+    uint32_t setControlPointStride(int stride, double cellSize = 0.0);
+    int controlPointStride() const { return m_cpStride; }
+    bool isControlPoint(size_t v) const;
+
+    uint32_t totalDofCount() const { return (uint32_t)m_dofVertex.size(); }
+    int dofVertex(uint32_t d) const { return m_dofVertex[d]; }
+    int dofAxis(uint32_t d) const { return m_dofAxis[d]; }
+    Vec assembleParameterVector() const;
+    // param -> vertices (lithology_model.cpp:210-248 in displacement form)
+    void applyParameterVector(const Vec& params);
+    // vertex gradient (3*n_v) -> parameter gradient (transpose of the map above)
+    Vec pullBackVertexGradient(const Vec& vgrad) const;
+    // The linear map itself, per parameter: parameter d moves coordinate dofAxis(d) of
+    // vertex vert[e] by w[e] (metres per parameter unit) for e in [off[d], off[d+1]).
+    void dofVertexMap(std::vector<int64_t>& off, std::vector<int32_t>& vert,
+                      std::vector<double>& w) const;
+    // params -/+ margin/axisScale (runner.cpp:241-248)
+    void getBounds(const Vec& params, double margin_m, Vec& lower, Vec& upper) const;
+
+private:
+    void rebuildDofs();
+    std::vector<double> m_vertices, m_rest;  // current and initial positions
+    std::vector<uint32_t> m_faces;
+    std::vector<int32_t> m_front, m_back;
+    std::vector<LithoGroup> m_groups;
+    std::vector<VertexFreedom> m_freedom;
+    double m_axisScale[3] = {1.0, 1.0, 1.0};
+    int m_cpStride = 0;
+    std::vector<uint8_t> m_isCP;
+    // non-CP vertex -> (cp vertex, weight) x K, CSR
+    std::vector<int64_t> m_wOff;
+    std::vector<int32_t> m_wCp;
+    std::vector<double> m_w;
+    std::vector<int32_t> m_dofVertex, m_dofAxis;
+};
+
+// modules/litho-forward/include/litho_invert/forward/forward_model.h:6-33
+class ForwardModel {
+public:
+    virtual ~ForwardModel() = default;
+    virtual Vec compute(const Vec& params) = 0;
+    // Default: central finite differences over params, step m_fdStep
+    // (gravity_forward.cpp:298-321).
+    virtual Mat computeJacobian(const Vec& params);
+    virtual void computeBoth(const Vec& params, Vec& predicted, Mat& jacobian);
+    // J^T w without forming J (new; the default forms J).
+    virtual Vec computeJTr(const Vec& params, const Vec& w);
+    void setFiniteDifferenceStep(double h) { m_fdStep = h; }
+    double finiteDifferenceStep() const { return m_fdStep; }
+    virtual size_t dataCount() const = 0;
+    virtual size_t parameterCount() const = 0;
+
+protected:
+    double m_fdStep = 0.01;
+};
+
+enum class Precision { FP64 = 0, FP32C = 1 };
+
+// GPU-backed potential-field forward shared by gravity and magnetics.
+class GpuForwardBase : public ForwardModel {
+public:
+    ~GpuForwardBase() override;
+    size_t dataCount() const override { return m_nData; }
+    size_t parameterCount() const override { return m_model->totalDofCount(); }
+    // J via the adjoint, one station at a time (for small problems and tests).
+    Mat computeJacobian(const Vec& params) override;
+    Vec computeJTr(const Vec& params, const Vec& w) override;
+    const std::string& lastError() const { return m_err; }
+
+protected:
+    GpuForwardBase(std::shared_ptr<LithologyModel> model, std::vector<double> stations,
+                   int want, Precision prec, const double fdir[3], int device);
+    void upload(const Vec& params);
+    void pushProperties();
+    Vec forwardRaw(int which);  // 0 = gz, 1 = tmi
+    virtual void buildProperties(std::vector<double>& dens, std::vector<double>& mag) const = 0;
+
+    std::shared_ptr<LithologyModel> m_model;
+    lf_ctx* m_ctx = nullptr;
+    size_t m_nData = 0;
+    int m_want = 0;
+    std::string m_err;
+};
+
+// modules/litho-forward/include/litho_invert/forward/gravity_forward.h:8-52
+class GravityForward : public GpuForwardBase {
+public:
+    GravityForward(std::shared_ptr<LithologyModel> model, const GravityData& data,
+                   Precision prec = Precision::FP64, int device = 0);
+    Vec compute(const Vec& params) override;
+    // unit-density response of one group at the current geometry (gravity_forward.cpp:132-145)
+    Vec computeGroupUnitResponse(int groupIndex);
+    static constexpr double G_SI = 6.67430e-11;
+    static constexpr double DENSITY_SCALE = 1000.0;
+    static constexpr double M_S2_TO_MGAL = 1e5;
+
+protected:
+    void buildProperties(std::vector<double>& dens, std::vector<double>& mag) const override;
+    int m_unitOnly = -1;
+};
+
+// modules/litho-forward/include/litho_invert/forward/magnetic_forward.h:8-77
+class MagneticForward : public GpuForwardBase {
+public:
+    MagneticForward(std::shared_ptr<LithologyModel> model, const MagneticData& data,
+                    double inc_deg, double dec_deg, double field_nT,
+                    Precision prec = Precision::FP64, bool textbook = false, int device = 0);
+    Vec compute(const Vec& params) override;
+    Vec computeGroupUnitResponse(int groupIndex);
+    void buildGroupMagnetization(int g, double& amp, double dir[3]) const;
+    void setRemanenceMode(RemanentMagnetizationMode m) { m_mode = m; pushProperties(); }
+    const double* earthFieldUnitVector() const { return m_fhat; }
+
+protected:
+    void buildProperties(std::vector<double>& dens, std::vector<double>& mag) const override;
+    double m_fhat[3];
+    double m_field;
+    RemanentMagnetizationMode m_mode = RemanentMagnetizationMode::EffectiveSusceptibility;
+    int m_unitOnly = -1;
+};
+
+// modules/litho-regularization/include/litho_invert/regularization/regularization.h:6-15
+class Regularization {
+public:
+    virtual ~Regularization() = default;
+    virtual double evaluate(const Vec& params) = 0;
+    virtual Vec gradient(const Vec& params) = 0;
+    void setWeight(double w) { m_weight = w; }
+    double weight() const { return m_weight; }
+
+protected:
+    double m_weight = 1.0;
+};
+
+// modules/litho-regularization/src/regularization/reference_model.cpp:16-25
+class ReferenceModelRegularization : public Regularization {
+public:
+    void captureReference(const Vec& params) { m_ref = params; m_captured = true; }
+    double evaluate(const Vec& params) override;
+    Vec gradient(const Vec& params) override;
+
+private:
+    Vec m_ref;
+    bool m_captured = false;
+};
+
+// modules/litho-inversion/src/inversion/objective.cpp:6-103 (gravity data misfit +
+// regularization); the gradient uses computeJTr instead of forming J.
+class ObjectiveFunction {
+public:
+    struct Components {
+        double dataMisfit = 0.0, regularization = 0.0, constraintPenalty = 0.0, total = 0.0;
+    };
+    ObjectiveFunction(std::shared_ptr<ForwardModel> forward, const GravityData& data);
+    void addRegularization(std::shared_ptr<Regularization> r) { m_regs.push_back(std::move(r)); }
+    double dataMisfit(const Vec& params);
+    Vec residuals(const Vec& params);
+    Vec dataMisfitGradient(const Vec& params);
+    double evaluate(const Vec& params) { return evaluateComponents(params).total; }
+    Vec gradient(const Vec& params);
+    Components evaluateComponents(const Vec& params);
+
+private:
+    std::shared_ptr<ForwardModel> m_forward;
+    const GravityData& m_data;
+    std::vector<std::shared_ptr<Regularization>> m_regs;
+};
+
+// modules/litho-inversion/src/inversion/joint_objective.cpp:42-173 (gravity + alpha *
+// magnetic misfit + regularization)
+class JointObjective {
+public:
+    struct JointComponents {
+        double gravityMisfit = 0.0, magneticMisfit = 0.0, regularization = 0.0,
+               constraintPenalty = 0.0, total = 0.0;
+    };
+    JointObjective(std::shared_ptr<ObjectiveFunction> grav) : m_grav(std::move(grav)) {}
+    void setMagnetic(std::shared_ptr<ForwardModel> fwd, const MagneticData& data, double alpha);
+    double evaluate(const Vec& params) { return evaluateComponents(params).total; }
+    Vec gradient(const Vec& params);
+    JointComponents evaluateComponents(const Vec& params);
+    double alphaMag() const { return m_alpha; }
+
+private:
+    std::shared_ptr<ObjectiveFunction> m_grav;
+    std::shared_ptr<ForwardModel> m_mag;
+    const MagneticData* m_magData = nullptr;
+    double m_alpha = 0.0;
+};
+
+// modules/litho-inversion/include/litho_invert/inversion/optimizer.h:7-36
+struct OptimizerResult {
+    Vec params;
+    double finalValue = 0.0;
+    int iterations = 0;
+    bool converged = false;
+    std::string message;
+};
+
+// Projected L-BFGS with Armijo backtracking: modules/litho-inversion/src/inversion/
+// lbfgsb_optimizer.cpp:13-263, with an optional diagonal initial inverse Hessian D:
+// H0 = gamma * D, gamma = s^T y / (y^T D y) (gamma * I with D = I, :40-46), and a first
+// step along -D g.
+class LBFGSBOptimizer {
+public:
+    using Objective = std::function<double(const Vec&)>;
+    using Gradient = std::function<Vec(const Vec&)>;
+    struct IterationRecord {
+        int iteration;
+        double f, pgNorm, alpha, seconds;
+        int objectiveCalls, gradientCalls;
+    };
+    OptimizerResult minimize(Objective objective, Gradient gradient, const Vec& x0,
+                             const Vec& lb, const Vec& ub);
+    void setMaxIterations(int n) { m_maxIter = n; }
+    void setTolerance(double t) { m_ftol = t; }
+    void setHistorySize(int m) { m_historySize = m; }
+    void setArmijoC1(double c) { m_c1 = c; }
+    void setLineSearchMaxIter(int n) { m_lsMax = n; }
+    void setInitialHessianDiagonal(const Vec& d) { m_h0 = d; }
+    // stop once f <= target (used to time "seconds to a target misfit")
+    void setTargetObjective(double f) { m_target = f; m_hasTarget = true; }
+    // Variables sitting on a bound with the gradient pushing outward are left out of the
+    // two-loop recursion and the step (off by default: the reference projects the full
+    // L-BFGS step, which can stop descending once bounds are active).
+    void setActiveSetAware(bool on) { m_activeSet = on; }
+    int iterationCount() const { return m_iter; }
+    const std::vector<IterationRecord>& history() const { return m_hist; }
+
+private:
+    Vec direction(const Vec& g, const std::vector<uint8_t>& freeMask) const;
+    std::vector<Vec> m_s, m_y;
+    std::vector<double> m_rho;
+    Vec m_h0;
+    int m_maxIter = 500, m_historySize = 10, m_lsMax = 50, m_iter = 0;
+    double m_ftol = 1e-6, m_c1 = 1e-4, m_target = 0.0;
+    bool m_hasTarget = false;
+    bool m_activeSet = false;
+    std::vector<IterationRecord> m_hist;
+};
+
+}  // namespace lithofwd

@@ -1,0 +1,306 @@
+"""ctypes binding of the C++ host layer (liblithohost.so): the lithology model's
+parameter map (vertex freedoms, axis scale, control points), the joint gravity +
+magnetic objective on the GPU forward/adjoint, and the projected L-BFGS optimizer
+(modules/litho-inversion/src/inversion/lbfgsb_optimizer.cpp) with an optional diagonal
+initial inverse Hessian."""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import math
+import os
+
+import numpy as np
+
+from . import lithofwd
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+HOST_PATH = os.path.join(_HERE, "liblithohost.so")
+
+FIXED, Z_ONLY, X_ONLY, Y_ONLY, XYZ_FREE = 0, 1, 2, 3, 8
+
+_P = ctypes.c_void_p
+_D = ctypes.POINTER(ctypes.c_double)
+_lib = None
+
+
+class _Options(ctypes.Structure):
+    _fields_ = [("max_iter", ctypes.c_int32), ("history", ctypes.c_int32),
+                ("ftol", ctypes.c_double), ("armijo_c1", ctypes.c_double),
+                ("target", ctypes.c_double), ("has_h0", ctypes.c_int32),
+                ("active_set", ctypes.c_int32)]
+
+
+OBJ_CB = ctypes.CFUNCTYPE(ctypes.c_double, _D, ctypes.c_int64, ctypes.c_void_p)
+GRAD_CB = ctypes.CFUNCTYPE(None, _D, ctypes.c_int64, _D, ctypes.c_void_p)
+
+
+def load_host():
+    global _lib
+    if _lib is not None:
+        return _lib
+    lithofwd.load_library()  # the GPU library it links against, loaded first
+    if not os.path.exists(HOST_PATH):
+        raise lithofwd.LithoFwdUnavailable(f"{HOST_PATH} is missing: run python -m lithoinv3d_amd.build")
+    L = ctypes.CDLL(HOST_PATH)
+    i64 = ctypes.c_int64
+    L.li_last_error.restype = ctypes.c_char_p
+    L.li_model_create.argtypes = [ctypes.POINTER(_P), i64, _P, i64, _P, _P, _P, ctypes.c_int32, _P, _P]
+    L.li_model_destroy.argtypes = [_P]
+    L.li_model_destroy.restype = None
+    L.li_model_fix_exterior.argtypes = [_P, ctypes.c_int]
+    L.li_model_set_freedom.argtypes = [_P, _P]
+    L.li_model_set_axis_scale.argtypes = [_P, ctypes.c_int, ctypes.c_double]
+    L.li_model_set_cp_stride.argtypes = [_P, ctypes.c_int, ctypes.c_double, ctypes.POINTER(i64)]
+    L.li_model_dof_count.argtypes = [_P]
+    L.li_model_dof_count.restype = i64
+    L.li_model_dofs.argtypes = [_P, _P, _P, _P]
+    L.li_model_params.argtypes = [_P, _P]
+    L.li_model_apply.argtypes = [_P, _P, _P]
+    L.li_model_pullback.argtypes = [_P, _P, _P]
+    L.li_model_bounds.argtypes = [_P, _P, ctypes.c_double, _P, _P]
+    L.li_model_dof_map.argtypes = [_P, _P, _P, _P, ctypes.POINTER(i64)]
+    L.li_problem_create.argtypes = [ctypes.POINTER(_P), _P, i64, _P, _P, _P, _P, _P,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+    L.li_problem_destroy.argtypes = [_P]
+    L.li_problem_destroy.restype = None
+    L.li_problem_evaluate.argtypes = [_P, _P, _P]
+    L.li_problem_gradient.argtypes = [_P, _P, _P]
+    L.li_problem_forward.argtypes = [_P, _P, _P, _P]
+    L.li_problem_jacobian.argtypes = [_P, _P, ctypes.c_int, _P]
+    L.li_problem_unit_response.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P]
+    mins = [i64, _P, _P, _P]
+    tail = [ctypes.POINTER(_Options), _P, _P, _P, i64, ctypes.POINTER(i64),
+            ctypes.POINTER(ctypes.c_int32), ctypes.c_char_p, i64]
+    L.li_minimize_callbacks.argtypes = mins + [OBJ_CB, GRAD_CB, _P] + tail
+    L.li_problem_minimize.argtypes = [_P, _P, _P, _P] + tail
+    _lib = L
+    return L
+
+
+def _chk(rc, what):
+    if rc != 0:
+        raise lithofwd.LithoFwdError(f"{what}: {load_host().li_last_error().decode()}")
+
+
+def _a(x, dt=np.float64):
+    return np.ascontiguousarray(x, dtype=dt)
+
+
+def _p(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+class Model:
+    """LithologyModel: shared vertices, (front, back) facets, vertex freedoms, axis
+    scale and control points (litho_host.hpp)."""
+
+    def __init__(self, m):
+        L = load_host()
+        self.src = m
+        self._keep = [_a(m.vertices), _a(m.faces, np.uint32), _a(m.front, np.int32),
+                      _a(m.back, np.int32), _a(m.density), _a(m.susceptibility)]
+        self._h = ctypes.c_void_p()
+        _chk(L.li_model_create(ctypes.byref(self._h), m.n_vertices, *[_p(k) for k in self._keep[:1]],
+                               m.n_faces, *[_p(k) for k in self._keep[1:4]], m.n_units,
+                               *[_p(k) for k in self._keep[4:]]), "li_model_create")
+        self.n_v = m.n_vertices
+
+    def __del__(self):
+        try:
+            if self._h:
+                load_host().li_model_destroy(self._h)
+        except Exception:
+            pass
+
+    def fix_exterior(self, interior=Z_ONLY):
+        _chk(load_host().li_model_fix_exterior(self._h, interior), "fix_exterior")
+
+    def set_freedom(self, freedom):
+        f = _a(freedom, np.uint8)
+        _chk(load_host().li_model_set_freedom(self._h, _p(f)), "set_freedom")
+
+    def set_axis_scale(self, s, axes=(0, 1, 2)):
+        for ax in axes:
+            _chk(load_host().li_model_set_axis_scale(self._h, ax, s), "set_axis_scale")
+
+    def set_cp_stride(self, stride, cell=0.0):
+        n = ctypes.c_int64()
+        _chk(load_host().li_model_set_cp_stride(self._h, stride, cell, ctypes.byref(n)), "cp")
+        return n.value
+
+    @property
+    def n_dof(self):
+        return int(load_host().li_model_dof_count(self._h))
+
+    def dofs(self):
+        n = self.n_dof
+        v = np.zeros(n, np.int32)
+        a = np.zeros(n, np.int32)
+        cp = np.zeros(self.n_v, np.uint8)
+        _chk(load_host().li_model_dofs(self._h, _p(v), _p(a), _p(cp)), "dofs")
+        return v, a, cp.astype(bool)
+
+    def params(self):
+        out = np.zeros(self.n_dof)
+        _chk(load_host().li_model_params(self._h, _p(out)), "params")
+        return out
+
+    def apply(self, params):
+        p = _a(params)
+        v = np.zeros((self.n_v, 3))
+        _chk(load_host().li_model_apply(self._h, _p(p), _p(v)), "apply")
+        return v
+
+    def pullback(self, vgrad):
+        g = _a(vgrad).reshape(-1)
+        out = np.zeros(self.n_dof)
+        _chk(load_host().li_model_pullback(self._h, _p(g), _p(out)), "pullback")
+        return out
+
+    def dof_map(self):
+        """(off, vertex, weight): parameter d moves its axis coordinate of vertex[e] by
+        weight[e] metres per unit, e in [off[d], off[d+1])."""
+        L = load_host()
+        ne = ctypes.c_int64()
+        off = np.zeros(self.n_dof + 1, np.int64)
+        _chk(L.li_model_dof_map(self._h, _p(off), None, None, ctypes.byref(ne)), "dof_map")
+        vert = np.zeros(ne.value, np.int32)
+        w = np.zeros(ne.value)
+        _chk(L.li_model_dof_map(self._h, _p(off), _p(vert), _p(w), ctypes.byref(ne)), "dof_map")
+        return off, vert, w
+
+    def bounds(self, params, margin_m):
+        p = _a(params)
+        lo, hi = np.zeros(len(p)), np.zeros(len(p))
+        _chk(load_host().li_model_bounds(self._h, _p(p), margin_m, _p(lo), _p(hi)), "bounds")
+        return lo, hi
+
+
+@dataclasses.dataclass
+class Result:
+    x: np.ndarray
+    iterations: int
+    message: str
+    history: np.ndarray  # rows: iteration, f, |pg|, alpha, seconds, n_obj, n_grad
+
+    @property
+    def seconds(self):
+        return float(self.history[-1, 4]) if len(self.history) else 0.0
+
+    @property
+    def f(self):
+        return float(self.history[-1, 1]) if len(self.history) else math.nan
+
+
+def _opts(max_iter, history, ftol, c1, target, h0, active_set=False):
+    o = _Options(max_iter, history, ftol, c1, math.nan if target is None else target,
+                 1 if h0 is not None else 0, 1 if active_set else 0)
+    return o
+
+
+def _finish(L, call, n, max_iter, *args):
+    x = np.zeros(n)
+    hist = np.zeros((max_iter + 2, 7))
+    nh = ctypes.c_int64()
+    it = ctypes.c_int32()
+    msg = ctypes.create_string_buffer(256)
+    rc = call(*args, _p(x), _p(hist), max_iter + 2, ctypes.byref(nh), ctypes.byref(it), msg, 256)
+    _chk(rc, "minimize")
+    return Result(x, it.value, msg.value.decode(), hist[:nh.value].copy())
+
+
+def minimize(fun, grad, x0, lb, ub, max_iter=100, history=10, ftol=1e-6, c1=1e-4,
+             target=None, h0=None, active_set=False):
+    """The host L-BFGS-B on Python callables (for tests of the optimizer itself)."""
+    L = load_host()
+    x0, lb, ub = _a(x0), _a(lb), _a(ub)
+    n = len(x0)
+
+    def f_cb(xp, nn, user):
+        return float(fun(np.ctypeslib.as_array(xp, (nn,)).copy()))
+
+    def g_cb(xp, nn, gp, user):
+        g = np.asarray(grad(np.ctypeslib.as_array(xp, (nn,)).copy()), dtype=np.float64)
+        np.ctypeslib.as_array(gp, (nn,))[:] = g
+
+    fcb, gcb = OBJ_CB(f_cb), GRAD_CB(g_cb)
+    h = None if h0 is None else _a(h0)
+    o = _opts(max_iter, history, ftol, c1, target, h, active_set)
+    return _finish(L, L.li_minimize_callbacks, n, max_iter, n, _p(x0), _p(lb), _p(ub), fcb, gcb,
+                   None, ctypes.byref(o), _p(h))
+
+
+class Problem:
+    """Joint gravity + magnetic objective (JointObjective) over the GPU forward and
+    adjoint for one station set: Phi = sum (w_g r_g)^2/2 + alpha sum (w_t r_t)^2/2
+    (+ reference-model term), w = 1/std."""
+
+    def __init__(self, model: Model, stations, gz_obs, gz_std=None, tmi_obs=None, tmi_std=None,
+                 inc=-60.0, dec=0.0, field=50000.0, alpha_mag=1.0, precision="fp64",
+                 ref_weight=0.0, device=0):
+        L = load_host()
+        self.model = model
+        st = _a(stations).reshape(-1, 3)
+        self.n_s = len(st)
+        arrs = [st, _a(gz_obs), None if gz_std is None else _a(gz_std),
+                None if tmi_obs is None else _a(tmi_obs), None if tmi_std is None else _a(tmi_std)]
+        self._keep = arrs
+        self._h = ctypes.c_void_p()
+        _chk(L.li_problem_create(ctypes.byref(self._h), model._h, self.n_s, *[_p(a) for a in arrs],
+                                 inc, dec, field, alpha_mag, 1 if precision == "fp32c" else 0,
+                                 ref_weight, device), "li_problem_create")
+
+    def close(self):
+        if self._h:
+            load_host().li_problem_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def evaluate(self, params):
+        """(gravity misfit, magnetic misfit, regularization, total)"""
+        p = _a(params)
+        c = np.zeros(4)
+        _chk(load_host().li_problem_evaluate(self._h, _p(p), _p(c)), "evaluate")
+        return c
+
+    def gradient(self, params):
+        p = _a(params)
+        g = np.zeros(len(p))
+        _chk(load_host().li_problem_gradient(self._h, _p(p), _p(g)), "gradient")
+        return g
+
+    def forward(self, params, tmi=True):
+        p = _a(params)
+        gz = np.zeros(self.n_s)
+        tm = np.zeros(self.n_s) if tmi else None
+        _chk(load_host().li_problem_forward(self._h, _p(p), _p(gz), _p(tm)), "forward")
+        return gz, tm
+
+    def jacobian(self, params, which="gz"):
+        p = _a(params)
+        J = np.zeros((len(p), self.n_s))  # column-major n_s x n_dof
+        _chk(load_host().li_problem_jacobian(self._h, _p(p), 0 if which == "gz" else 1, _p(J)),
+             "jacobian")
+        return J.T.copy()
+
+    def unit_response(self, group, which="gz"):
+        out = np.zeros(self.n_s)
+        _chk(load_host().li_problem_unit_response(self._h, 0 if which == "gz" else 1, group,
+                                                  _p(out)), "unit_response")
+        return out
+
+    def minimize(self, x0, lb, ub, max_iter=50, history=10, ftol=1e-6, c1=1e-4, target=None,
+                 h0=None, active_set=False):
+        L = load_host()
+        x0, lb, ub = _a(x0), _a(lb), _a(ub)
+        h = None if h0 is None else _a(h0)
+        o = _opts(max_iter, history, ftol, c1, target, h, active_set)
+        return _finish(L, L.li_problem_minimize, len(x0), max_iter, self._h, _p(x0), _p(lb),
+                       _p(ub), ctypes.byref(o), _p(h))

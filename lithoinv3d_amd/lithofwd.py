@@ -24,7 +24,7 @@ TIMING_CLASSES = ("face_setup", "forward", "forward_reduce", "adjoint_pairs",
 EXPORTED = (
     "lf_create", "lf_destroy", "lf_last_error", "lf_set_vertices", "lf_set_vertices_device",
     "lf_set_properties", "lf_forward", "lf_forward_device", "lf_adjoint", "lf_adjoint_device",
-    "lf_sync", "lf_set_timing", "lf_timing", "lf_stats",
+    "lf_sync", "lf_set_timing", "lf_timing", "lf_stats", "lf_colnorm2",
 )
 
 
@@ -95,6 +95,7 @@ def load_library(path: str = LIB_PATH):
     L.lf_set_timing.argtypes = [P, ctypes.c_int]
     L.lf_timing.argtypes = [P, _D, ctypes.POINTER(i64), ctypes.c_int]
     L.lf_stats.argtypes = [P, ctypes.POINTER(i64)]
+    L.lf_colnorm2.argtypes = [P, ctypes.c_int, i64, P, P, P, P, P]
     for name in EXPORTED:
         if name not in ("lf_destroy", "lf_last_error"):
             getattr(L, name).restype = ctypes.c_int
@@ -230,6 +231,21 @@ class ForwardContext:
                                                 ctypes.c_void_p(wt_ptr or None),
                                                 ctypes.c_void_p(grad_ptr)),
                     "lf_adjoint_device")
+
+    def colnorm2(self, axis, dof_off, dof_vertex, dof_weight):
+        """(sum_i (dgz_i/dp_j)^2, sum_i (dTMI_i/dp_j)^2) for parameters p_j that move
+        coordinate `axis` of vertices dof_vertex[dof_off[j]:dof_off[j+1]] by the weights."""
+        off = _c(dof_off, np.int64)
+        vert = _c(dof_vertex, np.int32)
+        w = _c(dof_weight, np.float64)
+        n = len(off) - 1
+        if n < 0 or len(vert) != off[-1] or len(w) != off[-1]:
+            raise ValueError("inconsistent DOF map")
+        og = np.zeros(n)
+        ot = np.zeros(n)
+        self._check(self._lib.lf_colnorm2(self._ctx, int(axis), n, _ptr(off), _ptr(vert), _ptr(w),
+                                          _ptr(og), _ptr(ot)), "lf_colnorm2")
+        return og, ot
 
     def sync(self):
         self._check(self._lib.lf_sync(self._ctx), "lf_sync")

@@ -90,3 +90,37 @@ def test_adjoint_contact_model(oracle):
         e = rel_err(g, gref)
         print(f"contact adjoint {prec}: rel err {e:.2e}")
         assert e < tol
+
+
+def test_colnorm2_vs_dense_jacobian(oracle):
+    """Per-DOF squared column norms (DOF = weighted combination of vertex z) against the
+    reference's per-face FD vertex Jacobian (oracle) contracted with the same map."""
+    rng = np.random.default_rng(31)
+    labels = rng.integers(0, 3, size=(5, 4, 3)).astype(np.int32)
+    m = synth.label_grid_model(labels, (-62.5, -50.0, -31.0), (25.0, 25.0, 25.0),
+                               [0.3, -0.2, 0.1], [0.02, 0.005, 0.03])
+    st = synth.topo_stations(9, 160.0)
+    fh = synth.field_unit_vector(-60.0, 5.0)
+    om = oracle.OracleModel.from_litho(m, 50000.0, fh)
+    Jg = om.gravity_vertex_jacobian(st)[:, :, 2]
+    Jm = om.magnetic_vertex_jacobian(fh, st)[:, :, 2]
+    # 30 parameters, each moving 1-4 random vertices with random weights
+    n = 30
+    sizes = rng.integers(1, 5, n)
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    vert = rng.integers(0, m.n_vertices, off[-1]).astype(np.int32)
+    w = rng.uniform(0.2, 10.0, off[-1])
+    A = np.zeros((m.n_vertices, n))
+    for j in range(n):
+        for e in range(off[j], off[j + 1]):
+            A[vert[e], j] += w[e]
+    ref_g = ((Jg @ A) ** 2).sum(0)
+    ref_m = ((Jm @ A) ** 2).sum(0)
+    for prec, tol in (("fp64", 5e-4), ("fp32c", 2e-3)):
+        say(f"colnorm2 {prec}")
+        with ForwardContext(m, st, fh, 50000.0, precision=prec) as ctx:
+            cg, cm = ctx.colnorm2(2, off, vert, w)
+        eg = np.abs(cg - ref_g).max() / ref_g.max()
+        em = np.abs(cm - ref_m).max() / ref_m.max()
+        print(f"colnorm2 {prec}: rel {eg:.2e} (gz), {em:.2e} (tmi)")
+        assert eg < tol and em < tol

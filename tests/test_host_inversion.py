@@ -1,0 +1,118 @@
+"""C++ host layer on the CPU: the projected L-BFGS optimizer on problems with known
+solutions, the diagonal initial-Hessian hook, and the parameter <-> vertex map
+(freedoms, axis scale, control points) with its transpose."""
+import numpy as np
+import pytest
+
+from lithoinv3d_amd import inversion, synth
+
+
+def test_bounded_quadratic_exact_solution():
+    rng = np.random.default_rng(0)
+    n = 40
+    a = np.exp(rng.uniform(-2, 2, n))
+    c = rng.normal(0, 3, n)
+    lb, ub = -np.ones(n), np.ones(n) * 2.0
+    f = lambda x: 0.5 * np.sum(a * (x - c) ** 2)
+    g = lambda x: a * (x - c)
+    # the reference algorithm projects the full L-BFGS step: it descends, stays feasible
+    # and may stall once bounds are active (lbfgsb_optimizer.cpp:103-137)
+    r = inversion.minimize(f, g, np.zeros(n), lb, ub, max_iter=200, ftol=1e-10)
+    assert (r.x >= lb).all() and (r.x <= ub).all()
+    assert np.all(np.diff(r.history[:, 1]) <= 1e-12)
+    assert r.f < f(np.zeros(n))
+    # the active-set variant reaches the exact bounded minimiser
+    r2 = inversion.minimize(f, g, np.zeros(n), lb, ub, max_iter=200, ftol=1e-10, active_set=True)
+    np.testing.assert_allclose(r2.x, np.clip(c, lb, ub), atol=1e-6)
+
+
+def test_rosenbrock_unbounded_region():
+    def f(x):
+        return float(np.sum(100.0 * (x[1:] - x[:-1] ** 2) ** 2 + (1 - x[:-1]) ** 2))
+
+    def g(x):
+        gr = np.zeros_like(x)
+        gr[:-1] = -400 * x[:-1] * (x[1:] - x[:-1] ** 2) - 2 * (1 - x[:-1])
+        gr[1:] += 200 * (x[1:] - x[:-1] ** 2)
+        return gr
+
+    x0 = np.full(6, -1.2)
+    r = inversion.minimize(f, g, x0, np.full(6, -5.0), np.full(6, 5.0), max_iter=500, ftol=1e-12)
+    np.testing.assert_allclose(r.x, np.ones(6), atol=1e-4)
+
+
+def test_diagonal_initial_hessian():
+    """On f = 1/2 sum a_i x_i^2 the exact inverse diagonal gives a one-step solve; the
+    plain gamma*I start needs many iterations."""
+    rng = np.random.default_rng(1)
+    n = 200
+    a = np.exp(rng.uniform(-4, 4, n))
+    x0 = rng.normal(size=n)
+    f = lambda x: 0.5 * np.sum(a * x * x)
+    g = lambda x: a * x
+    lo, hi = np.full(n, -10.0), np.full(n, 10.0)
+    plain = inversion.minimize(f, g, x0, lo, hi, max_iter=300, ftol=1e-8)
+    pre = inversion.minimize(f, g, x0, lo, hi, max_iter=300, ftol=1e-8, h0=1.0 / a)
+    assert pre.iterations <= 3
+    assert plain.iterations > 10 * pre.iterations
+    np.testing.assert_allclose(pre.x, 0.0, atol=1e-8)
+
+
+def test_target_objective_stop():
+    f = lambda x: float(np.sum(x * x))
+    g = lambda x: 2 * x
+    r = inversion.minimize(f, g, np.ones(5) * 3, -np.ones(5) * 9, np.ones(5) * 9, target=1.0)
+    assert r.message.startswith("Reached target")
+    assert r.f <= 1.0
+
+
+def small_model():
+    rng = np.random.default_rng(3)
+    labels = rng.integers(0, 3, size=(8, 7, 5)).astype(np.int32)
+    return synth.label_grid_model(labels, (-100.0, -87.5, -31.0), (25.0, 25.0, 25.0),
+                                  [0.2, -0.1, 0.3], [0.01, 0.002, 0.03])
+
+
+def test_dof_map_z_only_roundtrip():
+    m = small_model()
+    M = inversion.Model(m)
+    M.fix_exterior(inversion.Z_ONLY)
+    M.set_axis_scale(10.0)
+    ext = np.zeros(m.n_vertices, bool)
+    ext[m.faces[m.back < 0].ravel()] = True
+    assert M.n_dof == int((~ext).sum())
+    v, a, _ = M.dofs()
+    assert (a == 2).all() and not ext[v].any()
+    p = M.params()
+    np.testing.assert_allclose(p, m.vertices[v, 2] / 10.0)
+    np.testing.assert_allclose(M.apply(p), m.vertices)  # identity at the start
+    p2 = p + 0.5
+    V2 = M.apply(p2)
+    np.testing.assert_allclose(V2[v, 2], m.vertices[v, 2] + 5.0)
+    np.testing.assert_allclose(np.delete(V2, v, 0), np.delete(m.vertices, v, 0))
+
+
+@pytest.mark.parametrize("stride", [2, 4])
+def test_control_points_map_and_transpose(stride):
+    m = small_model()
+    M = inversion.Model(m)
+    M.fix_exterior(inversion.Z_ONLY)
+    M.set_axis_scale(10.0)
+    n_all = M.n_dof
+    n = M.set_cp_stride(stride, 25.0)
+    assert 0 < n < n_all
+    v, a, cp = M.dofs()
+    assert cp[v].all()
+    p0 = M.params()
+    np.testing.assert_allclose(M.apply(p0), m.vertices, atol=1e-12)  # displacement form
+    rng = np.random.default_rng(stride)
+    dp = rng.normal(size=n)
+    dV = M.apply(p0 + dp) - M.apply(p0)
+    # every vertex moves by a convex combination of control-point displacements
+    assert np.abs(dV[:, 2]).max() <= np.abs(dp).max() * 10.0 + 1e-9
+    vg = rng.normal(size=(m.n_vertices, 3))
+    # <vg, A dp> == <A^T vg, dp>
+    assert np.sum(vg * dV) == pytest.approx(M.pullback(vg) @ dp, rel=1e-12)
+    lo, hi = M.bounds(p0, 200.0)
+    np.testing.assert_allclose(hi - p0, 20.0)
+    np.testing.assert_allclose(p0 - lo, 20.0)
