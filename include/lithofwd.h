@@ -134,6 +134,28 @@ int lf_timing(lf_ctx* ctx, double* ms, int64_t* launches, int n_classes);
  * branch in FP32C (0 in FP64 mode). */
 int lf_stats(lf_ctx* ctx, int64_t* fp64_edges);
 
+/* ---- learned diagonal initial inverse Hessian for L-BFGS-B --------------------------
+ * No counterpart in the reference: it replaces H0 = gamma*I of
+ * modules/litho-inversion/src/inversion/lbfgsb_optimizer.cpp:40-62 by gamma*D with D
+ * predicted per parameter.  A 16 -> 64 -> 64 -> 1 GELU MLP; W1 and W2 are used in BF16
+ * on MFMA with FP32 accumulation; the FP32 master parameters are laid out as
+ * W1[16][64], b1[64], W2[64][64], b2[64], w3[64], b3 (lf_mlp_param_count() floats).
+ * Inputs x are n*16 floats (row-major), targets and outputs n floats; host pointers. */
+typedef struct lf_mlp lf_mlp;
+int64_t lf_mlp_param_count(void);
+int lf_mlp_create(lf_mlp** mlp, int device, uint64_t seed);
+void lf_mlp_destroy(lf_mlp* mlp);
+const char* lf_mlp_last_error(const lf_mlp* mlp);
+int lf_mlp_set_params(lf_mlp* mlp, const float* params);
+int lf_mlp_get_params(lf_mlp* mlp, float* params);
+int lf_mlp_forward(lf_mlp* mlp, int64_t n, const float* x, float* y);
+/* loss = sum_s (y_s - t_s)^2 / n and its gradient with respect to every parameter */
+int lf_mlp_loss_grad(lf_mlp* mlp, int64_t n, const float* x, const float* t, float* loss,
+                     float* grad);
+/* `steps` full-batch Adam steps (beta 0.9/0.999) on the MSE; losses[k] = loss before step k */
+int lf_mlp_train(lf_mlp* mlp, int64_t n, const float* x, const float* t, int steps, float lr,
+                 float* losses);
+
 #ifdef __cplusplus
 }
 #endif

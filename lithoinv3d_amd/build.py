@@ -37,7 +37,7 @@ CXX = os.environ.get("CXX", "g++")
 
 
 def sources():
-    return [os.path.join(CSRC, "lithofwd.hip")]
+    return [os.path.join(CSRC, "lithofwd.hip"), os.path.join(CSRC, "mlp.hip")]
 
 
 def build_host(verbose: bool = False) -> str:

@@ -1,0 +1,262 @@
+"""Inversion experiments on the seeded contact models.
+
+  * ``contact_problem``   a four-unit contact model, its control-point parameters
+                          (Z only, stride 4, axis scale 10 m, exterior vertices fixed,
+                          bounds +-200 m) and synthetic data: the forward of a smoothly
+                          perturbed "true" interface geometry plus Gaussian noise
+                          (0.05 mGal, 1 nT).
+  * ``training_set``      features and exact targets (log Gauss-Newton diagonal from the
+                          GPU column-norm kernel) over many seeded models.
+  * ``compare_solvers``   plain FP64, plain FP32C and learned-H0 FP32C L-BFGS-B runs on one
+                          held-out model; the learned and FP32C runs stop at the plain FP64
+                          run's final misfit, and report iterations and seconds to it.
+
+CLI:  python -m lithoinv3d_amd.experiments learned-h0 --train 200 --test 20 --out FILE
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import sys
+import time
+
+import numpy as np
+
+from . import inversion, mlp, synth
+from .lithofwd import ForwardContext
+
+AXIS_SCALE = 10.0
+SIGMA_GZ, SIGMA_TMI = 0.05, 1.0
+
+
+def log(msg):
+    print(f"[exp] {msg}", file=sys.stderr, flush=True)
+
+
+def true_displacement(xy, extent, seed, amp=40.0, n_bumps=6, width=300.0):
+    """Smooth vertical displacement field (m): Gaussian bumps of +-amp, tapered to zero
+    within one bump width of the model sides."""
+    rng = np.random.default_rng(seed)
+    half = extent / 2.0
+    dz = np.zeros(len(xy))
+    for _ in range(n_bumps):
+        c = rng.uniform(-half + width, half - width, 2)
+        a = rng.uniform(-amp, amp)
+        dz += a * np.exp(-((xy - c) ** 2).sum(1) / (2 * width ** 2))
+    taper = np.clip((half - np.abs(xy)).min(1) / width, 0.0, 1.0)
+    return dz * taper
+
+
+def contact_problem(seed, shape=(64, 64, 32), n_st=128, stride=4, noise_seed=None,
+                    perturb=True):
+    cfg = synth.config_contact(shape=shape, n_st=n_st, seed=seed, prop_seed=seed + 100000)
+    m, st = cfg["model"], cfg["stations"]
+    M = inversion.Model(m)
+    M.fix_exterior(inversion.Z_ONLY)
+    M.set_axis_scale(AXIS_SCALE)
+    n = M.set_cp_stride(stride, 25.0)
+    v, _, _ = M.dofs()
+    p0 = M.params()
+    extent = shape[0] * 25.0
+    ptrue = p0 + (true_displacement(m.vertices[v, :2], extent, seed + 7) / AXIS_SCALE
+                  if perturb else 0.0)
+    lo, hi = M.bounds(p0, 200.0)
+    ptrue = np.clip(ptrue, lo, hi)
+    return dict(model=m, M=M, stations=st, p0=p0, ptrue=ptrue, lo=lo, hi=hi, n_dof=n,
+                dof_vertex=v, cfg=cfg, seed=seed,
+                noise_seed=seed + 3 if noise_seed is None else noise_seed)
+
+
+def synth_data(prob, precision="fp64"):
+    st = prob["stations"]
+    P = inversion.Problem(prob["M"], st, np.zeros(len(st)), tmi_obs=np.zeros(len(st)),
+                          precision=precision)
+    gz, tm = P.forward(prob["ptrue"])
+    P.close()
+    rng = np.random.default_rng(prob["noise_seed"])
+    gz = gz + rng.normal(0, SIGMA_GZ, len(gz))
+    tm = tm + rng.normal(0, SIGMA_TMI, len(tm))
+    prob["gz_obs"], prob["tmi_obs"] = gz, tm
+    return gz, tm
+
+
+def make_problem(prob, precision):
+    st = prob["stations"]
+    n = len(st)
+    return inversion.Problem(prob["M"], st, prob["gz_obs"], np.full(n, SIGMA_GZ),
+                             prob["tmi_obs"], np.full(n, SIGMA_TMI), alpha_mag=1.0,
+                             precision=precision)
+
+
+def features_targets(prob, with_targets=True):
+    m, M = prob["model"], prob["M"]
+    dmap = M.dof_map()
+    X = mlp.dof_features(m, dmap, prob["dof_vertex"], prob["stations"], AXIS_SCALE)
+    t = None
+    if with_targets:
+        # the geometry the solver starts from
+        M.apply(prob["p0"])
+        fh = synth.field_unit_vector(-60.0, 0.0)
+        with ForwardContext(m, prob["stations"], fh, 50000.0, precision="fp32c") as ctx:
+            d = mlp.gn_diagonal(ctx, dmap, 2, SIGMA_GZ, SIGMA_TMI, 1.0)
+        t = np.log(np.maximum(d, 1e-30))
+    return X, t
+
+
+def training_set(seeds, shape=(64, 64, 32), n_st=128):
+    Xs, ts = [], []
+    for k, s in enumerate(seeds):
+        t0 = time.time()
+        prob = contact_problem(s, shape, n_st, perturb=False)
+        X, t = features_targets(prob)
+        Xs.append(X)
+        ts.append(t)
+        log(f"training model {k + 1}/{len(seeds)} seed {s}: {len(t)} parameters "
+            f"({time.time() - t0:.1f}s)")
+    return np.concatenate(Xs), np.concatenate(ts)
+
+
+def train_mlp(X, t, steps=3000, lr=2e-3, seed=0):
+    norm = mlp.Normalizer.fit(X, t)
+    net = mlp.MLP(seed=seed)
+    losses = net.train(norm.x(X), norm.t(t), steps=steps, lr=lr)
+    return net, norm, losses
+
+
+def run_solver(prob, precision, max_iter, target=None, h0=None):
+    P = make_problem(prob, precision)
+    t0 = time.perf_counter()
+    r = P.minimize(prob["p0"], prob["lo"], prob["hi"], max_iter=max_iter, target=target, h0=h0)
+    wall = time.perf_counter() - t0
+    P.close()
+    return r, wall
+
+
+def time_to(r, target):
+    """(iterations, seconds) at the first iterate with f <= target, or None."""
+    ok = np.nonzero(r.history[:, 1] <= target)[0]
+    if len(ok) == 0:
+        return None
+    k = ok[0]
+    return int(r.history[k, 0]), float(r.history[k, 4])
+
+
+def compare_solvers(prob, net=None, norm=None, max_iter=40, exact_diag=False):
+    synth_data(prob)
+    out = {"seed": prob["seed"], "n_dof": prob["n_dof"]}
+    r64, w64 = run_solver(prob, "fp64", max_iter)
+    f_star = r64.f
+    out["fp64"] = dict(iterations=r64.iterations, seconds=r64.seconds, f_final=f_star,
+                       f0=float(r64.history[0, 1]), message=r64.message)
+    r32, _ = run_solver(prob, "fp32c", 3 * max_iter, target=f_star)
+    hit = time_to(r32, f_star)
+    out["fp32c"] = dict(iterations=r32.iterations, seconds=r32.seconds, f_final=r32.f,
+                        reached=hit is not None, to_target=hit)
+    if net is not None:
+        t0 = time.perf_counter()
+        X, _ = features_targets(prob, with_targets=False)
+        pred = norm.t_inv(net.forward(norm.x(X)))
+        h0 = np.exp(-pred.astype(np.float64))
+        t_h0 = time.perf_counter() - t0
+        rl, _ = run_solver(prob, "fp32c", 3 * max_iter, target=f_star, h0=h0)
+        hit = time_to(rl, f_star)
+        out["learned_fp32c"] = dict(iterations=rl.iterations, seconds=rl.seconds + t_h0,
+                                    h0_seconds=t_h0, f_final=rl.f, reached=hit is not None,
+                                    to_target=None if hit is None else (hit[0], hit[1] + t_h0))
+        rl64, _ = run_solver(prob, "fp64", 3 * max_iter, target=f_star, h0=h0)
+        hit = time_to(rl64, f_star)
+        out["learned_fp64"] = dict(iterations=rl64.iterations, seconds=rl64.seconds + t_h0,
+                                   f_final=rl64.f, reached=hit is not None,
+                                   to_target=None if hit is None else (hit[0], hit[1] + t_h0))
+    if exact_diag:
+        _, t = features_targets(prob)
+        re, _ = run_solver(prob, "fp32c", 3 * max_iter, target=f_star, h0=np.exp(-t))
+        hit = time_to(re, f_star)
+        out["exact_diag_fp32c"] = dict(iterations=re.iterations, seconds=re.seconds,
+                                       reached=hit is not None, to_target=hit)
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    a = sub.add_parser("learned-h0")
+    a.add_argument("--train", type=int, default=200)
+    a.add_argument("--test", type=int, default=20)
+    a.add_argument("--shape", type=int, nargs=3, default=(64, 64, 32))
+    a.add_argument("--n-st", type=int, default=128)
+    a.add_argument("--steps", type=int, default=3000)
+    a.add_argument("--max-iter", type=int, default=40)
+    a.add_argument("--out", default=None)
+    b = sub.add_parser("invert")
+    b.add_argument("--seed", type=int, default=1)
+    b.add_argument("--shape", type=int, nargs=3, default=(64, 64, 32))
+    b.add_argument("--n-st", type=int, default=128)
+    b.add_argument("--max-iter", type=int, default=40)
+    b.add_argument("--out", default=None)
+    args = ap.parse_args(argv)
+    shape = tuple(args.shape)
+    if args.cmd == "invert":
+        prob = contact_problem(args.seed, shape, args.n_st)
+        res = compare_solvers(prob, max_iter=args.max_iter)
+        log(json.dumps(res))
+        if args.out:
+            json.dump(res, open(args.out, "w"), indent=1)
+        print(json.dumps(res))
+        return
+    t0 = time.time()
+    X, t = training_set(range(1000, 1000 + args.train), shape, args.n_st)
+    t_data = time.time() - t0
+    t1 = time.time()
+    net, norm, losses = train_mlp(X, t, steps=args.steps)
+    t_train = time.time() - t1
+    log(f"trained on {len(t)} parameters from {args.train} models: loss {losses[0]:.4f} -> "
+        f"{losses[-1]:.4f} ({t_train:.1f}s; data {t_data:.1f}s)")
+    results = []
+    for k, s in enumerate(range(2000, 2000 + args.test)):
+        prob = contact_problem(s, shape, args.n_st)
+        # prediction quality on the held-out model
+        Xh, th = features_targets(prob)
+        pred = norm.t_inv(net.forward(norm.x(Xh)))
+        r = compare_solvers(prob, net, norm, max_iter=args.max_iter)
+        r["log_diag_rmse"] = float(np.sqrt(np.mean((pred - th) ** 2)))
+        r["log_diag_std"] = float(np.std(th))
+        results.append(r)
+        log(f"held-out {k + 1}/{args.test}: {json.dumps(r)}")
+
+    def med(key, field):
+        vals = [r[key][field] for r in results if key in r]
+        return float(np.median(vals)) if vals else math.nan
+
+    def med_to(key, idx):
+        vals = [r[key]["to_target"][idx] for r in results if r.get(key, {}).get("to_target")]
+        return float(np.median(vals)) if vals else math.nan
+
+    summary = {
+        "train_models": args.train, "test_models": args.test, "train_samples": int(len(t)),
+        "train_loss_first_last": [float(losses[0]), float(losses[-1])],
+        "train_seconds": t_train, "data_seconds": t_data,
+        "median_fp64_seconds": med("fp64", "seconds"),
+        "median_fp64_iterations": med("fp64", "iterations"),
+        "median_fp32c_seconds_to_target": med_to("fp32c", 1),
+        "median_fp32c_iterations_to_target": med_to("fp32c", 0),
+        "median_learned_fp32c_seconds_to_target": med_to("learned_fp32c", 1),
+        "median_learned_fp32c_iterations_to_target": med_to("learned_fp32c", 0),
+        "median_learned_fp64_iterations_to_target": med_to("learned_fp64", 0),
+        "learned_fp32c_reached": sum(1 for r in results if r["learned_fp32c"]["reached"]),
+        "fp32c_reached": sum(1 for r in results if r["fp32c"]["reached"]),
+        "median_time_ratio_learned_fp32c_vs_fp64": float(np.median([
+            r["learned_fp32c"]["to_target"][1] / r["fp64"]["seconds"]
+            for r in results if r["learned_fp32c"]["to_target"]])) if results else math.nan,
+        "median_log_diag_rmse": float(np.median([r["log_diag_rmse"] for r in results])),
+        "median_log_diag_std": float(np.median([r["log_diag_std"] for r in results])),
+    }
+    out = {"summary": summary, "models": results}
+    if args.out:
+        json.dump(out, open(args.out, "w"), indent=1)
+    print(json.dumps(summary))
+
+
+if __name__ == "__main__":
+    main()

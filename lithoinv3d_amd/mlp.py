@@ -1,0 +1,234 @@
+"""Learned diagonal initial inverse Hessian for the projected L-BFGS.
+
+A 16 -> 64 -> 64 -> 1 GELU MLP (trained and evaluated by the lf_mlp_* kernels of
+liblithofwd.so: BF16 MFMA, FP32 accumulation) predicts, per control-point parameter, the
+log of the Gauss-Newton diagonal of the joint misfit,
+
+    d_j = sum_i (dgz_i/dp_j / sigma_gz)^2 + alpha * sum_i (dTMI_i/dp_j / sigma_tmi)^2,
+
+from local geometry; the optimizer then starts from H0 = gamma * diag(exp(-prediction)).
+The objective stays exact, so the plain solve (H0 = gamma * I) remains both the fallback
+and the judge of the gain.  Training targets are the exact column norms produced by the
+GPU forward machinery (lf_colnorm2).
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+
+import numpy as np
+
+from . import lithofwd
+
+N_FEATURES = 16
+FEATURE_NAMES = (
+    "depth_km", "log_area", "n_x", "n_y", "n_z", "log_stations_within_2_depths",
+    "abs_drho", "log_abs_dchi", "log_support", "weight_sum", "log_depth",
+    "log_area_over_depth2", "n_z_squared", "abs_drho_nz", "abs_dchi_nz_scaled",
+    "edge_distance_over_depth",
+)
+
+_lib_ready = False
+
+
+def _lib():
+    global _lib_ready
+    L = lithofwd.load_library()
+    if not _lib_ready:
+        P = ctypes.c_void_p
+        i64 = ctypes.c_int64
+        L.lf_mlp_param_count.restype = i64
+        L.lf_mlp_param_count.argtypes = []
+        L.lf_mlp_create.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.c_uint64]
+        L.lf_mlp_create.restype = ctypes.c_int
+        L.lf_mlp_destroy.argtypes = [P]
+        L.lf_mlp_destroy.restype = None
+        L.lf_mlp_last_error.argtypes = [P]
+        L.lf_mlp_last_error.restype = ctypes.c_char_p
+        for name, args in (("lf_mlp_set_params", [P, P]), ("lf_mlp_get_params", [P, P]),
+                           ("lf_mlp_forward", [P, i64, P, P]),
+                           ("lf_mlp_loss_grad", [P, i64, P, P, P, P]),
+                           ("lf_mlp_train", [P, i64, P, P, ctypes.c_int, ctypes.c_float, P])):
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        _lib_ready = True
+    return L
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _p(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+class MLP:
+    """The GPU MLP (parameters resident on the device, FP32 master copy)."""
+
+    def __init__(self, seed=0, device=0):
+        self.L = _lib()
+        self._h = ctypes.c_void_p()
+        rc = self.L.lf_mlp_create(ctypes.byref(self._h), device, seed)
+        if rc:
+            raise lithofwd.LithoFwdError(self.L.lf_mlp_last_error(None).decode())
+        self.n_params = int(self.L.lf_mlp_param_count())
+
+    def _chk(self, rc, what):
+        if rc:
+            raise lithofwd.LithoFwdError(f"{what}: {self.L.lf_mlp_last_error(self._h).decode()}")
+
+    def close(self):
+        if self._h:
+            self.L.lf_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self):
+        p = np.zeros(self.n_params, np.float32)
+        self._chk(self.L.lf_mlp_get_params(self._h, _p(p)), "get_params")
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = _f32(p)
+        assert p.size == self.n_params
+        self._chk(self.L.lf_mlp_set_params(self._h, _p(p)), "set_params")
+
+    def forward(self, x):
+        x = _f32(x).reshape(-1, N_FEATURES)
+        y = np.zeros(len(x), np.float32)
+        self._chk(self.L.lf_mlp_forward(self._h, len(x), _p(x), _p(y)), "forward")
+        return y
+
+    def loss_grad(self, x, t):
+        x = _f32(x).reshape(-1, N_FEATURES)
+        t = _f32(t)
+        loss = np.zeros(1, np.float32)
+        g = np.zeros(self.n_params, np.float32)
+        self._chk(self.L.lf_mlp_loss_grad(self._h, len(x), _p(x), _p(t), _p(loss), _p(g)),
+                  "loss_grad")
+        return float(loss[0]), g
+
+    def train(self, x, t, steps, lr=1e-3):
+        x = _f32(x).reshape(-1, N_FEATURES)
+        t = _f32(t)
+        losses = np.zeros(steps, np.float32)
+        self._chk(self.L.lf_mlp_train(self._h, len(x), _p(x), _p(t), steps, lr, _p(losses)), "train")
+        return losses
+
+
+# --------------------------------------------------------------------- features
+
+def vertex_areas_normals(m):
+    """Per vertex: one third of the incident facet areas, and the area-weighted normal of
+    the incident facets, plus area-weighted |drho| and |dchi| across them."""
+    V, F = m.vertices, m.faces.astype(np.int64)
+    cr = np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]])
+    area = 0.5 * np.linalg.norm(cr, axis=1)
+    nrm = cr / np.maximum(2 * area, 1e-30)[:, None]
+    rb = np.where(m.back >= 0, m.density[np.maximum(m.back, 0)], 0.0)
+    cb = np.where(m.back >= 0, m.susceptibility[np.maximum(m.back, 0)], 0.0)
+    drho = np.abs(m.density[m.front] - rb)
+    dchi = np.abs(m.susceptibility[m.front] - cb)
+    nv = m.n_vertices
+    A = np.zeros(nv)
+    N = np.zeros((nv, 3))
+    R = np.zeros(nv)
+    C = np.zeros(nv)
+    for k in range(3):
+        np.add.at(A, F[:, k], area / 3)
+        np.add.at(N, F[:, k], nrm * (area / 3)[:, None])
+        np.add.at(R, F[:, k], drho * area / 3)
+        np.add.at(C, F[:, k], dchi * area / 3)
+    return A, N, R, C
+
+
+def dof_features(m, dof_map, dof_vertex, stations, axis_scale):
+    """(n_dof, 16) features per control-point parameter (FEATURE_NAMES)."""
+    off, vert, w = dof_map
+    A, N, R, C = vertex_areas_normals(m)
+    n = len(off) - 1
+    ww = w / axis_scale
+    seg = np.repeat(np.arange(n), np.diff(off))
+
+    def dofsum(vals):
+        out = np.zeros(n) if vals.ndim == 1 else np.zeros((n,) + vals.shape[1:])
+        np.add.at(out, seg, (ww[:, None] * vals[vert]) if vals.ndim > 1 else ww * vals[vert])
+        return out
+
+    Aeff = dofsum(A)
+    Neff = dofsum(N)
+    nn = Neff / np.maximum(np.linalg.norm(Neff, axis=1, keepdims=True), 1e-30)
+    Reff = dofsum(R) / np.maximum(Aeff, 1e-30)
+    Ceff = dofsum(C) / np.maximum(Aeff, 1e-30)
+    P = m.vertices[dof_vertex]
+    ztop = float(np.mean(stations[:, 2]))
+    depth = np.maximum(ztop - P[:, 2], 1.0)
+    # stations within a horizontal radius of two depths (cell-binned count)
+    sxy = stations[:, :2]
+    cnt = np.zeros(n)
+    for lo in range(0, n, 2048):
+        hi = min(n, lo + 2048)
+        d2 = ((P[lo:hi, None, :2] - sxy[None, :, :]) ** 2).sum(-1)
+        cnt[lo:hi] = (d2 <= (2 * depth[lo:hi, None]) ** 2).sum(1)
+    xmin, ymin = sxy.min(0)
+    xmax, ymax = sxy.max(0)
+    edge = np.minimum.reduce([P[:, 0] - xmin, xmax - P[:, 0], P[:, 1] - ymin, ymax - P[:, 1]])
+    support = np.diff(off).astype(np.float64)
+    wsum = np.zeros(n)
+    np.add.at(wsum, seg, ww)
+    X = np.stack([
+        depth / 1000.0,
+        np.log1p(Aeff),
+        nn[:, 0], nn[:, 1], nn[:, 2],
+        np.log1p(cnt),
+        Reff,
+        np.log10(np.maximum(Ceff, 1e-6)),
+        np.log(support),
+        wsum,
+        np.log(depth),
+        np.log(np.maximum(Aeff, 1e-6) / depth ** 2),
+        nn[:, 2] ** 2,
+        Reff * np.abs(nn[:, 2]),
+        Ceff * np.abs(nn[:, 2]) * 100.0,
+        edge / depth,
+    ], axis=1)
+    return X
+
+
+@dataclasses.dataclass
+class Normalizer:
+    mean: np.ndarray
+    std: np.ndarray
+    t_mean: float
+    t_std: float
+
+    @classmethod
+    def fit(cls, X, t):
+        s = X.std(0)
+        s[s < 1e-12] = 1.0
+        return cls(X.mean(0), s, float(t.mean()), float(t.std() or 1.0))
+
+    def x(self, X):
+        return (X - self.mean) / self.std
+
+    def t(self, t):
+        return (t - self.t_mean) / self.t_std
+
+    def t_inv(self, y):
+        return y * self.t_std + self.t_mean
+
+
+def gn_diagonal(ctx, dof_map, axis=2, sigma_gz=0.05, sigma_tmi=1.0, alpha=1.0):
+    """Exact Gauss-Newton diagonal of the joint misfit for the parameters of dof_map."""
+    off, vert, w = dof_map
+    cg, ct = ctx.colnorm2(axis, off, vert, w)
+    return cg / sigma_gz ** 2 + alpha * ct / sigma_tmi ** 2

@@ -124,3 +124,40 @@ def test_colnorm2_vs_dense_jacobian(oracle):
         em = np.abs(cm - ref_m).max() / ref_m.max()
         print(f"colnorm2 {prec}: rel {eg:.2e} (gz), {em:.2e} (tmi)")
         assert eg < tol and em < tol
+
+
+def test_torch_autograd_bridge():
+    import torch
+
+    from lithoinv3d_amd.torch_ops import polyhedral_forward
+
+    c = synth.config0()
+    m = c["model"]
+    st = c["stations"][::11]
+    fh = synth.field_unit_vector(c["inc"], c["dec"])
+    rng = np.random.default_rng(12)
+    wg = rng.normal(size=len(st))
+    wt = rng.normal(size=len(st))
+    say("torch autograd bridge fp64")
+    with ForwardContext(m, st, fh, c["field"], precision="fp64") as ctx:
+        g_ref = ctx.adjoint(wg, wt)
+        gz_ref, tm_ref = ctx.forward()
+        v = torch.tensor(m.vertices, dtype=torch.float64, device="cuda", requires_grad=True)
+        gz, tm = polyhedral_forward(v, ctx)
+        loss = (torch.tensor(wg, device="cuda") * gz).sum() + (torch.tensor(wt, device="cuda") * tm).sum()
+        loss.backward()
+        np.testing.assert_allclose(gz.detach().cpu().numpy(), gz_ref, rtol=0, atol=1e-12)
+        np.testing.assert_allclose(tm.detach().cpu().numpy(), tm_ref, rtol=0, atol=1e-9)
+        np.testing.assert_allclose(v.grad.cpu().numpy(), g_ref, rtol=1e-12, atol=1e-14)
+        # central differences through the autograd function itself
+        h = 0.01
+        for vi, ax in [(3, 2), (100, 0), (400, 1)]:
+            with torch.no_grad():
+                vp = v.detach().clone()
+                vp[vi, ax] += h
+                a, b = polyhedral_forward(vp, ctx)
+                vm = v.detach().clone()
+                vm[vi, ax] -= h
+                c2, d2 = polyhedral_forward(vm, ctx)
+            fd = (wg @ (a - c2).cpu().numpy() + wt @ (b - d2).cpu().numpy()) / (2 * h)
+            assert fd == pytest.approx(v.grad[vi, ax].item(), rel=1e-5, abs=1e-9 * np.abs(g_ref).max())

@@ -1,0 +1,81 @@
+"""The MLP kernels (BF16 MFMA, FP32 accumulation) against the NumPy reference with the
+same BF16 rounding points (oracle/mlp_oracle.py), and a training run.
+
+Tolerances: forward 2e-3 of the output scale; gradient 3e-3 of each parameter block's
+largest entry (FP32 summation order and rare one-ulp BF16 rounding differences of
+intermediate activations)."""
+import numpy as np
+import pytest
+
+import mlp_oracle as M
+from lithoinv3d_amd import mlp as L
+
+pytestmark = pytest.mark.gpu
+
+BLOCKS = [("W1", M.OFF_W1, M.OFF_B1), ("b1", M.OFF_B1, M.OFF_W2), ("W2", M.OFF_W2, M.OFF_B2),
+          ("b2", M.OFF_B2, M.OFF_W3), ("w3", M.OFF_W3, M.OFF_B3), ("b3", M.OFF_B3, M.NPARAM)]
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def net():
+    say("mlp create")
+    n = L.MLP(seed=3)
+    yield n
+    n.close()
+
+
+def test_param_roundtrip(net):
+    p = M.init_params(7).astype(np.float32)
+    net.params = p
+    np.testing.assert_array_equal(net.params, p)
+
+
+@pytest.mark.parametrize("n", [1, 16, 37, 1000])
+def test_forward_matches_reference(net, n):
+    rng = np.random.default_rng(n)
+    p = M.init_params(11).astype(np.float32)
+    net.params = p
+    X = rng.normal(size=(n, 16)).astype(np.float32)
+    say(f"mlp forward n={n}")
+    y = net.forward(X)
+    yr, _ = M.forward(p, X, bf=True)
+    scale = max(1.0, np.abs(yr).max())
+    assert np.abs(y - yr).max() <= 2e-3 * scale
+
+
+@pytest.mark.parametrize("n", [16, 53, 4096])
+def test_loss_grad_matches_reference(net, n):
+    rng = np.random.default_rng(100 + n)
+    p = M.init_params(5).astype(np.float32)
+    net.params = p
+    X = rng.normal(size=(n, 16)).astype(np.float32)
+    t = rng.normal(size=n).astype(np.float32)
+    say(f"mlp loss_grad n={n}")
+    loss, g = net.loss_grad(X, t)
+    lr, gr = M.loss_grad(p, X, t, bf=True)
+    assert loss == pytest.approx(lr, rel=2e-3)
+    for name, a, b in BLOCKS:
+        err = np.abs(g[a:b] - gr[a:b]).max() / max(np.abs(gr[a:b]).max(), 1e-12)
+        print(f"n={n} {name}: rel err {err:.2e}")
+        assert err <= 3e-3, name
+
+
+def test_training_reduces_loss(net):
+    rng = np.random.default_rng(9)
+    n = 20000
+    X = rng.normal(size=(n, 16)).astype(np.float32)
+    a = rng.normal(size=16)
+    t = (np.sin(X @ a * 0.5) + 0.3 * X[:, 0] * X[:, 1]).astype(np.float32)
+    net.params = M.init_params(2).astype(np.float32)
+    say("mlp training")
+    losses = net.train(X, t, steps=400, lr=3e-3)
+    print(f"loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+    assert losses[-1] < 0.15 * losses[0]
+    # the trained network's own loss agrees with the reference at its parameters
+    lr, _ = M.loss_grad(net.params, X, t, bf=True)
+    l2, _ = net.loss_grad(X, t)
+    assert l2 == pytest.approx(lr, rel=5e-3)
