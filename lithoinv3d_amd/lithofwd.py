@@ -26,6 +26,12 @@ EXPORTED = (
     "lf_set_properties", "lf_forward", "lf_forward_device", "lf_adjoint", "lf_adjoint_device",
     "lf_sync", "lf_set_timing", "lf_timing", "lf_stats", "lf_colnorm2",
 )
+# the learned-preconditioner MLP (declared in mlp.py)
+EXPORTED_MLP = (
+    "lf_mlp_param_count", "lf_mlp_create", "lf_mlp_destroy", "lf_mlp_last_error",
+    "lf_mlp_set_params", "lf_mlp_get_params", "lf_mlp_forward", "lf_mlp_loss_grad",
+    "lf_mlp_train",
+)
 
 
 class LithoFwdUnavailable(RuntimeError):

@@ -28,7 +28,7 @@ def test_library_exports_declared_symbols():
     assert {"lf_create", "lf_forward", "lf_adjoint", "lf_destroy"} <= names
     missing = [n for n in sorted(names) if not hasattr(L, n)]
     assert not missing, missing
-    assert set(lithofwd.EXPORTED) == names
+    assert set(lithofwd.EXPORTED) | set(lithofwd.EXPORTED_MLP) == names
 
 
 def directed_edges(F):
