@@ -98,6 +98,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first run one plain device copy of this many MiB (a kernel of known "
+                         "traffic for scaling the HBM byte counters of a profiler run)")
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic.json"),
+                    help="measured HBM bytes per launch of the dominant kernels, if present")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -111,6 +116,14 @@ def main():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
     dev = torch.device("cuda:0")
+    if args.copy_calibration_mib > 0:
+        n = args.copy_calibration_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {args.copy_calibration_mib} MiB read + written once")
+        del src, dst
 
     t0 = time.time()
     cfg = workload.build(args.workload)
@@ -168,6 +181,12 @@ def main():
     dom = max(kern, key=kern.get)
     flops = workload.flop_eq(dom) * pairs
     achieved = flops / (kern[dom] * 1e-3) / 1e12
+    traffic = None
+    if os.path.exists(args.traffic_file):
+        tr = json.load(open(args.traffic_file))
+        rec = tr.get("kernels", {}).get(dom)
+        if rec and tr.get("workload") == args.workload:
+            traffic = rec.get("hbm_bytes_per_launch")
     per_kernel = {
         k: {
             "avg_ms": v,
@@ -235,7 +254,7 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": traffic,
             "flop_eq_per_pair": workload.flop_eq(dom),
         },
         "kernels": per_kernel,

@@ -41,7 +41,9 @@ enum SeedIdx {
 };
 
 struct StationW {
-    double x, y, z, wg, wt, pad0, pad1, pad2;
+    double x, y, z, wg, wt;
+    int64_t skip;  // 1 when both weights are zero (tested with scalar integer ops)
+    double pad0, pad1;
 };
 
 struct AdjointWork {
@@ -277,9 +279,12 @@ __global__ __launch_bounds__(256) void k_adjoint_pairs(const FaceFrame<T>* __res
     for (int k = 0; k < kSeeds; ++k) acc[k] = T(0);
     const int64_t i0 = (int64_t)chunk * st_per_chunk;
     const int64_t i1 = min(n_s, i0 + st_per_chunk);
+    // the next station record is requested (scalar loads) while this one is used
+    StationW Sn = st[i0 < i1 ? i0 : 0];
     for (int64_t i = i0; i < i1; ++i) {
-        const StationW S = st[i];
-        if (S.wg == 0.0 && S.wt == 0.0) continue;
+        const StationW S = Sn;
+        Sn = st[i + 1 < i1 ? i + 1 : i];
+        if (S.skip) continue;
         pair_seeds(F, S.x, S.y, S.z, T(S.wg), T(S.wt), acc);
     }
     if (active) {
@@ -520,7 +525,8 @@ __global__ __launch_bounds__(256) void k_pack_stations(const double* __restrict_
     s.z = sz[i];
     s.wg = wg[i];
     s.wt = wt[i];
-    s.pad0 = s.pad1 = s.pad2 = 0.0;
+    s.skip = (s.wg == 0.0 && s.wt == 0.0) ? 1 : 0;
+    s.pad0 = s.pad1 = 0.0;
     out[i] = s;
 }
 

@@ -131,8 +131,11 @@ __global__ __launch_bounds__(256) void k_forward_f32c(
 
     float gs = 0.f, gc = 0.f, ts = 0.f, tc = 0.f;
     unsigned nfb = 0;
+    // the next facet's record is requested (scalar loads) while this one is computed
+    FaceF32 Fn = faces[f0 < f1 ? f0 : 0];
     for (int64_t f = f0; f < f1; ++f) {
-        const FaceF32& F = faces[f];
+        const FaceF32 F = Fn;
+        Fn = faces[f + 1 < f1 ? f + 1 : f];
         // station offset from vertex a: FP64 difference, one rounding
         const float Px = (float)(ox - F.ax);
         const float Py = (float)(oy - F.ay);
@@ -166,17 +169,19 @@ __global__ __launch_bounds__(256) void k_forward_f32c(
         float lam1 = edge_log2<float>(s1b, s1b + F.l1, db, dc, rho1);
         float lam2 = edge_log2<float>(s1c, s1c + F.l2, dc, da, rho2);
         // FP64 for edges whose line passes close to the station while the station
-        // projects inside the edge (rho < d/32): there the FP32 offsets' absolute
-        // error would dominate the log.
-        const bool fb0 = (-px < 0.f) && (t >= 0.f) && (rho0 * 1024.f < da * db);
-        const bool fb1 = (s1b < 0.f) && (s1b + F.l1 >= 0.f) && (rho1 * 1024.f < db * dc);
-        const bool fb2 = (s1c < 0.f) && (s1c + F.l2 >= 0.f) && (rho2 * 1024.f < dc * da);
-        if (fb0 | fb1 | fb2) {
-            const double* g = geo64 + 6 * f;
-            if (fb0) lam0 = edge_log2_f64(ox, oy, oz, F.ax, F.ay, F.az, g[0], g[1], g[2]);
-            if (fb1) lam1 = edge_log2_f64(ox, oy, oz, g[0], g[1], g[2], g[3], g[4], g[5]);
-            if (fb2) lam2 = edge_log2_f64(ox, oy, oz, g[3], g[4], g[5], F.ax, F.ay, F.az);
-            nfb += (unsigned)fb0 + (unsigned)fb1 + (unsigned)fb2;
+        // projects inside the edge (rho < sqrt(d1 d2)/32): there the FP32 offsets'
+        // absolute error would dominate the log.  Only possible near the facet plane.
+        if (fabsf(pz) < F.zfb) {
+            const bool fb0 = (-px < 0.f) && (t >= 0.f) && (rho0 * 1024.f < da * db);
+            const bool fb1 = (s1b < 0.f) && (s1b + F.l1 >= 0.f) && (rho1 * 1024.f < db * dc);
+            const bool fb2 = (s1c < 0.f) && (s1c + F.l2 >= 0.f) && (rho2 * 1024.f < dc * da);
+            if (fb0 | fb1 | fb2) {
+                const double* g = geo64 + 6 * f;
+                if (fb0) lam0 = edge_log2_f64(ox, oy, oz, F.ax, F.ay, F.az, g[0], g[1], g[2]);
+                if (fb1) lam1 = edge_log2_f64(ox, oy, oz, g[0], g[1], g[2], g[3], g[4], g[5]);
+                if (fb2) lam2 = edge_log2_f64(ox, oy, oz, g[3], g[4], g[5], F.ax, F.ay, F.az);
+                nfb += (unsigned)fb0 + (unsigned)fb1 + (unsigned)fb2;
+            }
         }
         if (GZ) {
             const float es = fmaf(py, lam0, fmaf(dp1, lam1, dp2 * lam2));

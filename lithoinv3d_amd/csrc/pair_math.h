@@ -38,7 +38,8 @@ struct __attribute__((aligned(16))) FaceFrame {
     T e1u, e1v, l1, e2u, e2v, l2;              // unit edges b->c, c->a in the frame; lengths
     T K1, K2;                                  // gz  = K1*sum_e dperp_e*log2Q_e + K2*pz*atan2(N,D)
     T W0, W1, W2, W3;                          // tmi = W0*atan2(N,D) + sum_e W_e*log2Q_e
-    T twoA, pad;                               // 2*area = l0*cv
+    T twoA;                                    // 2*area = l0*cv
+    T zfb;  // max edge length / 60: no edge can need the FP64 branch when |pz| >= zfb
 };
 using FaceF32 = FaceFrame<float>;
 static_assert(sizeof(FaceF32) == 128, "FaceF32 must be 128 bytes");
@@ -268,6 +269,7 @@ __device__ __forceinline__ void store_frame(const FrameVals<double>& fv, const d
         r.e2u = T(0); r.e2v = T(-1); r.l2 = T(1);
         r.K1 = r.K2 = r.W0 = r.W1 = r.W2 = r.W3 = T(0);
         r.twoA = T(1);
+        r.zfb = T(0);
     } else {
         r.ux = T(fv.u[0]); r.uy = T(fv.u[1]); r.uz = T(fv.u[2]);
         r.vx = T(fv.v[0]); r.vy = T(fv.v[1]); r.vz = T(fv.v[2]);
@@ -278,8 +280,11 @@ __device__ __forceinline__ void store_frame(const FrameVals<double>& fv, const d
         r.K1 = T(fv.K1); r.K2 = T(fv.K2);
         r.W0 = T(fv.W0); r.W1 = T(fv.W1); r.W2 = T(fv.W2); r.W3 = T(fv.W3);
         r.twoA = T(fv.twoA);
+        // A straddling edge needs the FP64 branch only if rho^2 * 1024 < d1 d2; with
+        // d1 d2 <= ((len + 2 rho)/2)^2 that requires rho < len/62, and rho >= |pz|.
+        const double lmax = fmax(fv.len[0], fmax(fv.len[1], fv.len[2]));
+        r.zfb = T(lmax / 60.0);
     }
-    r.pad = T(0);
 }
 
 }  // namespace lfk
