@@ -23,6 +23,9 @@ HIP_FLAGS = [
     "-fPIC",
     "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # code object v5: loadable by this image's ROCm 7.2 runtime and by the ROCm 7.0
+    # runtime bundled with PyTorch (one runtime per process, see lithofwd.load_library)
+    "-mcode-object-version=5",
     "-Wall",
     "-Wno-unused-function",
     "-Wno-unused-variable",

@@ -552,6 +552,10 @@ inline int adjoint_init(AdjointWork& w, int64_t n_v, int64_t n_f, int64_t n_s,
     chunks = std::max<int64_t>(chunks, (n_s + 2047) / 2048);
     chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, (n_s + 15) / 16));
     chunks = std::min<int64_t>(chunks, 65535);
+    // keep the seed buffer under 4 GiB: fewer, longer station chunks on large models
+    const size_t seed_bytes = (size_t)kSeeds * std::max<int64_t>(n_f, 1) *
+                              (precision == 1 ? sizeof(float) : sizeof(double));
+    chunks = std::max<int64_t>(1, std::min<int64_t>(chunks, (int64_t)((4ull << 30) / seed_bytes)));
     w.st_per_chunk = std::max<int64_t>(1, (n_s + chunks - 1) / chunks);
     w.n_chunks = (int)std::max<int64_t>(1, (n_s + w.st_per_chunk - 1) / w.st_per_chunk);
     const size_t esz = precision == 1 ? sizeof(float) : sizeof(double);

@@ -79,6 +79,13 @@ def load_library(path: str = LIB_PATH):
         raise LithoFwdUnavailable(
             f"{path} is missing: build it with `python -m lithoinv3d_amd.build` "
             "(there is no CPU fallback)")
+    # PyTorch ships its own HIP runtime (same soname, libamdhip64.so.7).  Loading it
+    # first makes this library bind to that one instance; loading ours first would let
+    # torch bring a second runtime into the process, which then finds no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     try:
         L = ctypes.CDLL(path)
     except OSError as e:  # pragma: no cover - depends on the runtime
