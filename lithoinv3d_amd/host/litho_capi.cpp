@@ -35,6 +35,14 @@ struct Problem {
     std::shared_ptr<ObjectiveFunction> gobj;
     std::shared_ptr<ReferenceModelRegularization> ref;
     std::unique_ptr<JointObjective> joint;
+    std::unique_ptr<FusedJointObjective> fused;  // used when present and enabled
+    bool useFused = true;
+
+    JointObjective::JointComponents components(const Vec& p)
+    {
+        return (fused && useFused) ? fused->evaluateComponents(p) : joint->evaluateComponents(p);
+    }
+    Vec grad(const Vec& p) { return (fused && useFused) ? fused->gradient(p) : joint->gradient(p); }
 };
 
 struct ModelHandle {
@@ -194,6 +202,9 @@ int li_problem_create(void** out, void* model, int64_t n_s, const double* statio
         if (tmi_obs && alpha_mag > 0.0) {
             P->mag = std::make_shared<MagneticForward>(P->model, P->mdata, inc, dec, field, prec);
             P->joint->setMagnetic(P->mag, P->mdata, alpha_mag);
+            P->fused = std::make_unique<FusedJointObjective>(P->model, P->gdata, P->mdata, inc,
+                                                             dec, field, alpha_mag, prec, device);
+            if (P->ref) P->fused->addRegularization(P->ref);
         }
         *out = P.release();
     });
@@ -201,12 +212,29 @@ int li_problem_create(void** out, void* model, int64_t n_s, const double* statio
 
 void li_problem_destroy(void* h) { delete (Problem*)h; }
 
+// 1: the fused one-context objective (default when gravity and magnetic data share the
+// stations); 0: the reference-shaped JointObjective over separate forwards
+int li_problem_set_fused(void* h, int on)
+{
+    return guard([&] { ((Problem*)h)->useFused = on != 0; });
+}
+
+// forward / adjoint passes made by the fused objective so far
+int li_problem_pass_counts(void* h, int64_t* fwd, int64_t* adj)
+{
+    return guard([&] {
+        auto* P = (Problem*)h;
+        *fwd = P->fused ? P->fused->forwardPasses() : 0;
+        *adj = P->fused ? P->fused->adjointPasses() : 0;
+    });
+}
+
 // components: gravity misfit, magnetic misfit, regularization, total
 int li_problem_evaluate(void* h, const double* params, double* comp)
 {
     return guard([&] {
         auto* P = (Problem*)h;
-        const auto c = P->joint->evaluateComponents(Vec(params, params + P->model->totalDofCount()));
+        const auto c = P->components(Vec(params, params + P->model->totalDofCount()));
         comp[0] = c.gravityMisfit;
         comp[1] = c.magneticMisfit;
         comp[2] = c.regularization;
@@ -218,7 +246,7 @@ int li_problem_gradient(void* h, const double* params, double* out)
 {
     return guard([&] {
         auto* P = (Problem*)h;
-        const Vec g = P->joint->gradient(Vec(params, params + P->model->totalDofCount()));
+        const Vec g = P->grad(Vec(params, params + P->model->totalDofCount()));
         std::memcpy(out, g.data(), sizeof(double) * g.size());
     });
 }
@@ -335,8 +363,8 @@ int li_problem_minimize(void* h, const double* x0, const double* lb, const doubl
 {
     auto* P = (Problem*)h;
     const int64_t n = P->model->totalDofCount();
-    auto obj = [&](const Vec& x) { return P->joint->evaluate(x); };
-    auto grad = [&](const Vec& x) { return P->joint->gradient(x); };
+    auto obj = [&](const Vec& x) { return P->components(x).total; };
+    auto grad = [&](const Vec& x) { return P->grad(x); };
     return run_minimize(obj, grad, n, x0, lb, ub, o, h0, x_out, hist, hist_cap, n_hist, iterations,
                         msg, msg_cap);
 }

@@ -661,6 +661,121 @@ JointObjective::JointComponents JointObjective::evaluateComponents(const Vec& pa
     return c;
 }
 
+// ======================================================== fused joint objective
+
+FusedJointObjective::FusedJointObjective(std::shared_ptr<LithologyModel> model,
+                                         const GravityData& gdata, const MagneticData& mdata,
+                                         double inc_deg, double dec_deg, double field_nT,
+                                         double alpha_mag, Precision prec, int device)
+    : m_model(std::move(model)), m_g(gdata), m_m(mdata), m_alpha(alpha_mag)
+{
+    if (gdata.size() != mdata.size())
+        throw std::invalid_argument("FusedJointObjective: gravity and magnetic stations differ");
+    for (size_t i = 0; i < gdata.size(); ++i)
+        for (int k = 0; k < 3; ++k)
+            if (gdata[i].position[k] != mdata[i].position[k])
+                throw std::invalid_argument("FusedJointObjective: gravity and magnetic stations differ");
+    double fh[3];
+    fieldUnitVector(inc_deg, dec_deg, fh);
+    std::vector<double> st;
+    for (auto& p : gdata) st.insert(st.end(), p.position, p.position + 3);
+    const int ng = m_model->groupCount();
+    std::vector<double> dens(ng), mag(3 * ng);
+    for (int g = 0; g < ng; ++g) {
+        dens[g] = m_model->group(g).density;
+        const double amp = m_model->group(g).susceptibility * field_nT / 100.0;  // induced
+        for (int k = 0; k < 3; ++k) mag[3 * g + k] = amp * fh[k];
+    }
+    lf_desc d{};
+    d.n_vertices = (int64_t)m_model->vertexCount();
+    d.n_faces = (int64_t)m_model->faceCount();
+    d.n_stations = (int64_t)gdata.size();
+    d.n_units = ng;
+    d.precision = prec == Precision::FP32C ? LF_FP32C : LF_FP64;
+    d.tmi_mode = LF_TMI_REFERENCE;
+    d.want = LF_WANT_GZ | LF_WANT_TMI;
+    d.device = device;
+    d.vertices = m_model->vertices().data();
+    d.faces = m_model->faces().data();
+    d.front = m_model->front().data();
+    d.back = m_model->back().data();
+    d.stations = st.data();
+    d.density = dens.data();
+    d.magnetization = mag.data();
+    for (int k = 0; k < 3; ++k) d.field_dir[k] = fh[k];
+    if (lf_create(&m_ctx, &d) != 0)
+        throw std::runtime_error(std::string("lf_create: ") + lf_last_error(nullptr));
+}
+
+FusedJointObjective::~FusedJointObjective()
+{
+    if (m_ctx) lf_destroy(m_ctx);
+}
+
+void FusedJointObjective::ensureForward(const Vec& params)
+{
+    if (m_cacheValid && params == m_cacheParams) return;
+    m_model->applyParameterVector(params);
+    check(lf_set_vertices(m_ctx, m_model->vertices().data()), m_ctx, "lf_set_vertices");
+    m_gz.assign(m_g.size(), 0.0);
+    m_tmi.assign(m_m.size(), 0.0);
+    check(lf_forward(m_ctx, m_gz.data(), m_tmi.data()), m_ctx, "lf_forward");
+    ++m_nForward;
+    m_cacheParams = params;
+    m_cacheValid = true;
+}
+
+void FusedJointObjective::forward(const Vec& params, Vec& gz, Vec& tmi)
+{
+    ensureForward(params);
+    gz = m_gz;
+    tmi = m_tmi;
+}
+
+JointObjective::JointComponents FusedJointObjective::evaluateComponents(const Vec& params)
+{
+    ensureForward(params);
+    JointObjective::JointComponents c;
+    for (size_t i = 0; i < m_g.size(); ++i) {
+        const double w = m_g[i].g_std > 0 ? 1.0 / m_g[i].g_std : 1.0;
+        const double r = w * (m_g[i].g_obs - m_gz[i]);
+        c.gravityMisfit += 0.5 * r * r;
+    }
+    if (m_alpha > 0.0)
+        for (size_t i = 0; i < m_m.size(); ++i) {
+            const double w = m_m[i].t_std > 0 ? 1.0 / m_m[i].t_std : 1.0;
+            const double r = w * (m_m[i].t_obs - m_tmi[i]);
+            c.magneticMisfit += 0.5 * r * r;
+        }
+    for (auto& r : m_regs) c.regularization += r->evaluate(params);
+    c.total = c.gravityMisfit + m_alpha * c.magneticMisfit + c.regularization;
+    return c;
+}
+
+Vec FusedJointObjective::gradient(const Vec& params)
+{
+    ensureForward(params);
+    const size_t n = m_g.size();
+    Vec wg(n), wt(n, 0.0);
+    for (size_t i = 0; i < n; ++i) {
+        const double a = m_g[i].g_std > 0 ? 1.0 / m_g[i].g_std : 1.0;
+        wg[i] = -a * a * (m_g[i].g_obs - m_gz[i]);
+        if (m_alpha > 0.0) {
+            const double b = m_m[i].t_std > 0 ? 1.0 / m_m[i].t_std : 1.0;
+            wt[i] = -m_alpha * b * b * (m_m[i].t_obs - m_tmi[i]);
+        }
+    }
+    Vec vg(3 * m_model->vertexCount());
+    check(lf_adjoint(m_ctx, wg.data(), wt.data(), vg.data()), m_ctx, "lf_adjoint");
+    ++m_nAdjoint;
+    Vec g = m_model->pullBackVertexGradient(vg);
+    for (auto& r : m_regs) {
+        const Vec gr = r->gradient(params);
+        for (size_t i = 0; i < g.size(); ++i) g[i] += gr[i];
+    }
+    return g;
+}
+
 // =================================================================== L-BFGS-B
 
 namespace {

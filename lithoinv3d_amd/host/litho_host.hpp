@@ -295,6 +295,38 @@ private:
     double m_alpha = 0.0;
 };
 
+// JointObjective on one GPU context for gravity and magnetic data observed at the same
+// stations: one forward pass yields gz and TMI (the pair geometry is shared) and one
+// adjoint pass takes both weight vectors.  The last forward is cached, so gradient(x)
+// right after evaluate(x) -- what the line search does on an accepted step -- costs
+// only the adjoint.  Same objective as JointObjective (joint_objective.cpp:113-173).
+class FusedJointObjective {
+public:
+    FusedJointObjective(std::shared_ptr<LithologyModel> model, const GravityData& gdata,
+                        const MagneticData& mdata, double inc_deg, double dec_deg,
+                        double field_nT, double alpha_mag, Precision prec, int device = 0);
+    ~FusedJointObjective();
+    void addRegularization(std::shared_ptr<Regularization> r) { m_regs.push_back(std::move(r)); }
+    JointObjective::JointComponents evaluateComponents(const Vec& params);
+    double evaluate(const Vec& params) { return evaluateComponents(params).total; }
+    Vec gradient(const Vec& params);
+    void forward(const Vec& params, Vec& gz, Vec& tmi);
+    int forwardPasses() const { return m_nForward; }
+    int adjointPasses() const { return m_nAdjoint; }
+
+private:
+    void ensureForward(const Vec& params);
+    std::shared_ptr<LithologyModel> m_model;
+    const GravityData& m_g;
+    const MagneticData& m_m;
+    double m_alpha;
+    lf_ctx* m_ctx = nullptr;
+    std::vector<std::shared_ptr<Regularization>> m_regs;
+    Vec m_cacheParams, m_gz, m_tmi;
+    bool m_cacheValid = false;
+    int m_nForward = 0, m_nAdjoint = 0;
+};
+
 // modules/litho-inversion/include/litho_invert/inversion/optimizer.h:7-36
 struct OptimizerResult {
     Vec params;

@@ -65,6 +65,8 @@ def load_host():
                                     ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int]
     L.li_problem_destroy.argtypes = [_P]
     L.li_problem_destroy.restype = None
+    L.li_problem_set_fused.argtypes = [_P, ctypes.c_int]
+    L.li_problem_pass_counts.argtypes = [_P, ctypes.POINTER(i64), ctypes.POINTER(i64)]
     L.li_problem_evaluate.argtypes = [_P, _P, _P]
     L.li_problem_gradient.argtypes = [_P, _P, _P]
     L.li_problem_forward.argtypes = [_P, _P, _P, _P]
@@ -262,6 +264,18 @@ class Problem:
             self.close()
         except Exception:
             pass
+
+    def set_fused(self, on: bool):
+        """True (default with magnetic data): one GPU context for gz and TMI, one adjoint
+        pass, cached forward.  False: the reference-shaped JointObjective over separate
+        GravityForward / MagneticForward objects."""
+        _chk(load_host().li_problem_set_fused(self._h, int(bool(on))), "set_fused")
+
+    def pass_counts(self):
+        f, a = ctypes.c_int64(), ctypes.c_int64()
+        _chk(load_host().li_problem_pass_counts(self._h, ctypes.byref(f), ctypes.byref(a)),
+             "pass_counts")
+        return f.value, a.value
 
     def evaluate(self, params):
         """(gravity misfit, magnetic misfit, regularization, total)"""

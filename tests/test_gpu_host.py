@@ -124,3 +124,30 @@ def test_small_inversion_recovers(prec):
     print(f"{prec}: f0 {f0:.3e} -> {r.f:.3e} in {r.iterations} it ({r.message})")
     assert r.f < 1e-2 * f0
     P.close()
+
+
+def test_fused_objective_matches_reference_shaped():
+    """One GPU context for gz and TMI (fused) against the reference-shaped JointObjective
+    over separate forwards: same value and gradient."""
+    m, st = model_and_stations(8)
+    M = inversion.Model(m)
+    M.fix_exterior(inversion.Z_ONLY)
+    M.set_axis_scale(AXIS_SCALE)
+    n = M.set_cp_stride(2, 25.0)
+    p0 = M.params()
+    rng = np.random.default_rng(4)
+    gz_obs = rng.normal(0, 0.5, len(st))
+    tm_obs = rng.normal(0, 20.0, len(st))
+    say("fused vs separate objective fp64")
+    P = inversion.Problem(M, st, gz_obs, np.full(len(st), 0.05), tm_obs, np.full(len(st), 1.0),
+                          alpha_mag=0.7, ref_weight=0.01)
+    p = p0 + rng.normal(0, 0.3, n)
+    P.set_fused(False)
+    c_sep, g_sep = P.evaluate(p), P.gradient(p)
+    P.set_fused(True)
+    c_fus, g_fus = P.evaluate(p), P.gradient(p)
+    np.testing.assert_allclose(c_fus, c_sep, rtol=1e-11)
+    np.testing.assert_allclose(g_fus, g_sep, rtol=1e-9, atol=1e-9 * np.abs(g_sep).max())
+    f, a = P.pass_counts()
+    assert (f, a) == (1, 1)  # the gradient reused the forward of the evaluation
+    P.close()
