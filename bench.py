@@ -191,7 +191,7 @@ def main():
         k: {
             "avg_ms": v,
             "tflops": workload.flop_eq(k) * pairs / (v * 1e-3) / 1e12,
-            "arith_tflops": workload.OPS[k]["arith"] * pairs / (v * 1e-3) / 1e12,
+            "arith_tflops": workload.arith_flops(k) * pairs / (v * 1e-3) / 1e12,
             "pairs_per_s": pairs / (v * 1e-3),
         }
         for k, v in kern.items()

@@ -73,28 +73,26 @@ struct AdjointLaunch {
     double* grad;
 };
 
-// d log2(Q)/d(s1, s2, d1, d2, rho^2) for the three cases of edge_log2.
+// ln(Q) of edge_ln and its derivatives with respect to (s1, s2, d1, d2, rho^2) in the
+// form of each case: Q = X2/X1, X1/X2 or X1 X2/rho^2 with X = |s| + d.  The forms agree
+// where d^2 = s^2 + rho^2, which the frame parametrization keeps, so the chain rule
+// through it is exact whichever form is differentiated.
 template <class T>
-__device__ __forceinline__ T edge_log2_grad(T s1, T s2, T d1, T d2, T rho2, T& gs1, T& gs2,
-                                            T& gd1, T& gd2, T& grho)
+__device__ __forceinline__ T edge_ln_grad(T s1, T ell, T d1, T d2, T rho2, T& gs1, T& gs2,
+                                          T& gd1, T& gd2, T& grho)
 {
-    const T X1 = fabs(s1) + d1;
-    const T X2 = fabs(s2) + d2;
-    const bool pos1 = s1 >= T(0);
-    const bool neg2 = s2 < T(0);
+    T X1, X2;
+    bool pos1, neg2;
+    const T lam = edge_ln(s1, ell, d1, d2, rho2, X1, X2, pos1, neg2);
     const bool mid = !(pos1 || neg2);
-    T num = neg2 ? X1 : X2;
-    T den = pos1 ? X1 : (neg2 ? X2 : rho2);
-    num = mid ? num * X1 : num;
-    const T k = T(1.4426950408889634);  // 1/ln 2
-    const T iX1 = k * fast_rcp(X1);
-    const T iX2 = k * fast_rcp(X2);
+    const T iX1 = fast_rcp(X1);
+    const T iX2 = fast_rcp(X2);
     gs1 = -iX1;
     gs2 = iX2;
     gd1 = pos1 ? -iX1 : iX1;
     gd2 = neg2 ? -iX2 : iX2;
-    grho = mid ? -k * fast_rcp(rho2) : T(0);
-    return fast_log2(num * fast_rcp(den));
+    grho = mid ? -fast_rcp(rho2) : T(0);
+    return lam;
 }
 
 // One (station, facet) pair: frame-form forward and its reverse sweep for the weights
@@ -130,9 +128,9 @@ __device__ __forceinline__ void pair_seeds(const FaceFrame<T>& F, double ox, dou
     const T rho1 = dp1 * dp1 + pz2;
     const T rho2 = dp2 * dp2 + pz2;
     T g0s1, g0s2, g0d1, g0d2, g0r, g1s1, g1s2, g1d1, g1d2, g1r, g2s1, g2s2, g2d1, g2d2, g2r;
-    const T lam0 = edge_log2_grad(-px, t, da, db, rho0, g0s1, g0s2, g0d1, g0d2, g0r);
-    const T lam1 = edge_log2_grad(s1b, s1b + F.l1, db, dc, rho1, g1s1, g1s2, g1d1, g1d2, g1r);
-    const T lam2 = edge_log2_grad(s1c, s1c + F.l2, dc, da, rho2, g2s1, g2s2, g2d1, g2d2, g2r);
+    const T lam0 = edge_ln_grad(-px, F.l0, da, db, rho0, g0s1, g0s2, g0d1, g0d2, g0r);
+    const T lam1 = edge_ln_grad(s1b, F.l1, db, dc, rho1, g1s1, g1s2, g1d1, g1d2, g1r);
+    const T lam2 = edge_ln_grad(s1c, F.l2, dc, da, rho2, g2s1, g2s2, g2d1, g2d2, g2r);
     const T es = py * lam0 + dp1 * lam1 + dp2 * lam2;
     // ---- coefficient seeds
     acc[S_K1] += lam * es;

@@ -165,9 +165,11 @@ __global__ __launch_bounds__(256) void k_forward_f32c(
         const float s1c = fmaf(dcx, F.e2u, dcy * F.e2v);
         const float rho1 = fmaf(dp1, dp1, pz2);
         const float rho2 = fmaf(dp2, dp2, pz2);
-        float lam0 = edge_log2<float>(-px, t, da, db, rho0);
-        float lam1 = edge_log2<float>(s1b, s1b + F.l1, db, dc, rho1);
-        float lam2 = edge_log2<float>(s1c, s1c + F.l2, dc, da, rho2);
+        float X1, X2;
+        bool p1, n2;
+        float lam0 = edge_ln<float>(-px, F.l0, da, db, rho0, X1, X2, p1, n2);
+        float lam1 = edge_ln<float>(s1b, F.l1, db, dc, rho1, X1, X2, p1, n2);
+        float lam2 = edge_ln<float>(s1c, F.l2, dc, da, rho2, X1, X2, p1, n2);
         // FP64 for edges whose line passes close to the station while the station
         // projects inside the edge (rho < sqrt(d1 d2)/32): there the FP32 offsets'
         // absolute error would dominate the log.  Only possible near the facet plane.
@@ -177,9 +179,9 @@ __global__ __launch_bounds__(256) void k_forward_f32c(
             const bool fb2 = (s1c < 0.f) && (s1c + F.l2 >= 0.f) && (rho2 * 1024.f < dc * da);
             if (fb0 | fb1 | fb2) {
                 const double* g = geo64 + 6 * f;
-                if (fb0) lam0 = edge_log2_f64(ox, oy, oz, F.ax, F.ay, F.az, g[0], g[1], g[2]);
-                if (fb1) lam1 = edge_log2_f64(ox, oy, oz, g[0], g[1], g[2], g[3], g[4], g[5]);
-                if (fb2) lam2 = edge_log2_f64(ox, oy, oz, g[3], g[4], g[5], F.ax, F.ay, F.az);
+                if (fb0) lam0 = edge_ln_f64(ox, oy, oz, F.ax, F.ay, F.az, g[0], g[1], g[2]);
+                if (fb1) lam1 = edge_ln_f64(ox, oy, oz, g[0], g[1], g[2], g[3], g[4], g[5]);
+                if (fb2) lam2 = edge_ln_f64(ox, oy, oz, g[3], g[4], g[5], F.ax, F.ay, F.az);
                 nfb += (unsigned)fb0 + (unsigned)fb1 + (unsigned)fb2;
             }
         }

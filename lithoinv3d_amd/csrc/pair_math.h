@@ -17,14 +17,13 @@
 //    (u along edge a->b, v = n x u, n): p = (px, py, pz).  The three vertex distances,
 //    the solid-angle numerator (2A*h, h = -pz) and denominator, and every edge's
 //    (s1, s2, dperp, rho^2) follow from p and nine in-plane shape numbers, and the
-//    edge log is evaluated without cancellation (edge_log2).
+//    edge log is evaluated without cancellation (edge_ln).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace lfk {
 
-constexpr double kLn2d = 0.69314718055994530942;
 
 // ---------------------------------------------------------------------------- records
 
@@ -36,8 +35,8 @@ struct __attribute__((aligned(16))) FaceFrame {
     T ux, uy, uz, vx, vy, vz, nx, ny, nz;      // facet frame
     T l0, cu, cv;                              // |b-a|; (c-a) in the frame
     T e1u, e1v, l1, e2u, e2v, l2;              // unit edges b->c, c->a in the frame; lengths
-    T K1, K2;                                  // gz  = K1*sum_e dperp_e*log2Q_e + K2*pz*atan2(N,D)
-    T W0, W1, W2, W3;                          // tmi = W0*atan2(N,D) + sum_e W_e*log2Q_e
+    T K1, K2;                                  // gz  = K1*sum_e dperp_e*lnQ_e + K2*pz*atan2(N,D)
+    T W0, W1, W2, W3;                          // tmi = W0*atan2(N,D) + sum_e W_e*lnQ_e
     T twoA;                                    // 2*area = l0*cv
     T zfb;  // max edge length / 60: no edge can need the FP64 branch when |pz| >= zfb
 };
@@ -101,29 +100,47 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float x)
 
 // --------------------------------------------------------------- stable edge log
 
-// log2 of (s2+d2)/(s1+d1) for an edge with s2 = s1 + length > s1, X = |s| + d:
-//   s1 >= 0          : X2 / X1                (no cancellation)
-//   s2 <  0          : X1 / X2                (identity (s+d)(d-s) = rho^2 at both ends)
-//   s1 < 0 <= s2     : X1 * X2 / rho^2        (station projects inside the edge)
-// where rho^2 = dperp^2 + h^2 is the squared distance to the edge line.
-template <class T>
-__device__ __forceinline__ T edge_log2(T s1, T s2, T d1, T d2, T rho2)
+// ln(1 + y) for y >= 0.  u = fl(1 + y); u - 1 is exact for 1 <= u < 2^24 and so is
+// c = y - (u - 1), the rounding error of u; ln(1 + y) = ln u + ln(1 + c/u) ~ ln u + c/u.
+// For u < 2 replacing c/u by c costs at most |c| y (relative 6e-8); for u >= 2 the term is
+// below 1e-7 of ln u and dropped.  ln u from the hardware base-2 log (1 ulp).
+__device__ __forceinline__ float log1p_t(float y)
 {
-    const T X1 = fabs(s1) + d1;
-    const T X2 = fabs(s2) + d2;
-    const bool pos1 = s1 >= T(0);
-    const bool neg2 = s2 < T(0);
-    T num = neg2 ? X1 : X2;
-    T den = pos1 ? X1 : (neg2 ? X2 : rho2);
-    num = (pos1 || neg2) ? num : num * X1;
-    return fast_log2(num * fast_rcp(den));
+    const float u = 1.0f + y;
+    const float c = y - (u - 1.0f);
+    return fmaf(0.693147180559945f, fast_log2(u), u < 2.0f ? c : 0.0f);
+}
+__device__ __forceinline__ double log1p_t(double y) { return log1p(y); }
+
+// ln((s2+d2)/(s1+d1)) for an edge of length ell (s2 = s1 + ell), X = |s| + d, without
+// cancellation even when the edge is far away (the ratio is then 1 + O(ell/d)).  With
+// dd = d2 - d1 = ell (2 s1 + ell) / (d1 + d2) (exact, since d2^2 - d1^2 = 2 s1 ell + ell^2):
+//   s1 >= 0      : Q - 1 = (ell + dd)/X1 = ell (X1 + X2) / (X1 (d1 + d2))
+//   s2 <  0      : Q = X1/X2 (identity (s+d)(d-s) = rho^2 at both ends),
+//                  Q - 1 = (ell - dd)/X2 = ell (X1 + X2) / (X2 (d1 + d2))
+//   s1 < 0 <= s2 : Q = X1 X2 / rho^2 (station projects inside the edge)
+// so every case is ln(1 + y) with y >= 0 formed from positive products and one reciprocal.
+// Also returns X1, X2 and the case flags for the derivative.
+template <class T>
+__device__ __forceinline__ T edge_ln(T s1, T ell, T d1, T d2, T rho2, T& X1, T& X2, bool& pos1,
+                                     bool& neg2)
+{
+    const T s2 = s1 + ell;
+    X1 = fabs(s1) + d1;
+    X2 = fabs(s2) + d2;
+    pos1 = s1 >= T(0);
+    neg2 = s2 < T(0);
+    const bool mid = !(pos1 || neg2);
+    const T num = mid ? X1 * X2 - rho2 : ell * (X1 + X2);
+    const T den = mid ? rho2 : (pos1 ? X1 : X2) * (d1 + d2);
+    return log1p_t(num * fast_rcp(den));
 }
 
-// log2 of the same ratio in FP64 from original coordinates (used by the FP32C forward
-// for edges whose line passes close to the station).
-__device__ __forceinline__ float edge_log2_f64(double ox, double oy, double oz,
-                                               double p1x, double p1y, double p1z,
-                                               double p2x, double p2y, double p2z)
+// The same logarithm in FP64 from original coordinates (used by the FP32C forward for
+// edges whose line passes close to the station).
+__device__ __forceinline__ float edge_ln_f64(double ox, double oy, double oz, double p1x,
+                                             double p1y, double p1z, double p2x, double p2y,
+                                             double p2z)
 {
     const double ex = p2x - p1x, ey = p2y - p1y, ez = p2z - p1z;
     const double il = 1.0 / sqrt(ex * ex + ey * ey + ez * ez);
@@ -141,7 +158,7 @@ __device__ __forceinline__ float edge_log2_f64(double ox, double oy, double oz,
     if (s1 >= 0.0) { num = X2; den = X1; }
     else if (s2 < 0.0) { num = X1; den = X2; }
     else { num = X1 * X2; den = rho2; }
-    return (float)(log(num / den) * 1.4426950408889634);
+    return (float)log(num / den);
 }
 
 // ---------------------------------------------------------- forward-mode dual number
@@ -242,12 +259,12 @@ __device__ __forceinline__ void facet_frame(const S P[3][3], double drho, const 
     o.e2u = o.eh[2][0] * o.u[0] + o.eh[2][1] * o.u[1] + o.eh[2][2] * o.u[2];
     o.e2v = o.eh[2][0] * o.v[0] + o.eh[2][1] * o.v[1] + o.eh[2][2] * o.v[2];
     o.l2 = o.len[2];
-    o.K1 = o.kz * S(kLn2d);
+    o.K1 = o.kz;
     o.K2 = S(2.0) * o.kz;
     o.W0 = S(2.0) * o.wom;
-    o.W1 = o.we[0] * S(kLn2d);
-    o.W2 = o.we[1] * S(kLn2d);
-    o.W3 = o.we[2] * S(kLn2d);
+    o.W1 = o.we[0];
+    o.W2 = o.we[1];
+    o.W3 = o.we[2];
     o.twoA = o.l0 * o.cv;
 }
 

@@ -1,11 +1,13 @@
 """Named workloads and the per-pair operation counts used for roofline figures.
 
-Operation counts are algorithmic: counted from the per-pair source of each kernel
-(csrc/lithofwd.hip k_forward_f32c, csrc/adjoint.h k_adjoint_pairs) with a fused
-multiply-add as 2 flops and each transcendental (sqrt, reciprocal, log2) as 8
-flop-equivalents -- on gfx950 these issue at a quarter of the FMA rate, so one occupies
-the issue slots of four FMAs.  A pair is one (station, facet) combination with the
-facet stored once (contact facets are not duplicated).
+Operation counts are algorithmic, per (station, facet) pair, counted from the per-pair
+source of each kernel (csrc/lithofwd.hip k_forward_f32c, csrc/adjoint.h k_adjoint_pairs;
+the compiled loops hold the same mix): a fused multiply-add is 2 flops, an add, subtract
+or multiply 1, and each transcendental (sqrt, reciprocal, log2) 8 flop-equivalents -- on
+gfx950 these issue at a quarter of the FMA rate, so one occupies the issue slots of four
+FMAs.  Compares, selects, sign and min/max operations and conversions are not counted.
+A pair is one (station, facet) combination with the facet stored once (contact facets are
+not duplicated).
 """
 from __future__ import annotations
 
@@ -15,21 +17,27 @@ TRANSCENDENTAL_FLOP_EQ = 8
 
 # per (station, facet) pair
 OPS = {
-    # forward, gz + TMI fused, FP32C: station offset 3, frame projection 15, squared
-    # distances 14, solid-angle D and N 19, atan2 20, edge (s, dperp, rho^2) 18,
-    # three stable edge logs 12, FP64-branch tests 6, gz combine + Kahan 12,
-    # TMI combine + Kahan 11  -> 130 arithmetic; 3 sqrt + 4 rcp + 3 log2 = 10 special
-    "forward_f32c": dict(arith=130, special=10),
-    # adjoint, gz + TMI seeds: forward part 112, coefficient seeds 13, reverse sweep 141,
-    # accumulation of the 22 facet seeds 25 -> 291 arithmetic;
-    # 3 sqrt + 1 rcp (atan2) + 15 (three edge logs with their reciprocals) + 4 reverse rcp
-    "adjoint_pairs_f32": dict(arith=291, special=23),
+    # forward, gz + TMI fused, FP32C: FP64 station offset (3 adds), frame projection,
+    # squared distances, solid-angle N and D, atan2 polynomial, edge (s, dperp, rho^2),
+    # three cancellation-free edge logs (ln(1+y) with the rounding correction of 1+y),
+    # gz and TMI combine with Kahan sums; 3 sqrt + 4 rcp + 3 log2
+    "forward_f32c": dict(fma=50, addmul=58, special=10),
+    # adjoint, gz + TMI seeds: the forward part, coefficient seeds, reverse sweep through
+    # the edge logs, solid angle and distances, accumulation of the 22 facet seeds;
+    # 3 sqrt + 17 rcp (atan2, edge logs and their derivatives, reverse sweep) + 3 log2
+    "adjoint_pairs_f32": dict(fma=107, addmul=119, special=23),
 }
 
 
 def flop_eq(kernel: str) -> int:
     o = OPS[kernel]
-    return o["arith"] + TRANSCENDENTAL_FLOP_EQ * o["special"]
+    return 2 * o["fma"] + o["addmul"] + TRANSCENDENTAL_FLOP_EQ * o["special"]
+
+
+def arith_flops(kernel: str) -> int:
+    """Flops of the pair without the transcendental weighting (FMA = 2)."""
+    o = OPS[kernel]
+    return 2 * o["fma"] + o["addmul"]
 
 
 WORKLOADS = {
