@@ -216,6 +216,20 @@ def main():
             "fp64_forward_s": tb - ta,
             "fp32c_edges_on_fp64_branch": fb,
         }
+        # the reference's TMI edge coefficient leaves a field of order 1e6 nT on a closed
+        # body, so FP32 rounding alone is ~1 nT there; the textbook coefficient gives the
+        # physical field (a few hundred nT) and shows the precision mode's own error
+        with ForwardContext(m, st, fh, field, precision="fp32c", tmi_mode="textbook",
+                            want=("tmi",)) as c:
+            _, tt32 = c.forward()
+        with ForwardContext(m, st, fh, field, precision="fp64", tmi_mode="textbook",
+                            want=("tmi",)) as c:
+            _, tt64 = c.forward()
+        acc["max_rel_dtmi"] = float(np.abs(t32 - t64).max() / np.abs(t64).max())
+        acc["textbook_tmi"] = {
+            "max_abs_dtmi_nT": float(np.abs(tt32 - tt64).max()),
+            "tmi_range_nT": [float(tt64.min()), float(tt64.max())],
+        }
         log(f"accuracy: {acc}")
 
     cpu = None

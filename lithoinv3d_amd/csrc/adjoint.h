@@ -76,23 +76,32 @@ struct AdjointLaunch {
 // ln(Q) of edge_ln and its derivatives with respect to (s1, s2, d1, d2, rho^2) in the
 // form of each case: Q = X2/X1, X1/X2 or X1 X2/rho^2 with X = |s| + d.  The forms agree
 // where d^2 = s^2 + rho^2, which the frame parametrization keeps, so the chain rule
-// through it is exact whichever form is differentiated.
+// through it is exact whichever form is differentiated.  The same y = Q - 1 as edge_ln,
+// with 1/X1, 1/X2 and the ratio's reciprocal taken from one reciprocal of
+// X1 X2 (d1 + d2), and 1/rho^2 from a second.
 template <class T>
 __device__ __forceinline__ T edge_ln_grad(T s1, T ell, T d1, T d2, T rho2, T& gs1, T& gs2,
                                           T& gd1, T& gd2, T& grho)
 {
-    T X1, X2;
-    bool pos1, neg2;
-    const T lam = edge_ln(s1, ell, d1, d2, rho2, X1, X2, pos1, neg2);
+    const T s2 = s1 + ell;
+    const T X1 = fabs(s1) + d1;
+    const T X2 = fabs(s2) + d2;
+    const bool pos1 = s1 >= T(0);
+    const bool neg2 = s2 < T(0);
     const bool mid = !(pos1 || neg2);
-    const T iX1 = fast_rcp(X1);
-    const T iX2 = fast_rcp(X2);
+    const T dsum = d1 + d2;
+    const T X12 = X1 * X2;
+    const T r = fast_rcp(X12 * dsum);
+    const T irho = fast_rcp(rho2);
+    const T iX1 = X2 * dsum * r;
+    const T iX2 = X1 * dsum * r;
+    const T y = mid ? (X12 - rho2) * irho : ell * (X1 + X2) * (pos1 ? X2 : X1) * r;
     gs1 = -iX1;
     gs2 = iX2;
     gd1 = pos1 ? -iX1 : iX1;
     gd2 = neg2 ? -iX2 : iX2;
-    grho = mid ? -fast_rcp(rho2) : T(0);
-    return lam;
+    grho = mid ? -irho : T(0);
+    return log1p_t(y);
 }
 
 // One (station, facet) pair: frame-form forward and its reverse sweep for the weights
@@ -261,8 +270,15 @@ __device__ __forceinline__ void pair_seeds(const FaceFrame<T>& F, double ox, dou
     acc[S_TWOA] += stwoA;
 }
 
+// Optional occupancy target for the pair kernel (waves per SIMD), a build-time tuning knob.
+#ifdef LF_ADJOINT_WAVES_PER_EU
+#define LF_ADJOINT_OCC __attribute__((amdgpu_waves_per_eu(LF_ADJOINT_WAVES_PER_EU)))
+#else
+#define LF_ADJOINT_OCC
+#endif
+
 template <class T>
-__global__ __launch_bounds__(256) void k_adjoint_pairs(const FaceFrame<T>* __restrict__ faces,
+__global__ __launch_bounds__(256) LF_ADJOINT_OCC void k_adjoint_pairs(const FaceFrame<T>* __restrict__ faces,
                                                        int64_t n_f,
                                                        const StationW* __restrict__ st,
                                                        int64_t n_s, int64_t st_per_chunk,

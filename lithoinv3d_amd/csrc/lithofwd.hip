@@ -113,8 +113,15 @@ __device__ __forceinline__ void block_coords(int n_tiles, int& tile, int& chunk)
 
 // ----------------------------------------------------------------- FP32C forward
 
+// Optional occupancy target for the pair kernel (waves per SIMD), a build-time tuning knob.
+#ifdef LF_FORWARD_WAVES_PER_EU
+#define LF_FORWARD_OCC __attribute__((amdgpu_waves_per_eu(LF_FORWARD_WAVES_PER_EU)))
+#else
+#define LF_FORWARD_OCC
+#endif
+
 template <bool GZ, bool TMI>
-__global__ __launch_bounds__(256) void k_forward_f32c(
+__global__ __launch_bounds__(256) LF_FORWARD_OCC void k_forward_f32c(
     const FaceF32* __restrict__ faces, const double* __restrict__ geo64, int64_t n_f,
     int64_t faces_per_chunk, const double* __restrict__ sx, const double* __restrict__ sy,
     const double* __restrict__ sz, int64_t n_s, int n_tiles, double* __restrict__ part_gz,

@@ -6,7 +6,9 @@ log of the Gauss-Newton diagonal of the joint misfit,
 
     d_j = sum_i (dgz_i/dp_j / sigma_gz)^2 + alpha * sum_i (dTMI_i/dp_j / sigma_tmi)^2,
 
-from local geometry; the optimizer then starts from H0 = gamma * diag(exp(-prediction)).
+from local geometry; the optimizer then starts from H0 = gamma * diag(exp(-prediction/2))
+(one over the predicted column norm; full Jacobi scaling, exp(-prediction), is slower on
+these problems).
 The objective stays exact, so the plain solve (H0 = gamma * I) remains both the fallback
 and the judge of the gain.  Training targets are the exact column norms produced by the
 GPU forward machinery (lf_colnorm2).
@@ -22,10 +24,10 @@ from . import lithofwd
 
 N_FEATURES = 16
 FEATURE_NAMES = (
-    "depth_km", "log_area", "n_x", "n_y", "n_z", "log_stations_within_2_depths",
-    "abs_drho", "log_abs_dchi", "log_support", "weight_sum", "log_depth",
-    "log_area_over_depth2", "n_z_squared", "abs_drho_nz", "abs_dchi_nz_scaled",
-    "edge_distance_over_depth",
+    "depth_km", "log_area", "log_chi_sum_area", "log_chi_sum_area_fpar", "n_z",
+    "log_stations_within_2_depths", "abs_drho", "log_abs_dchi", "log_support", "weight_sum",
+    "log_depth", "log_area_over_depth2", "log_chi_sum_area_nz", "abs_drho_nz",
+    "abs_dchi_nz_scaled", "edge_distance_over_depth",
 )
 
 _lib_ready = False
@@ -127,68 +129,78 @@ class MLP:
 
 # --------------------------------------------------------------------- features
 
-def vertex_areas_normals(m):
-    """Per vertex: one third of the incident facet areas, and the area-weighted normal of
-    the incident facets, plus area-weighted |drho| and |dchi| across them."""
+def vertex_areas_normals(m, field_dir=(0.0, 0.0, 1.0)):
+    """Per-vertex sums over the incident facets of one third of each facet's area A_f, of
+    A_f n_f, A_f |drho_f| and A_f |dchi_f| (contrasts across the facet), and of
+    A_f (chi_front + chi_back) weighted by 1, by 1 - (F.n_f)^2 and by |n_f,z|.  The sum
+    of susceptibilities follows the reference's TMI edge coefficient, which adds the
+    magnetizations on both sides of a contact instead of differencing them."""
     V, F = m.vertices, m.faces.astype(np.int64)
     cr = np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]])
     area = 0.5 * np.linalg.norm(cr, axis=1)
     nrm = cr / np.maximum(2 * area, 1e-30)[:, None]
-    rb = np.where(m.back >= 0, m.density[np.maximum(m.back, 0)], 0.0)
-    cb = np.where(m.back >= 0, m.susceptibility[np.maximum(m.back, 0)], 0.0)
+    has_back = m.back >= 0
+    bk = np.maximum(m.back, 0)
+    rb = np.where(has_back, m.density[bk], 0.0)
+    cb = np.where(has_back, m.susceptibility[bk], 0.0)
     drho = np.abs(m.density[m.front] - rb)
     dchi = np.abs(m.susceptibility[m.front] - cb)
+    csum = m.susceptibility[m.front] + cb
+    fpar = 1.0 - (nrm @ np.asarray(field_dir, np.float64)) ** 2
+    a3 = area / 3
+    cols = np.stack([a3, nrm[:, 0] * a3, nrm[:, 1] * a3, nrm[:, 2] * a3, drho * a3,
+                     dchi * a3, csum * a3, csum * fpar * a3, csum * np.abs(nrm[:, 2]) * a3], 1)
     nv = m.n_vertices
-    A = np.zeros(nv)
-    N = np.zeros((nv, 3))
-    R = np.zeros(nv)
-    C = np.zeros(nv)
+    acc = np.zeros((nv, cols.shape[1]))
     for k in range(3):
-        np.add.at(A, F[:, k], area / 3)
-        np.add.at(N, F[:, k], nrm * (area / 3)[:, None])
-        np.add.at(R, F[:, k], drho * area / 3)
-        np.add.at(C, F[:, k], dchi * area / 3)
-    return A, N, R, C
+        for j in range(cols.shape[1]):
+            acc[:, j] += np.bincount(F[:, k], weights=cols[:, j], minlength=nv)
+    return dict(A=acc[:, 0], N=acc[:, 1:4], R=acc[:, 4], C=acc[:, 5], S=acc[:, 6],
+                S_par=acc[:, 7], S_z=acc[:, 8])
 
 
-def dof_features(m, dof_map, dof_vertex, stations, axis_scale):
-    """(n_dof, 16) features per control-point parameter (FEATURE_NAMES)."""
+def dof_features(m, dof_map, dof_vertex, stations, axis_scale, field_dir=(0.0, 0.0, 1.0)):
+    """(n_dof, 16) features per control-point parameter (FEATURE_NAMES), from the
+    parameter's vertices and their interpolation weights (dof_map), the stations and
+    the inducing-field direction."""
+    from scipy.spatial import cKDTree
+
     off, vert, w = dof_map
-    A, N, R, C = vertex_areas_normals(m)
+    va = vertex_areas_normals(m, field_dir)
     n = len(off) - 1
     ww = w / axis_scale
     seg = np.repeat(np.arange(n), np.diff(off))
 
     def dofsum(vals):
-        out = np.zeros(n) if vals.ndim == 1 else np.zeros((n,) + vals.shape[1:])
-        np.add.at(out, seg, (ww[:, None] * vals[vert]) if vals.ndim > 1 else ww * vals[vert])
-        return out
+        if vals.ndim == 1:
+            return np.bincount(seg, weights=ww * vals[vert], minlength=n)
+        return np.stack([np.bincount(seg, weights=ww * vals[vert, j], minlength=n)
+                         for j in range(vals.shape[1])], 1)
 
-    Aeff = dofsum(A)
-    Neff = dofsum(N)
+    Aeff = dofsum(va["A"])
+    Neff = dofsum(va["N"])
     nn = Neff / np.maximum(np.linalg.norm(Neff, axis=1, keepdims=True), 1e-30)
-    Reff = dofsum(R) / np.maximum(Aeff, 1e-30)
-    Ceff = dofsum(C) / np.maximum(Aeff, 1e-30)
+    Reff = dofsum(va["R"]) / np.maximum(Aeff, 1e-30)
+    Ceff = dofsum(va["C"]) / np.maximum(Aeff, 1e-30)
     P = m.vertices[dof_vertex]
     ztop = float(np.mean(stations[:, 2]))
     depth = np.maximum(ztop - P[:, 2], 1.0)
-    # stations within a horizontal radius of two depths (cell-binned count)
+    # stations within a horizontal radius of two depths
     sxy = stations[:, :2]
-    cnt = np.zeros(n)
-    for lo in range(0, n, 2048):
-        hi = min(n, lo + 2048)
-        d2 = ((P[lo:hi, None, :2] - sxy[None, :, :]) ** 2).sum(-1)
-        cnt[lo:hi] = (d2 <= (2 * depth[lo:hi, None]) ** 2).sum(1)
+    cnt = cKDTree(sxy).query_ball_point(P[:, :2], r=2 * depth, return_length=True)
+    cnt = np.asarray(cnt, dtype=np.float64)
     xmin, ymin = sxy.min(0)
     xmax, ymax = sxy.max(0)
     edge = np.minimum.reduce([P[:, 0] - xmin, xmax - P[:, 0], P[:, 1] - ymin, ymax - P[:, 1]])
     support = np.diff(off).astype(np.float64)
-    wsum = np.zeros(n)
-    np.add.at(wsum, seg, ww)
+    wsum = np.bincount(seg, weights=ww, minlength=n)
+    tiny = 1e-12
     X = np.stack([
         depth / 1000.0,
         np.log1p(Aeff),
-        nn[:, 0], nn[:, 1], nn[:, 2],
+        np.log(np.abs(dofsum(va["S"])) + tiny),
+        np.log(np.abs(dofsum(va["S_par"])) + tiny),
+        nn[:, 2],
         np.log1p(cnt),
         Reff,
         np.log10(np.maximum(Ceff, 1e-6)),
@@ -196,7 +208,7 @@ def dof_features(m, dof_map, dof_vertex, stations, axis_scale):
         wsum,
         np.log(depth),
         np.log(np.maximum(Aeff, 1e-6) / depth ** 2),
-        nn[:, 2] ** 2,
+        np.log(np.abs(dofsum(va["S_z"])) + tiny),
         Reff * np.abs(nn[:, 2]),
         Ceff * np.abs(nn[:, 2]) * 100.0,
         edge / depth,
