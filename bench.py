@@ -57,15 +57,15 @@ def cpu_baseline(model, st, fh, field, seconds):
     ref_faces = model.reference_face_count()
     rng = np.random.default_rng(11)
 
-    def run(idx):
+    def run(idx, threads=1):
         s = st[idx]
         w = rng.normal(size=len(idx))
         t0 = time.perf_counter()
-        om.gravity(s, threads=1)
-        om.magnetic(fh, s, threads=1)
+        om.gravity(s, threads=threads)
+        om.magnetic(fh, s, threads=threads)
         t1 = time.perf_counter()
-        om.gravity_vertex_jtw(s, w, threads=1)
-        om.magnetic_vertex_jtw(fh, s, w, threads=1)
+        om.gravity_vertex_jtw(s, w, threads=threads)
+        om.magnetic_vertex_jtw(fh, s, w, threads=threads)
         t2 = time.perf_counter()
         return t1 - t0, t2 - t1
 
@@ -75,6 +75,12 @@ def cpu_baseline(model, st, fh, field, seconds):
     idx = rng.choice(len(st), n, replace=False)
     tf, tg = run(idx)
     pairs = n * ref_faces
+    # the same code on the host's cores (one station per thread at a time), for scale;
+    # 16 threads at most, the CPU share of one GPU on the GPU hosts
+    nt = max(1, min(16, os.cpu_count() or 1))
+    idx_t = rng.choice(len(st), n * nt, replace=False)
+    tft, tgt = run(idx_t, threads=nt)
+    pairs_t = n * nt * ref_faces
     return {
         "value": pairs / (tf + tg),
         "unit": "station x facet pairs/s (fwd+grad, reference facet layout)",
@@ -85,6 +91,8 @@ def cpu_baseline(model, st, fh, field, seconds):
                   f"host {cpu_model()}",
         "forward_pairs_per_s": pairs / tf,
         "grad_pairs_per_s": pairs / tg,
+        "threaded": {"threads": nt, "nproc": os.cpu_count(), "value": pairs_t / (tft + tgt),
+                     "sample": f"{n * nt} stations, forward {tft:.2f} s, J^T w {tgt:.2f} s"},
     }
 
 

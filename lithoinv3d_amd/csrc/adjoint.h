@@ -123,7 +123,9 @@ __device__ __forceinline__ void pair_seeds(const FaceFrame<T>& F, double ox, dou
     const T db2 = t * t + rho0;
     const T dcx = F.cu - px, dcy = F.cv - py;
     const T dc2 = dcx * dcx + dcy * dcy + pz2;
-    const T da = fast_sqrt(da2), db = fast_sqrt(db2), dc = fast_sqrt(dc2);
+    // one reciprocal square root per distance gives both d and the 1/d of the reverse
+    const T rda = fast_rsqrt(da2), rdb = fast_rsqrt(db2), rdc = fast_rsqrt(dc2);
+    const T da = da2 * rda, db = db2 * rdb, dc = dc2 * rdc;
     const T rab = da2 - F.l0 * px;
     const T rac = da2 - (F.cu * px + F.cv * py);
     const T rbc = t * dcx - py * dcy + pz2;
@@ -227,9 +229,9 @@ __device__ __forceinline__ void pair_seeds(const FaceFrame<T>& F, double ox, dou
     spy -= srac * F.cv;
     T sl0c = -srab * px;
     // square roots
-    sda2 += T(0.5) * sda * fast_rcp(da);
-    const T sdb2 = T(0.5) * sdb * fast_rcp(db);
-    const T sdc2 = T(0.5) * sdc * fast_rcp(dc);
+    sda2 += T(0.5) * sda * rda;
+    const T sdb2 = T(0.5) * sdb * rdb;
+    const T sdc2 = T(0.5) * sdc * rdc;
     // dc2 = dcx^2 + dcy^2 + pz2
     sdcx += T(2) * dcx * sdc2;
     sdcy += T(2) * dcy * sdc2;
@@ -439,6 +441,8 @@ __global__ __launch_bounds__(128) void k_facet_T_axis(
 // lane = facet; for each station of the block (wave-uniform) the per-pair seeds for a
 // unit weight on gz and on TMI, contracted with T: the derivative of gz_s and TMI_s with
 // respect to the chosen coordinate of each facet corner, G[s][3 f + corner].
+// blockIdx.y splits the block's stations into groups of kStationGroup.
+constexpr int kStationGroup = 16;
 template <class T>
 __global__ __launch_bounds__(256) void k_station_facet_grad(
     const FaceFrame<T>* __restrict__ faces, const T* __restrict__ Tax, int64_t n_f,
@@ -448,7 +452,9 @@ __global__ __launch_bounds__(256) void k_station_facet_grad(
     if (f >= n_f) return;
     const FaceFrame<T> F = faces[f];
     const T* tf = Tax + (int64_t)f * kSeeds * 3;
-    for (int s = 0; s < n_blk; ++s) {
+    const int s0 = blockIdx.y * kStationGroup;
+    const int s1 = min(n_blk, s0 + kStationGroup);
+    for (int s = s0; s < s1; ++s) {
         const StationW S = st[s];
         T ag[kSeeds], at[kSeeds];
 #pragma unroll
@@ -484,7 +490,9 @@ __global__ __launch_bounds__(256) void k_gather_block(const float* __restrict__ 
     D[(int64_t)s * n_v + v] = acc;
 }
 
-// out[j] += sum_s (sum_e w_e D[s][vertex_e])^2 over the entries of DOF j
+// out[j] += sum_s (sum_e w_e D[s][vertex_e])^2 over the entries of DOF j.  One wave per
+// DOF, one lane per station of the block (n_blk <= 64); the squares are summed across the
+// wave in a fixed butterfly order and one lane adds them to out[j].
 __global__ __launch_bounds__(256) void k_dof_colnorm(const float* __restrict__ D, int n_blk,
                                                      int64_t n_v, int64_t n_dof,
                                                      const int64_t* __restrict__ off,
@@ -492,16 +500,18 @@ __global__ __launch_bounds__(256) void k_dof_colnorm(const float* __restrict__ D
                                                      const double* __restrict__ w,
                                                      double* __restrict__ out)
 {
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n_dof) return;
-    double acc = 0.0;
-    for (int s = 0; s < n_blk; ++s) {
+    const int64_t j = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    const int s = threadIdx.x % 64;
+    if (j >= n_dof) return;  // uniform across the wave
+    double val = 0.0;
+    if (s < n_blk) {
         const float* d = D + (int64_t)s * n_v;
-        double val = 0.0;
         for (int64_t e = off[j]; e < off[j + 1]; ++e) val += w[e] * (double)d[vert[e]];
-        acc += val * val;
     }
-    out[j] += acc;
+    double sq = val * val;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) sq += __shfl_xor(sq, m, 64);
+    if (s == 0) out[j] += sq;
 }
 
 __global__ __launch_bounds__(256) void k_vertex_gather(const double* __restrict__ face_grad,

@@ -811,21 +811,23 @@ int lf_colnorm2(lf_ctx* c, int axis, int64_t n_dof, const int64_t* dof_off,
                 hipLaunchKernelGGL(k_pack_stations, dim3(1), dim3(256), 0, c->stream, c->d_sx + b0,
                                    c->d_sy + b0, c->d_sz + b0, c->adj.d_zero, c->adj.d_zero,
                                    (int64_t)nb, dst);
-                const unsigned gb = (unsigned)((c->n_f + 255) / 256);
+                const dim3 gb((unsigned)((c->n_f + 255) / 256),
+                              (unsigned)((nb + kStationGroup - 1) / kStationGroup));
                 if (c->precision == LF_FP32C)
-                    hipLaunchKernelGGL(k_station_facet_grad<float>, dim3(gb), dim3(256), 0, c->stream,
+                    hipLaunchKernelGGL(k_station_facet_grad<float>, gb, dim3(256), 0, c->stream,
                                        c->d_f32, (const float*)dT, c->n_f, dst, nb, Gg, Gt);
                 else
-                    hipLaunchKernelGGL(k_station_facet_grad<double>, dim3(gb), dim3(256), 0, c->stream,
+                    hipLaunchKernelGGL(k_station_facet_grad<double>, gb, dim3(256), 0, c->stream,
                                        c->d_fr64, (const double*)dT, c->n_f, dst, nb, Gg, Gt);
                 const dim3 vg((unsigned)((c->n_v + 255) / 256), (unsigned)nb);
                 hipLaunchKernelGGL(k_gather_block, vg, dim3(256), 0, c->stream, Gg, c->adj.d_csr_off,
                                    c->adj.d_csr_ent, c->n_v, c->n_f, Dg);
                 hipLaunchKernelGGL(k_gather_block, vg, dim3(256), 0, c->stream, Gt, c->adj.d_csr_off,
                                    c->adj.d_csr_ent, c->n_v, c->n_f, Dt);
-                hipLaunchKernelGGL(k_dof_colnorm, dim3(jb), dim3(256), 0, c->stream, Dg, nb, c->n_v,
+                const unsigned cb = (unsigned)((n_dof + 3) / 4);  // one wave per DOF
+                hipLaunchKernelGGL(k_dof_colnorm, dim3(cb), dim3(256), 0, c->stream, Dg, nb, c->n_v,
                                    n_dof, doff, dvert, dw, og);
-                hipLaunchKernelGGL(k_dof_colnorm, dim3(jb), dim3(256), 0, c->stream, Dt, nb, c->n_v,
+                hipLaunchKernelGGL(k_dof_colnorm, dim3(cb), dim3(256), 0, c->stream, Dt, nb, c->n_v,
                                    n_dof, doff, dvert, dw, ot);
                 CN(hipGetLastError(), "column-norm kernels");
             }

@@ -59,9 +59,11 @@ static_assert(sizeof(FaceF64) == 288, "FaceF64 layout");
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float fast_log2(float x) { return __builtin_amdgcn_logf(x); }
+__device__ __forceinline__ float fast_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ double fast_rcp(double x) { return 1.0 / x; }
 __device__ __forceinline__ double fast_sqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ double fast_log2(double x) { return log2(x); }
+__device__ __forceinline__ double fast_rsqrt(double x) { return 1.0 / sqrt(x); }
 
 // atan2 in FP32: argument reduction to a = min(|x|,|y|)/max(|x|,|y|) in [0, 1], odd
 // minimax polynomial atan(a) ~= a*P(a^2) of degree 7 in a^2 (max abs error 3.7e-8 rad;

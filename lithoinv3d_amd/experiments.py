@@ -11,7 +11,11 @@
                           held-out model; the learned and FP32C runs stop at the plain FP64
                           run's final misfit, and report iterations and seconds to it.
 
+  * ``large_inversion``   a few FP32C iterations of the joint XYZ_FREE inversion at the
+                          large workload, resident on one GPU.
+
 CLI:  python -m lithoinv3d_amd.experiments learned-h0 --train 200 --test 20 --out FILE
+      python -m lithoinv3d_amd.experiments large --max-iter 3
 """
 from __future__ import annotations
 
@@ -23,7 +27,7 @@ import time
 
 import numpy as np
 
-from . import inversion, mlp, synth
+from . import inversion, mlp, synth, workload
 from .lithofwd import ForwardContext
 
 AXIS_SCALE = 10.0
@@ -188,6 +192,71 @@ def compare_solvers(prob, net=None, norm=None, max_iter=40, exact_diag=False):
     return out
 
 
+def large_inversion(name="contact256x256x96_st512", stride=8, max_iter=3, ref_weight=1e-2,
+                    seed=5):
+    """Joint gz + TMI inversion at the large workload, everything resident on one GPU:
+    XYZ_FREE control points (every ``stride``-th cell), FP32C forward and adjoint, the
+    reference-model term, synthetic data from a perturbed interface.  The first step is
+    scaled so the largest parameter moves one unit (the unscaled -g step would cost a
+    long series of halvings, each a full forward).  Reports setup, per-iteration times,
+    misfit and device memory in use."""
+    import torch
+
+    out = {"workload": name}
+    t0 = time.perf_counter()
+    cfg = workload.build(name)
+    m, st = cfg["model"], cfg["stations"]
+    out["n_vertices"], out["n_facets"], out["n_stations"] = m.n_vertices, m.n_faces, len(st)
+    out["pairs_per_forward"] = float(m.n_faces) * len(st)
+    cell = float(workload.WORKLOADS[name].get("cell", 25.0))
+    M = inversion.Model(m)
+    M.fix_exterior(inversion.XYZ_FREE)
+    M.set_axis_scale(AXIS_SCALE)
+    n = M.set_cp_stride(stride, cell)
+    v, ax, _ = M.dofs()
+    p0 = M.params()
+    extent = workload.WORKLOADS[name]["shape"][0] * cell
+    ptrue = p0.copy()
+    zsel = ax == 2
+    ptrue[zsel] += true_displacement(m.vertices[v[zsel], :2], extent, seed, amp=30.0,
+                                     width=600.0) / AXIS_SCALE
+    lo, hi = M.bounds(p0, 200.0)
+    ptrue = np.clip(ptrue, lo, hi)
+    out["n_dof"] = int(n)
+    out["setup_s"] = time.perf_counter() - t0
+    log(f"large: {m.n_vertices} vertices, {m.n_faces} facets, {len(st)} stations, {n} "
+        f"parameters; setup {out['setup_s']:.1f}s")
+    inc, dec, fld = cfg["inc"], cfg["dec"], cfg["field"]
+    P = inversion.Problem(M, st, np.zeros(len(st)), tmi_obs=np.zeros(len(st)), inc=inc,
+                          dec=dec, field=fld, precision="fp32c")
+    t1 = time.perf_counter()
+    gz, tm = P.forward(ptrue)
+    out["forward_s"] = time.perf_counter() - t1
+    P.close()
+    rng = np.random.default_rng(seed + 1)
+    gz = gz + rng.normal(0, SIGMA_GZ, len(gz))
+    tm = tm + rng.normal(0, SIGMA_TMI, len(tm))
+    log(f"large: synthetic data in {out['forward_s']:.1f}s")
+    P = inversion.Problem(M, st, gz, np.full(len(st), SIGMA_GZ), tm, np.full(len(st), SIGMA_TMI),
+                          inc=inc, dec=dec, field=fld, alpha_mag=1.0, precision="fp32c",
+                          ref_weight=ref_weight)
+    t2 = time.perf_counter()
+    g0 = P.gradient(p0)
+    out["first_gradient_s"] = time.perf_counter() - t2
+    free, total = torch.cuda.mem_get_info(0)
+    out["device_memory_in_use_gb"] = (total - free) / 1e9
+    h0 = np.full(n, 1.0 / max(float(np.abs(g0).max()), 1e-300))
+    log(f"large: first gradient {out['first_gradient_s']:.1f}s; "
+        f"{out['device_memory_in_use_gb']:.1f} GB in use")
+    r = P.minimize(p0, lo, hi, max_iter=max_iter, h0=h0)
+    P.close()
+    h = r.history
+    out["history"] = [dict(iteration=int(row[0]), f=float(row[1]), seconds=float(row[4]),
+                           n_obj=int(row[5]), n_grad=int(row[6])) for row in h]
+    out["message"] = r.message
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -205,7 +274,18 @@ def main(argv=None):
     b.add_argument("--n-st", type=int, default=128)
     b.add_argument("--max-iter", type=int, default=40)
     b.add_argument("--out", default=None)
+    c = sub.add_parser("large")
+    c.add_argument("--workload", default="contact256x256x96_st512")
+    c.add_argument("--stride", type=int, default=8)
+    c.add_argument("--max-iter", type=int, default=3)
+    c.add_argument("--out", default=None)
     args = ap.parse_args(argv)
+    if args.cmd == "large":
+        res = large_inversion(args.workload, args.stride, args.max_iter)
+        if args.out:
+            json.dump(res, open(args.out, "w"), indent=1)
+        print(json.dumps(res))
+        return
     shape = tuple(args.shape)
     if args.cmd == "invert":
         prob = contact_problem(args.seed, shape, args.n_st)

@@ -5,7 +5,8 @@ source of each kernel (csrc/lithofwd.hip k_forward_f32c, csrc/adjoint.h k_adjoin
 the compiled loops hold the same mix): a fused multiply-add is 2 flops, an add, subtract
 or multiply 1, and each transcendental (sqrt, reciprocal, log2) 8 flop-equivalents: on
 gfx950 these issue at 32 lanes/clock/CU, a quarter of the 128 FMA lanes/clock/CU that the
-157.3 TFLOP/s FP32 peak assumes, so one occupies the issue slots of four FMAs.  Compares, selects, sign and min/max operations and conversions are not counted.
+157.3 TFLOP/s FP32 peak assumes, so one occupies the issue slots of four FMAs.
+Compares, selects, sign and min/max operations and conversions are not counted.
 A pair is one (station, facet) combination with the facet stored once (contact facets are
 not duplicated).
 """
@@ -24,8 +25,9 @@ OPS = {
     "forward_f32c": dict(fma=50, addmul=58, special=10),
     # adjoint, gz + TMI seeds: the forward part, coefficient seeds, reverse sweep through
     # the edge logs, solid angle and distances, accumulation of the 22 facet seeds;
-    # 3 sqrt + 11 rcp (atan2, two per edge log with its derivative, reverse sweep) + 3 log2
-    "adjoint_pairs_f32": dict(fma=98, addmul=152, special=17),
+    # 3 rsqrt (distances and their reciprocals) + 8 rcp (atan2, two per edge log with its
+    # derivative, the atan2 derivative) + 3 log2
+    "adjoint_pairs_f32": dict(fma=103, addmul=150, special=14),
 }
 
 

@@ -161,3 +161,56 @@ def test_torch_autograd_bridge():
                 c2, d2 = polyhedral_forward(vm, ctx)
             fd = (wg @ (a - c2).cpu().numpy() + wt @ (b - d2).cpu().numpy()) / (2 * h)
             assert fd == pytest.approx(v.grad[vi, ax].item(), rel=1e-5, abs=1e-9 * np.abs(g_ref).max())
+
+
+def test_adjoint_at_scale_vs_central_fd():
+    """The 192x192x64 contact model (2.2 M facets, 1.08 M vertices): FP32C and FP64
+    adjoints against central differences (step 0.01 m) of the FP64 forward at 1000 seeded
+    random vertices x 3 axes, for 64 random stations over the footprint (random positions
+    keep stations off edge lines).  Vertices are perturbed on the device, so each
+    difference costs two 1.4e8-pair FP64 forwards.  Tolerances: relative to the largest
+    gradient component, 1e-5 (FP64) and 2e-3 (FP32C)."""
+    import torch
+
+    from lithoinv3d_amd import workload
+
+    cfg = workload.build("contact192x192x64_st256")
+    m = cfg["model"]
+    fh = synth.field_unit_vector(cfg["inc"], cfg["dec"])
+    rng = np.random.default_rng(4)
+    half = 0.5 * 192 * 25.0
+    st = np.column_stack([rng.uniform(-half, half, 64), rng.uniform(-half, half, 64),
+                          rng.uniform(-20.0, 10.0, 64)])
+    wg = rng.normal(size=64)
+    wt = rng.normal(size=64)
+    verts = rng.choice(m.n_vertices, 1000, replace=False)
+    say("adjoint at scale: fp64 / fp32c adjoints")
+    with ForwardContext(m, st, fh, cfg["field"], precision="fp32c") as ctx:
+        g32 = ctx.adjoint(wg, wt)
+    with ForwardContext(m, st, fh, cfg["field"], precision="fp64") as ctx:
+        g64 = ctx.adjoint(wg, wt)
+        say("adjoint at scale: central differences of the fp64 forward")
+        dv = torch.tensor(m.vertices, dtype=torch.float64, device="cuda")
+        h = 0.01
+        fd = np.zeros((len(verts), 3))
+        for i, v in enumerate(verts):
+            for ax in range(3):
+                x0 = float(m.vertices[v, ax])
+                vals = []
+                for sgn in (1.0, -1.0):
+                    dv[v, ax] = x0 + sgn * h
+                    torch.cuda.synchronize()
+                    ctx.set_vertices_device(dv.data_ptr())
+                    gz, tm = ctx.forward()
+                    vals.append(wg @ gz + wt @ tm)
+                dv[v, ax] = x0
+                fd[i, ax] = (vals[0] - vals[1]) / (2 * h)
+        torch.cuda.synchronize()
+    scale = np.abs(g64).max()
+    e64 = np.abs(g64[verts] - fd).max() / scale
+    e32 = np.abs(g32[verts] - fd).max() / scale
+    e3264 = np.abs(g32 - g64).max() / scale
+    print(f"at scale: fp64 adjoint vs FD {e64:.2e}, fp32c vs FD {e32:.2e}, "
+          f"fp32c vs fp64 over all vertices {e3264:.2e}")
+    assert e64 < 1e-5
+    assert e32 < 2e-3 and e3264 < 2e-3
