@@ -6,10 +6,12 @@
 //                     (normal, edge frame, coefficients) from FP64 vertices
 //                     (the per-pair recomputation in gravity_forward.cpp:17-35 and
 //                     magnetic_forward.cpp:317-341 hoisted out of the station loop).
-//   k_forward_f32c    lane = station, wave-uniform facet: facet records arrive through
-//   k_forward_f64     scalar loads into SGPRs and feed the VALU directly; each lane keeps
-//                     its own (Kahan) accumulators, so the facet reduction needs no
-//                     cross-lane traffic.  Grid = station tiles x facet chunks.
+//   k_forward_f32c    lane = station, wave-uniform facet: facet records stream through
+//                     LDS in stages of 64 and are read at a uniform address; each lane
+//                     keeps its own (Kahan) accumulators, so the facet reduction needs
+//                     no cross-lane traffic.  Grid = station tiles x facet chunks.
+//   k_forward_f64     the same map in FP64 with the reference's operation sequence;
+//                     records arrive through scalar loads.
 //   k_reduce_chunks   sums the per-chunk partials of each station in a fixed order.
 //   k_adjoint_*       see adjoint.h.
 #include <hip/hip_runtime.h>
@@ -120,6 +122,82 @@ __device__ __forceinline__ void block_coords(int n_tiles, int& tile, int& chunk)
 #define LF_FORWARD_OCC
 #endif
 
+// One (station, facet) pair of the FP32C forward, accumulated into the station's Kahan
+// sums.  F is the facet record (wave-uniform), f its index (for the FP64 branch).
+template <bool GZ, bool TMI>
+__device__ __forceinline__ void forward_pair(const FaceF32& F, const double* __restrict__ geo64,
+                                             int64_t f, double ox, double oy, double oz,
+                                             float& gs, float& gc, float& ts, float& tc,
+                                             unsigned& nfb)
+{
+    // station offset from vertex a: FP64 difference, one rounding
+    const float Px = (float)(ox - F.ax);
+    const float Py = (float)(oy - F.ay);
+    const float Pz = (float)(oz - F.az);
+    const float px = Px * F.ux + Py * F.uy + Pz * F.uz;
+    const float py = Px * F.vx + Py * F.vy + Pz * F.vz;
+    const float pz = Px * F.nx + Py * F.ny + Pz * F.nz;  // h = -pz
+    const float pz2 = pz * pz;
+    const float rho0 = fmaf(py, py, pz2);
+    const float da2 = fmaf(px, px, rho0);
+    const float t = F.l0 - px;  // (b - o).u
+    const float db2 = fmaf(t, t, rho0);
+    const float dcx = F.cu - px, dcy = F.cv - py;
+    const float dc2 = fmaf(dcx, dcx, fmaf(dcy, dcy, pz2));
+    const float da = fast_sqrt(da2), db = fast_sqrt(db2), dc = fast_sqrt(dc2);
+    // solid angle: N = r_a.(r_b x r_c) = 2A*h, D from the pairwise dots
+    const float rab = fmaf(-F.l0, px, da2);
+    const float rac = da2 - fmaf(F.cu, px, F.cv * py);
+    const float rbc = fmaf(t, dcx, fmaf(-py, dcy, pz2));
+    const float D = fmaf(da * db, dc, fmaf(rab, dc, fmaf(rac, db, rbc * da)));
+    const float N = -F.twoA * pz;
+    const float th = atan2_t(N, D);  // Omega = 2*th
+    // edges: (s1, s2, d1, d2, dperp, rho^2)
+    const float dp1 = fmaf(t, F.e1v, py * F.e1u);
+    const float dp2 = fmaf(dcx, F.e2v, -dcy * F.e2u);
+    const float s1b = fmaf(t, F.e1u, -py * F.e1v);
+    const float s1c = fmaf(dcx, F.e2u, dcy * F.e2v);
+    const float rho1 = fmaf(dp1, dp1, pz2);
+    const float rho2 = fmaf(dp2, dp2, pz2);
+    float X1, X2;
+    bool p1, n2;
+    float lam0 = edge_ln<float>(-px, F.l0, da, db, rho0, X1, X2, p1, n2);
+    float lam1 = edge_ln<float>(s1b, F.l1, db, dc, rho1, X1, X2, p1, n2);
+    float lam2 = edge_ln<float>(s1c, F.l2, dc, da, rho2, X1, X2, p1, n2);
+    // FP64 for edges whose line passes close to the station while the station
+    // projects inside the edge (rho < sqrt(d1 d2)/32): there the FP32 offsets'
+    // absolute error would dominate the log.  Only possible near the facet plane.
+    if (fabsf(pz) < F.zfb) {
+        const bool fb0 = (-px < 0.f) && (t >= 0.f) && (rho0 * 1024.f < da * db);
+        const bool fb1 = (s1b < 0.f) && (s1b + F.l1 >= 0.f) && (rho1 * 1024.f < db * dc);
+        const bool fb2 = (s1c < 0.f) && (s1c + F.l2 >= 0.f) && (rho2 * 1024.f < dc * da);
+        if (fb0 | fb1 | fb2) {
+            const double* g = geo64 + 6 * f;
+            if (fb0) lam0 = edge_ln_f64(ox, oy, oz, F.ax, F.ay, F.az, g[0], g[1], g[2]);
+            if (fb1) lam1 = edge_ln_f64(ox, oy, oz, g[0], g[1], g[2], g[3], g[4], g[5]);
+            if (fb2) lam2 = edge_ln_f64(ox, oy, oz, g[3], g[4], g[5], F.ax, F.ay, F.az);
+            nfb += (unsigned)fb0 + (unsigned)fb1 + (unsigned)fb2;
+        }
+    }
+    if (GZ) {
+        const float es = fmaf(py, lam0, fmaf(dp1, lam1, dp2 * lam2));
+        kahan_add(gs, gc, fmaf(F.K1, es, F.K2 * (pz * th)));
+    }
+    if (TMI) {
+        kahan_add(ts, tc, fmaf(F.W0, th, fmaf(F.W1, lam0, fmaf(F.W2, lam1, F.W3 * lam2))));
+    }
+}
+
+// Lane = station; the facet records of the block's chunk stream through LDS: the block
+// copies kStage records at a time with coalesced vector loads (issued one stage ahead and
+// held in registers while the current stage is computed), and every wave reads the
+// current record from LDS at a uniform address, so its fields reach the VALU as VGPR
+// operands (VALU instructions with a scalar-register operand issue at half the rate on
+// gfx950).  Each lane keeps its own Kahan sums: the facet reduction needs no cross-lane
+// traffic.
+constexpr int kStage = 64;
+static_assert(kStage * sizeof(FaceF32) == kBlock * 32, "one 32-byte piece per thread");
+
 template <bool GZ, bool TMI>
 __global__ __launch_bounds__(256) LF_FORWARD_OCC void k_forward_f32c(
     const FaceF32* __restrict__ faces, const double* __restrict__ geo64, int64_t n_f,
@@ -127,6 +205,7 @@ __global__ __launch_bounds__(256) LF_FORWARD_OCC void k_forward_f32c(
     const double* __restrict__ sz, int64_t n_s, int n_tiles, double* __restrict__ part_gz,
     double* __restrict__ part_tmi, unsigned long long* __restrict__ fb_count)
 {
+    __shared__ FaceF32 stage[2][kStage];
     int tile, chunk;
     block_coords(n_tiles, tile, chunk);
     const int64_t s = (int64_t)tile * kBlock + threadIdx.x;
@@ -136,69 +215,44 @@ __global__ __launch_bounds__(256) LF_FORWARD_OCC void k_forward_f32c(
     const int64_t f0 = (int64_t)chunk * faces_per_chunk;
     const int64_t f1 = min(n_f, f0 + faces_per_chunk);
 
+    // thread t copies bytes [32 (t % 4), +32) of record t / 4 of a stage
+    const int rec = threadIdx.x >> 2;
+    const int piece = (threadIdx.x & 3) * 2;  // in float4 units
+    float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f), q1 = q0;
+    auto fetch = [&](int64_t fs) {
+        const int64_t fr = fs + rec;
+        if (fr < f1) {
+            const float4* src = reinterpret_cast<const float4*>(faces + fr) + piece;
+            q0 = src[0];
+            q1 = src[1];
+        }
+    };
+    auto park = [&](int b) {
+        float4* dst = reinterpret_cast<float4*>(&stage[b][rec]) + piece;
+        dst[0] = q0;
+        dst[1] = q1;
+    };
+
     float gs = 0.f, gc = 0.f, ts = 0.f, tc = 0.f;
     unsigned nfb = 0;
-    // the next facet's record is requested (scalar loads) while this one is computed
-    FaceF32 Fn = faces[f0 < f1 ? f0 : 0];
-    for (int64_t f = f0; f < f1; ++f) {
-        const FaceF32 F = Fn;
-        Fn = faces[f + 1 < f1 ? f + 1 : f];
-        // station offset from vertex a: FP64 difference, one rounding
-        const float Px = (float)(ox - F.ax);
-        const float Py = (float)(oy - F.ay);
-        const float Pz = (float)(oz - F.az);
-        const float px = Px * F.ux + Py * F.uy + Pz * F.uz;
-        const float py = Px * F.vx + Py * F.vy + Pz * F.vz;
-        const float pz = Px * F.nx + Py * F.ny + Pz * F.nz;  // h = -pz
-        const float pz2 = pz * pz;
-        const float rho0 = fmaf(py, py, pz2);
-        const float da2 = fmaf(px, px, rho0);
-        const float t = F.l0 - px;  // (b - o).u
-        const float db2 = fmaf(t, t, rho0);
-        const float dcx = F.cu - px, dcy = F.cv - py;
-        const float dc2 = fmaf(dcx, dcx, fmaf(dcy, dcy, pz2));
-        const float da = fast_sqrt(da2), db = fast_sqrt(db2), dc = fast_sqrt(dc2);
-        // solid angle: N = r_a.(r_b x r_c) = 2A*h, D from the pairwise dots
-        const float rab = fmaf(-F.l0, px, da2);
-        const float rac = da2 - fmaf(F.cu, px, F.cv * py);
-        const float rbc = fmaf(t, dcx, fmaf(-py, dcy, pz2));
-        const float D = fmaf(da * db, dc, fmaf(rab, dc, fmaf(rac, db, rbc * da)));
-        const float N = -F.twoA * pz;
-        const float th = atan2_t(N, D);  // Omega = 2*th
-        // edges: (s1, s2, d1, d2, dperp, rho^2)
-        const float dp1 = fmaf(t, F.e1v, py * F.e1u);
-        const float dp2 = fmaf(dcx, F.e2v, -dcy * F.e2u);
-        const float s1b = fmaf(t, F.e1u, -py * F.e1v);
-        const float s1c = fmaf(dcx, F.e2u, dcy * F.e2v);
-        const float rho1 = fmaf(dp1, dp1, pz2);
-        const float rho2 = fmaf(dp2, dp2, pz2);
-        float X1, X2;
-        bool p1, n2;
-        float lam0 = edge_ln<float>(-px, F.l0, da, db, rho0, X1, X2, p1, n2);
-        float lam1 = edge_ln<float>(s1b, F.l1, db, dc, rho1, X1, X2, p1, n2);
-        float lam2 = edge_ln<float>(s1c, F.l2, dc, da, rho2, X1, X2, p1, n2);
-        // FP64 for edges whose line passes close to the station while the station
-        // projects inside the edge (rho < sqrt(d1 d2)/32): there the FP32 offsets'
-        // absolute error would dominate the log.  Only possible near the facet plane.
-        if (fabsf(pz) < F.zfb) {
-            const bool fb0 = (-px < 0.f) && (t >= 0.f) && (rho0 * 1024.f < da * db);
-            const bool fb1 = (s1b < 0.f) && (s1b + F.l1 >= 0.f) && (rho1 * 1024.f < db * dc);
-            const bool fb2 = (s1c < 0.f) && (s1c + F.l2 >= 0.f) && (rho2 * 1024.f < dc * da);
-            if (fb0 | fb1 | fb2) {
-                const double* g = geo64 + 6 * f;
-                if (fb0) lam0 = edge_ln_f64(ox, oy, oz, F.ax, F.ay, F.az, g[0], g[1], g[2]);
-                if (fb1) lam1 = edge_ln_f64(ox, oy, oz, g[0], g[1], g[2], g[3], g[4], g[5]);
-                if (fb2) lam2 = edge_ln_f64(ox, oy, oz, g[3], g[4], g[5], F.ax, F.ay, F.az);
-                nfb += (unsigned)fb0 + (unsigned)fb1 + (unsigned)fb2;
-            }
+    if (f0 < f1) {
+        fetch(f0);
+        park(0);
+    }
+    __syncthreads();
+    int b = 0;
+    for (int64_t fs = f0; fs < f1; fs += kStage) {
+        const bool more = fs + kStage < f1;
+        if (more) fetch(fs + kStage);
+        const int nst = (int)min((int64_t)kStage, f1 - fs);
+        for (int j = 0; j < nst; ++j) {
+            const FaceF32 F = stage[b][j];
+            forward_pair<GZ, TMI>(F, geo64, fs + j, ox, oy, oz, gs, gc, ts, tc, nfb);
         }
-        if (GZ) {
-            const float es = fmaf(py, lam0, fmaf(dp1, lam1, dp2 * lam2));
-            kahan_add(gs, gc, fmaf(F.K1, es, F.K2 * (pz * th)));
-        }
-        if (TMI) {
-            kahan_add(ts, tc, fmaf(F.W0, th, fmaf(F.W1, lam0, fmaf(F.W2, lam1, F.W3 * lam2))));
-        }
+        // the other buffer was last read before the previous barrier
+        if (more) park(b ^ 1);
+        __syncthreads();
+        b ^= 1;
     }
     if (active) {
         if (GZ) part_gz[(int64_t)chunk * n_s + s] = (double)gs - (double)gc;

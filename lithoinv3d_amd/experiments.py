@@ -151,7 +151,8 @@ def time_to(r, target):
     return int(r.history[k, 0]), float(r.history[k, 4])
 
 
-def compare_solvers(prob, net=None, norm=None, max_iter=40, exact_diag=False):
+def compare_solvers(prob, net=None, norm=None, max_iter=40, exact_diag=False,
+                    h0_power=H0_POWER, extra_powers=()):
     synth_data(prob)
     out = {"seed": prob["seed"], "n_dof": prob["n_dof"]}
     r64, w64 = run_solver(prob, "fp64", max_iter)
@@ -166,13 +167,20 @@ def compare_solvers(prob, net=None, norm=None, max_iter=40, exact_diag=False):
         t0 = time.perf_counter()
         X, _ = features_targets(prob, with_targets=False)
         pred = norm.t_inv(net.forward(norm.x(X)))
-        h0 = np.exp(-H0_POWER * pred.astype(np.float64))
+        h0 = np.exp(-h0_power * pred.astype(np.float64))
         t_h0 = time.perf_counter() - t0
         rl, _ = run_solver(prob, "fp32c", 3 * max_iter, target=f_star, h0=h0)
         hit = time_to(rl, f_star)
         out["learned_fp32c"] = dict(iterations=rl.iterations, seconds=rl.seconds + t_h0,
                                     h0_seconds=t_h0, f_final=rl.f, reached=hit is not None,
                                     to_target=None if hit is None else (hit[0], hit[1] + t_h0))
+        for pw in extra_powers:
+            rp, _ = run_solver(prob, "fp32c", 3 * max_iter, target=f_star,
+                               h0=np.exp(-pw * pred.astype(np.float64)))
+            hit = time_to(rp, f_star)
+            out[f"learned_fp32c_power{pw:g}"] = dict(
+                iterations=rp.iterations, reached=hit is not None,
+                to_target=None if hit is None else (hit[0], hit[1] + t_h0))
         rl64, _ = run_solver(prob, "fp64", 3 * max_iter, target=f_star, h0=h0)
         hit = time_to(rl64, f_star)
         out["learned_fp64"] = dict(iterations=rl64.iterations, seconds=rl64.seconds + t_h0,
@@ -183,7 +191,7 @@ def compare_solvers(prob, net=None, norm=None, max_iter=40, exact_diag=False):
         _, t = features_targets(prob)
         t_diag = time.perf_counter() - t0
         re, _ = run_solver(prob, "fp32c", 3 * max_iter, target=f_star,
-                           h0=np.exp(-H0_POWER * t))
+                           h0=np.exp(-h0_power * t))
         hit = time_to(re, f_star)
         out["exact_diag_fp32c"] = dict(iterations=re.iterations, seconds=re.seconds + t_diag,
                                        diag_seconds=t_diag, reached=hit is not None,
@@ -267,6 +275,8 @@ def main(argv=None):
     a.add_argument("--n-st", type=int, default=128)
     a.add_argument("--steps", type=int, default=3000)
     a.add_argument("--max-iter", type=int, default=40)
+    a.add_argument("--h0-power", type=float, default=H0_POWER)
+    a.add_argument("--extra-powers", type=float, nargs="*", default=())
     a.add_argument("--out", default=None)
     b = sub.add_parser("invert")
     b.add_argument("--seed", type=int, default=1)
@@ -309,7 +319,8 @@ def main(argv=None):
         # prediction quality on the held-out model
         Xh, th = features_targets(prob)
         pred = norm.t_inv(net.forward(norm.x(Xh)))
-        r = compare_solvers(prob, net, norm, max_iter=args.max_iter, exact_diag=True)
+        r = compare_solvers(prob, net, norm, max_iter=args.max_iter, exact_diag=True,
+                            h0_power=args.h0_power, extra_powers=args.extra_powers)
         r["log_diag_rmse"] = float(np.sqrt(np.mean((pred - th) ** 2)))
         r["log_diag_std"] = float(np.std(th))
         results.append(r)
@@ -347,6 +358,8 @@ def main(argv=None):
             r["learned_fp32c"]["to_target"][1] / r["fp32c"]["to_target"][1]
             for r in results if r["learned_fp32c"]["to_target"] and r["fp32c"]["to_target"]]))
         if results else math.nan,
+        **{f"median_learned_fp32c_power{pw:g}_iterations_to_target":
+           med_to(f"learned_fp32c_power{pw:g}", 0) for pw in args.extra_powers},
         "median_log_diag_rmse": float(np.median([r["log_diag_rmse"] for r in results])),
         "median_log_diag_std": float(np.median([r["log_diag_std"] for r in results])),
     }

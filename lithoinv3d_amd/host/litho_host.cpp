@@ -351,28 +351,42 @@ Vec ForwardModel::computeJTr(const Vec& params, const Vec& w)
 GpuForwardBase::GpuForwardBase(std::shared_ptr<LithologyModel> model,
                                std::vector<double> stations, int want, Precision prec,
                                const double fdir[3], int device)
-    : m_model(std::move(model)), m_nData(stations.size() / 3), m_want(want)
+    : m_model(std::move(model)), m_nData(stations.size() / 3), m_want(want),
+      m_stations(std::move(stations)), m_prec(prec), m_device(device)
 {
+    for (int k = 0; k < 3; ++k) m_fdir[k] = fdir[k];
+}
+
+// The device context (model, facet records, stations resident on the GPU) is created on
+// first use, so objects that are constructed but never evaluated hold no device memory.
+lf_ctx* GpuForwardBase::ctx()
+{
+    if (m_ctx) return m_ctx;
     lf_desc d{};
     d.n_vertices = (int64_t)m_model->vertexCount();
     d.n_faces = (int64_t)m_model->faceCount();
     d.n_stations = (int64_t)m_nData;
     d.n_units = m_model->groupCount();
-    d.precision = prec == Precision::FP32C ? LF_FP32C : LF_FP64;
-    d.tmi_mode = LF_TMI_REFERENCE;
-    d.want = want;
-    d.device = device;
+    d.precision = m_prec == Precision::FP32C ? LF_FP32C : LF_FP64;
+    d.tmi_mode = m_textbook ? LF_TMI_TEXTBOOK : LF_TMI_REFERENCE;
+    d.want = m_want;
+    d.device = m_device;
     d.vertices = m_model->vertices().data();
     d.faces = m_model->faces().data();
     d.front = m_model->front().data();
     d.back = m_model->back().data();
-    d.stations = stations.data();
-    std::vector<double> dens(m_model->groupCount(), 0.0), mag(3 * m_model->groupCount(), 0.0);
+    d.stations = m_stations.data();
+    std::vector<double> dens, mag;
+    buildProperties(dens, mag);
     d.density = dens.data();
     d.magnetization = mag.data();
-    for (int k = 0; k < 3; ++k) d.field_dir[k] = fdir[k];
+    for (int k = 0; k < 3; ++k) d.field_dir[k] = m_fdir[k];
     const int rc = lf_create(&m_ctx, &d);
-    if (rc != 0) throw std::runtime_error(std::string("lf_create: ") + lf_last_error(nullptr));
+    if (rc != 0) {
+        m_ctx = nullptr;
+        throw std::runtime_error(std::string("lf_create: ") + lf_last_error(nullptr));
+    }
+    return m_ctx;
 }
 
 GpuForwardBase::~GpuForwardBase()
@@ -382,6 +396,7 @@ GpuForwardBase::~GpuForwardBase()
 
 void GpuForwardBase::pushProperties()
 {
+    if (!m_ctx) return;  // the context takes the current properties when it is created
     std::vector<double> dens, mag;
     buildProperties(dens, mag);
     check(lf_set_properties(m_ctx, dens.data(), mag.data()), m_ctx, "lf_set_properties");
@@ -390,14 +405,16 @@ void GpuForwardBase::pushProperties()
 void GpuForwardBase::upload(const Vec& params)
 {
     m_model->applyParameterVector(params);
-    check(lf_set_vertices(m_ctx, m_model->vertices().data()), m_ctx, "lf_set_vertices");
+    lf_ctx* c = ctx();
+    check(lf_set_vertices(c, m_model->vertices().data()), c, "lf_set_vertices");
 }
 
 Vec GpuForwardBase::forwardRaw(int which)
 {
     Vec out(m_nData);
-    check(lf_forward(m_ctx, which == 0 ? out.data() : nullptr, which == 1 ? out.data() : nullptr),
-          m_ctx, "lf_forward");
+    lf_ctx* c = ctx();
+    check(lf_forward(c, which == 0 ? out.data() : nullptr, which == 1 ? out.data() : nullptr),
+          c, "lf_forward");
     return out;
 }
 
@@ -406,8 +423,8 @@ Vec GpuForwardBase::computeJTr(const Vec& params, const Vec& w)
     upload(params);
     Vec vg(3 * m_model->vertexCount());
     const bool gz = (m_want & LF_WANT_GZ) != 0;
-    check(lf_adjoint(m_ctx, gz ? w.data() : nullptr, gz ? nullptr : w.data(), vg.data()), m_ctx,
-          "lf_adjoint");
+    check(lf_adjoint(m_ctx, gz ? w.data() : nullptr, gz ? nullptr : w.data(), vg.data()),
+          m_ctx, "lf_adjoint");  // upload() created the context
     return m_model->pullBackVertexGradient(vg);
 }
 
@@ -482,7 +499,7 @@ MagneticForward::MagneticForward(std::shared_ptr<LithologyModel> model, const Ma
       m_field(field_nT)
 {
     fieldUnitVector(inc_deg, dec_deg, m_fhat);
-    (void)textbook;
+    m_textbook = textbook;
     pushProperties();
 }
 

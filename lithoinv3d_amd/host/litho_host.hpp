@@ -177,6 +177,7 @@ public:
 protected:
     GpuForwardBase(std::shared_ptr<LithologyModel> model, std::vector<double> stations,
                    int want, Precision prec, const double fdir[3], int device);
+    lf_ctx* ctx();  // creates the device context on first use
     void upload(const Vec& params);
     void pushProperties();
     Vec forwardRaw(int which);  // 0 = gz, 1 = tmi
@@ -186,6 +187,11 @@ protected:
     lf_ctx* m_ctx = nullptr;
     size_t m_nData = 0;
     int m_want = 0;
+    std::vector<double> m_stations;
+    Precision m_prec = Precision::FP64;
+    double m_fdir[3] = {0.0, 0.0, 1.0};
+    int m_device = 0;
+    bool m_textbook = false;
     std::string m_err;
 };
 
