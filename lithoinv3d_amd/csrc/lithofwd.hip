@@ -197,6 +197,11 @@ __device__ __forceinline__ void forward_pair(const FaceF32& F, const double* __r
 // traffic.
 constexpr int kStage = 64;
 static_assert(kStage * sizeof(FaceF32) == kBlock * 32, "one 32-byte piece per thread");
+// stations per lane: each record read from LDS serves this many pairs
+#ifndef LF_FORWARD_SPL
+#define LF_FORWARD_SPL 1
+#endif
+constexpr int kSPL = LF_FORWARD_SPL;
 
 template <bool GZ, bool TMI>
 __global__ __launch_bounds__(256) LF_FORWARD_OCC void k_forward_f32c(
@@ -208,10 +213,15 @@ __global__ __launch_bounds__(256) LF_FORWARD_OCC void k_forward_f32c(
     __shared__ FaceF32 stage[2][kStage];
     int tile, chunk;
     block_coords(n_tiles, tile, chunk);
-    const int64_t s = (int64_t)tile * kBlock + threadIdx.x;
-    const bool active = s < n_s;
-    const int64_t si = active ? s : n_s - 1;
-    const double ox = sx[si], oy = sy[si], oz = sz[si];
+    double ox[kSPL], oy[kSPL], oz[kSPL];
+#pragma unroll
+    for (int k = 0; k < kSPL; ++k) {
+        const int64_t s = ((int64_t)tile * kSPL + k) * kBlock + threadIdx.x;
+        const int64_t si = s < n_s ? s : n_s - 1;
+        ox[k] = sx[si];
+        oy[k] = sy[si];
+        oz[k] = sz[si];
+    }
     const int64_t f0 = (int64_t)chunk * faces_per_chunk;
     const int64_t f1 = min(n_f, f0 + faces_per_chunk);
 
@@ -233,7 +243,9 @@ __global__ __launch_bounds__(256) LF_FORWARD_OCC void k_forward_f32c(
         dst[1] = q1;
     };
 
-    float gs = 0.f, gc = 0.f, ts = 0.f, tc = 0.f;
+    float gs[kSPL], gc[kSPL], ts[kSPL], tc[kSPL];
+#pragma unroll
+    for (int k = 0; k < kSPL; ++k) gs[k] = gc[k] = ts[k] = tc[k] = 0.f;
     unsigned nfb = 0;
     if (f0 < f1) {
         fetch(f0);
@@ -247,18 +259,25 @@ __global__ __launch_bounds__(256) LF_FORWARD_OCC void k_forward_f32c(
         const int nst = (int)min((int64_t)kStage, f1 - fs);
         for (int j = 0; j < nst; ++j) {
             const FaceF32 F = stage[b][j];
-            forward_pair<GZ, TMI>(F, geo64, fs + j, ox, oy, oz, gs, gc, ts, tc, nfb);
+#pragma unroll
+            for (int k = 0; k < kSPL; ++k)
+                forward_pair<GZ, TMI>(F, geo64, fs + j, ox[k], oy[k], oz[k], gs[k], gc[k], ts[k],
+                                      tc[k], nfb);
         }
         // the other buffer was last read before the previous barrier
         if (more) park(b ^ 1);
         __syncthreads();
         b ^= 1;
     }
-    if (active) {
-        if (GZ) part_gz[(int64_t)chunk * n_s + s] = (double)gs - (double)gc;
-        if (TMI) part_tmi[(int64_t)chunk * n_s + s] = (double)ts - (double)tc;
-        if (nfb) atomicAdd(fb_count, (unsigned long long)nfb);
+#pragma unroll
+    for (int k = 0; k < kSPL; ++k) {
+        const int64_t s = ((int64_t)tile * kSPL + k) * kBlock + threadIdx.x;
+        if (s < n_s) {
+            if (GZ) part_gz[(int64_t)chunk * n_s + s] = (double)gs[k] - (double)gc[k];
+            if (TMI) part_tmi[(int64_t)chunk * n_s + s] = (double)ts[k] - (double)tc[k];
+        }
     }
+    if (nfb) atomicAdd(fb_count, (unsigned long long)nfb);
 }
 
 // ------------------------------------------------------------------ FP64 forward
@@ -448,7 +467,8 @@ struct TimedLaunch {
 
 int choose_chunks(lf_ctx* c)
 {
-    c->n_tiles = (int)((c->n_s + kBlock - 1) / kBlock);
+    const int64_t tile_st = (int64_t)kBlock * (c->precision == LF_FP32C ? kSPL : 1);
+    c->n_tiles = (int)((c->n_s + tile_st - 1) / tile_st);
     int dev_cus = 256;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)

@@ -8,6 +8,7 @@
 #include <string>
 
 #include "litho_host.hpp"
+#include "litho_io.hpp"
 
 using namespace lithofwd;
 
@@ -47,6 +48,21 @@ struct Problem {
 
 struct ModelHandle {
     std::shared_ptr<LithologyModel> m;
+};
+
+struct MeshHandle {
+    SurfaceMeshData m;
+    std::string warnings;
+};
+struct StationsHandle {
+    std::vector<StationRecord> s;
+    std::string warnings;
+};
+struct GridHandle {
+    LabelGridData g;
+};
+struct AssembledHandle {
+    AssembledModel a;
 };
 }  // namespace
 
@@ -368,5 +384,179 @@ int li_problem_minimize(void* h, const double* x0, const double* lb, const doubl
     return run_minimize(obj, grad, n, x0, lb, ub, o, h0, x_out, hist, hist_cap, n_hist, iterations,
                         msg, msg_cap);
 }
+
+// ------------------------------------------------------------------ file formats
+
+int li_mesh_load(const char* path, void** out)
+{
+    return guard([&] {
+        auto h = std::make_unique<MeshHandle>();
+        h->m = loadSurfaceMesh(path, &h->warnings);
+        *out = h.release();
+    });
+}
+
+int li_mesh_counts(void* h, int64_t* n_vertices, int64_t* n_triangles)
+{
+    auto* H = (MeshHandle*)h;
+    *n_vertices = (int64_t)H->m.vertexCount();
+    *n_triangles = (int64_t)H->m.triangleCount();
+    return 0;
+}
+
+int li_mesh_get(void* h, double* vertices, uint32_t* triangles)
+{
+    auto* H = (MeshHandle*)h;
+    std::memcpy(vertices, H->m.vertices.data(), sizeof(double) * H->m.vertices.size());
+    std::memcpy(triangles, H->m.triangles.data(), sizeof(uint32_t) * H->m.triangles.size());
+    return 0;
+}
+
+const char* li_mesh_warnings(void* h) { return ((MeshHandle*)h)->warnings.c_str(); }
+
+void li_mesh_destroy(void* h) { delete (MeshHandle*)h; }
+
+int li_mesh_write_ts(const char* path, const char* name, int64_t n_vertices,
+                     const double* vertices, int64_t n_triangles, const uint32_t* triangles)
+{
+    return guard([&] {
+        SurfaceMeshData m;
+        m.vertices.assign(vertices, vertices + 3 * n_vertices);
+        m.triangles.assign(triangles, triangles + 3 * n_triangles);
+        for (uint32_t t : m.triangles)
+            if ((int64_t)t >= n_vertices)
+                throw std::invalid_argument("li_mesh_write_ts: triangle index out of range");
+        writeTSurf(m, path, name);
+    });
+}
+
+int li_stations_load(const char* path, void** out)
+{
+    return guard([&] {
+        auto h = std::make_unique<StationsHandle>();
+        h->s = loadStationCSV(path, &h->warnings);
+        *out = h.release();
+    });
+}
+
+int64_t li_stations_count(void* h) { return (int64_t)((StationsHandle*)h)->s.size(); }
+
+int li_stations_get(void* h, double* xyz, double* obs, double* stdev)
+{
+    const auto& S = ((StationsHandle*)h)->s;
+    for (size_t i = 0; i < S.size(); ++i) {
+        xyz[3 * i] = S[i].x;
+        xyz[3 * i + 1] = S[i].y;
+        xyz[3 * i + 2] = S[i].z;
+        obs[i] = S[i].obs;
+        stdev[i] = S[i].stdev;
+    }
+    return 0;
+}
+
+const char* li_stations_warnings(void* h) { return ((StationsHandle*)h)->warnings.c_str(); }
+
+void li_stations_destroy(void* h) { delete (StationsHandle*)h; }
+
+int li_label_grid_load(const char* path, void** out)
+{
+    return guard([&] {
+        auto h = std::make_unique<GridHandle>();
+        h->g = loadLabelGrid(path);
+        *out = h.release();
+    });
+}
+
+int li_label_grid_info(void* h, int32_t* dims, double* origin, double* cell)
+{
+    const auto& g = ((GridHandle*)h)->g;
+    dims[0] = g.nx;
+    dims[1] = g.ny;
+    dims[2] = g.nz;
+    origin[0] = g.x0;
+    origin[1] = g.y0;
+    origin[2] = g.z0;
+    cell[0] = g.dx;
+    cell[1] = g.dy;
+    cell[2] = g.dz;
+    return 0;
+}
+
+int li_label_grid_get(void* h, int32_t* labels)
+{
+    const auto& g = ((GridHandle*)h)->g;
+    std::memcpy(labels, g.labels.data(), sizeof(int32_t) * g.labels.size());
+    return 0;
+}
+
+void li_label_grid_destroy(void* h) { delete (GridHandle*)h; }
+
+int li_label_grid_write(const char* path, const int32_t* dims, const double* origin,
+                        const double* cell, const int32_t* labels)
+{
+    return guard([&] {
+        LabelGridData g;
+        g.nx = dims[0];
+        g.ny = dims[1];
+        g.nz = dims[2];
+        if (g.nx <= 0 || g.ny <= 0 || g.nz <= 0)
+            throw std::invalid_argument("li_label_grid_write: non-positive grid size");
+        g.x0 = origin[0];
+        g.y0 = origin[1];
+        g.z0 = origin[2];
+        g.dx = cell[0];
+        g.dy = cell[1];
+        g.dz = cell[2];
+        g.labels.assign(labels, labels + (size_t)g.nx * g.ny * g.nz);
+        writeLabelGrid(g, path);
+    });
+}
+
+// Units' meshes concatenated: unit u owns vertices [v_off[u], v_off[u+1]) of verts and
+// triangles [t_off[u], t_off[u+1]) of tris, whose indices are local to the unit.
+int li_assemble(int32_t n_units, const int64_t* v_off, const double* verts, const int64_t* t_off,
+                const uint32_t* tris, void** out)
+{
+    return guard([&] {
+        if (n_units < 0) throw std::invalid_argument("li_assemble: negative unit count");
+        std::vector<SurfaceMeshData> meshes(n_units);
+        for (int32_t u = 0; u < n_units; ++u) {
+            const int64_t v0 = v_off[u], v1 = v_off[u + 1], t0 = t_off[u], t1 = t_off[u + 1];
+            if (v1 < v0 || t1 < t0) throw std::invalid_argument("li_assemble: offsets not monotone");
+            meshes[u].vertices.assign(verts + 3 * v0, verts + 3 * v1);
+            meshes[u].triangles.assign(tris + 3 * t0, tris + 3 * t1);
+            for (uint32_t t : meshes[u].triangles)
+                if ((int64_t)t >= v1 - v0)
+                    throw std::invalid_argument("li_assemble: unit " + std::to_string(u) +
+                                                ": triangle index out of range");
+        }
+        auto h = std::make_unique<AssembledHandle>();
+        h->a = assembleUnitMeshes(meshes);
+        *out = h.release();
+    });
+}
+
+int li_assembled_counts(void* h, int64_t* n_vertices, int64_t* n_faces, int64_t* n_contacts,
+                        int64_t* n_merged)
+{
+    const auto& a = ((AssembledHandle*)h)->a;
+    *n_vertices = (int64_t)(a.vertices.size() / 3);
+    *n_faces = (int64_t)a.front.size();
+    *n_contacts = a.contacts;
+    *n_merged = a.mergedVertices;
+    return 0;
+}
+
+int li_assembled_get(void* h, double* vertices, uint32_t* faces, int32_t* front, int32_t* back)
+{
+    const auto& a = ((AssembledHandle*)h)->a;
+    std::memcpy(vertices, a.vertices.data(), sizeof(double) * a.vertices.size());
+    std::memcpy(faces, a.faces.data(), sizeof(uint32_t) * a.faces.size());
+    std::memcpy(front, a.front.data(), sizeof(int32_t) * a.front.size());
+    std::memcpy(back, a.back.data(), sizeof(int32_t) * a.back.size());
+    return 0;
+}
+
+void li_assembled_destroy(void* h) { delete (AssembledHandle*)h; }
 
 }  // extern "C"

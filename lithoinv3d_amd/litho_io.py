@@ -1,0 +1,204 @@
+"""File formats of LithoInv3D around the forward path, through the C++ host layer
+(lithoinv3d_amd/host/litho_io.cpp): GOCAD TSurf / OBJ unit surfaces
+(modules/litho-io/src/io/surface_loader.cpp, exporters.cpp:116-150), CSV gravity and
+magnetic stations (gravity_loader.cpp:8-69, magnetic_loader.cpp:8-69), the binary label
+grid (python/lithoseed/src/lithoseed/_surfaces.py:659-671), and the assembly of per-unit
+closed meshes into the shared-vertex facet set the GPU forward consumes."""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import os
+
+import numpy as np
+
+from . import inversion, synth
+
+_P = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_declared = False
+
+
+def _lib():
+    global _declared
+    L = inversion.load_host()
+    if not _declared:
+        L.li_mesh_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(_P)]
+        L.li_mesh_counts.argtypes = [_P, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]
+        L.li_mesh_get.argtypes = [_P, _P, _P]
+        L.li_mesh_warnings.argtypes = [_P]
+        L.li_mesh_warnings.restype = ctypes.c_char_p
+        L.li_mesh_destroy.argtypes = [_P]
+        L.li_mesh_destroy.restype = None
+        L.li_mesh_write_ts.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _i64, _P, _i64, _P]
+        L.li_stations_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(_P)]
+        L.li_stations_count.argtypes = [_P]
+        L.li_stations_count.restype = _i64
+        L.li_stations_get.argtypes = [_P, _P, _P, _P]
+        L.li_stations_warnings.argtypes = [_P]
+        L.li_stations_warnings.restype = ctypes.c_char_p
+        L.li_stations_destroy.argtypes = [_P]
+        L.li_stations_destroy.restype = None
+        L.li_label_grid_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(_P)]
+        L.li_label_grid_info.argtypes = [_P, _P, _P, _P]
+        L.li_label_grid_get.argtypes = [_P, _P]
+        L.li_label_grid_destroy.argtypes = [_P]
+        L.li_label_grid_destroy.restype = None
+        L.li_label_grid_write.argtypes = [ctypes.c_char_p, _P, _P, _P, _P]
+        L.li_assemble.argtypes = [ctypes.c_int32, _P, _P, _P, _P, ctypes.POINTER(_P)]
+        L.li_assembled_counts.argtypes = [_P] + [ctypes.POINTER(_i64)] * 4
+        L.li_assembled_get.argtypes = [_P, _P, _P, _P, _P]
+        L.li_assembled_destroy.argtypes = [_P]
+        L.li_assembled_destroy.restype = None
+        _declared = True
+    return L
+
+
+def _chk(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what}: {inversion.load_host().li_last_error().decode()}")
+
+
+def _p(a):
+    return a.ctypes.data_as(_P)
+
+
+def _path(p):
+    return os.fsencode(os.fspath(p))
+
+
+@dataclasses.dataclass
+class Mesh:
+    vertices: np.ndarray   # (n, 3) float64
+    triangles: np.ndarray  # (m, 3) uint32, 0-based
+    warnings: str = ""
+
+
+def read_surface(path) -> Mesh:
+    """A .ts (GOCAD TSurf) or .obj unit surface, by extension."""
+    L = _lib()
+    h = _P()
+    _chk(L.li_mesh_load(_path(path), ctypes.byref(h)), f"read_surface({path})")
+    try:
+        nv, nt = _i64(), _i64()
+        L.li_mesh_counts(h, ctypes.byref(nv), ctypes.byref(nt))
+        v = np.zeros((nv.value, 3))
+        t = np.zeros((nt.value, 3), np.uint32)
+        L.li_mesh_get(h, _p(v), _p(t))
+        w = L.li_mesh_warnings(h).decode()
+    finally:
+        L.li_mesh_destroy(h)
+    return Mesh(v, t, w)
+
+
+def write_ts(path, vertices, triangles, name="surface"):
+    """GOCAD TSurf in the reference exporter's layout (8-digit scientific coordinates)."""
+    v = np.ascontiguousarray(vertices, np.float64).reshape(-1, 3)
+    t = np.ascontiguousarray(triangles, np.uint32).reshape(-1, 3)
+    _chk(_lib().li_mesh_write_ts(_path(path), name.encode(), len(v), _p(v), len(t), _p(t)),
+         f"write_ts({path})")
+
+
+@dataclasses.dataclass
+class Stations:
+    xyz: np.ndarray    # (n, 3)
+    obs: np.ndarray    # mGal or nT
+    stdev: np.ndarray  # 0 = unweighted
+    warnings: str = ""
+
+
+def read_stations(path) -> Stations:
+    """x, y, z, observation[, standard deviation] per line; '#' comments and header
+    lines (first field starting with a letter) skipped, short lines skipped."""
+    L = _lib()
+    h = _P()
+    _chk(L.li_stations_load(_path(path), ctypes.byref(h)), f"read_stations({path})")
+    try:
+        n = L.li_stations_count(h)
+        xyz, obs, sd = np.zeros((n, 3)), np.zeros(n), np.zeros(n)
+        L.li_stations_get(h, _p(xyz), _p(obs), _p(sd))
+        w = L.li_stations_warnings(h).decode()
+    finally:
+        L.li_stations_destroy(h)
+    return Stations(xyz, obs, sd, w)
+
+
+@dataclasses.dataclass
+class LabelGrid:
+    labels: np.ndarray  # (nx, ny, nz) int32
+    origin: np.ndarray  # x0, y0, z0
+    cell: np.ndarray    # dx, dy, dz
+
+
+def read_label_grid(path) -> LabelGrid:
+    L = _lib()
+    h = _P()
+    _chk(L.li_label_grid_load(_path(path), ctypes.byref(h)), f"read_label_grid({path})")
+    try:
+        dims = np.zeros(3, np.int32)
+        origin, cell = np.zeros(3), np.zeros(3)
+        L.li_label_grid_info(h, _p(dims), _p(origin), _p(cell))
+        flat = np.zeros(int(np.prod(dims)), np.int32)
+        L.li_label_grid_get(h, _p(flat))
+    finally:
+        L.li_label_grid_destroy(h)
+    return LabelGrid(flat.reshape(tuple(dims), order="F"), origin, cell)
+
+
+def write_label_grid(path, labels, origin, cell):
+    lab = np.asarray(labels, np.int32)
+    dims = np.array(lab.shape, np.int32)
+    flat = np.ascontiguousarray(lab.ravel(order="F"))
+    o = np.ascontiguousarray(origin, np.float64)
+    c = np.ascontiguousarray(cell, np.float64)
+    _chk(_lib().li_label_grid_write(_path(path), _p(dims), _p(o), _p(c), _p(flat)),
+         f"write_label_grid({path})")
+
+
+@dataclasses.dataclass
+class Assembly:
+    vertices: np.ndarray
+    faces: np.ndarray
+    front: np.ndarray
+    back: np.ndarray
+    contacts: int
+    merged_vertices: int
+
+
+def assemble(meshes) -> Assembly:
+    """Per-unit meshes (each a Mesh or a (vertices, triangles) pair, unit u = position in
+    the list) -> one shared-vertex facet set; a triangle whose reversed copy is in another
+    unit becomes one facet with (front, back)."""
+    vs, ts = [], []
+    for m in meshes:
+        v, t = (m.vertices, m.triangles) if isinstance(m, Mesh) else m
+        vs.append(np.asarray(v, np.float64).reshape(-1, 3))
+        ts.append(np.asarray(t, np.uint32).reshape(-1, 3))
+    v_off = np.zeros(len(vs) + 1, np.int64)
+    t_off = np.zeros(len(ts) + 1, np.int64)
+    v_off[1:] = np.cumsum([len(v) for v in vs])
+    t_off[1:] = np.cumsum([len(t) for t in ts])
+    V = np.ascontiguousarray(np.concatenate(vs) if vs else np.zeros((0, 3)))
+    T = np.ascontiguousarray(np.concatenate(ts) if ts else np.zeros((0, 3), np.uint32))
+    L = _lib()
+    h = _P()
+    _chk(L.li_assemble(len(vs), _p(v_off), _p(V), _p(t_off), _p(T), ctypes.byref(h)), "assemble")
+    try:
+        nv, nf, nc, nm = _i64(), _i64(), _i64(), _i64()
+        L.li_assembled_counts(h, ctypes.byref(nv), ctypes.byref(nf), ctypes.byref(nc),
+                              ctypes.byref(nm))
+        verts = np.zeros((nv.value, 3))
+        faces = np.zeros((nf.value, 3), np.uint32)
+        front = np.zeros(nf.value, np.int32)
+        back = np.zeros(nf.value, np.int32)
+        L.li_assembled_get(h, _p(verts), _p(faces), _p(front), _p(back))
+    finally:
+        L.li_assembled_destroy(h)
+    return Assembly(verts, faces, front, back, int(nc.value), int(nm.value))
+
+
+def litho_model(meshes, density, susceptibility) -> synth.LithoModel:
+    """The forward's model from per-unit meshes and per-unit properties."""
+    a = assemble(meshes)
+    return synth.LithoModel(a.vertices, a.faces, a.front, a.back,
+                            np.asarray(density, np.float64), np.asarray(susceptibility, np.float64))
