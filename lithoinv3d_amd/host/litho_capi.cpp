@@ -185,6 +185,23 @@ int li_model_bounds(void* h, const double* params, double margin_m, double* lo, 
 }
 
 // Joint gravity + magnetic problem on one station set.  *_std <= 0 means unweighted.
+// SurfaceSmoothness of the model at params: weight * penalty and its parameter gradient.
+int li_model_smoothness(void* h, const double* params, double weight, double* value,
+                        double* grad)
+{
+    return guard([&] {
+        auto& m = ((ModelHandle*)h)->m;
+        SurfaceSmoothness s(m);
+        s.setWeight(weight);
+        const Vec p(params, params + m->totalDofCount());
+        *value = s.evaluate(p);
+        if (grad) {
+            const Vec g = s.gradient(p);
+            std::memcpy(grad, g.data(), sizeof(double) * g.size());
+        }
+    });
+}
+
 int li_problem_create(void** out, void* model, int64_t n_s, const double* stations,
                       const double* gz_obs, const double* gz_std, const double* tmi_obs,
                       const double* tmi_std, double inc, double dec, double field,
@@ -227,6 +244,18 @@ int li_problem_create(void** out, void* model, int64_t n_s, const double* statio
 }
 
 void li_problem_destroy(void* h) { delete (Problem*)h; }
+
+// Adds SurfaceSmoothness with this weight to the problem's objectives.
+int li_problem_add_smoothness(void* h, double weight)
+{
+    return guard([&] {
+        auto* P = (Problem*)h;
+        auto s = std::make_shared<SurfaceSmoothness>(P->model);
+        s->setWeight(weight);
+        P->gobj->addRegularization(s);
+        if (P->fused) P->fused->addRegularization(s);
+    });
+}
 
 // 1: the fused one-context objective (default when gravity and magnetic data share the
 // stations); 0: the reference-shaped JointObjective over separate forwards

@@ -83,6 +83,20 @@ void LithologyModel::setAxisScale(int axis, double s)
 
 bool LithologyModel::isControlPoint(size_t v) const { return m_cpStride <= 0 || m_isCP[v]; }
 
+namespace {
+// axes a vertex of this freedom may move along
+bool axisFree(VertexFreedom f, int a)
+{
+    switch (f) {
+        case VertexFreedom::XYZ_FREE: return true;
+        case VertexFreedom::X_ONLY: return a == 0;
+        case VertexFreedom::Y_ONLY: return a == 1;
+        case VertexFreedom::Z_ONLY: return a == 2;
+        default: return false;
+    }
+}
+}  // namespace
+
 uint32_t LithologyModel::setControlPointStride(int stride, double cellSize)
 {
     const size_t nv = vertexCount();
@@ -241,10 +255,12 @@ void LithologyModel::applyParameterVector(const Vec& params)
     for (size_t d = 0; d < params.size(); ++d)
         m_vertices[3 * (size_t)m_dofVertex[d] + m_dofAxis[d]] = params[d] * m_axisScale[m_dofAxis[d]];
     if (m_cpStride > 0) {
-        // displacement of each control point, spread to its dependants
+        // displacement of each control point, spread to its dependants along the
+        // dependant's own free axes (a FIXED vertex never moves)
         for (size_t v = 0; v < vertexCount(); ++v) {
             if (m_isCP[v]) continue;
             for (int a = 0; a < 3; ++a) {
+                if (!axisFree(m_freedom[v], a)) continue;
                 double disp = 0.0;
                 for (int64_t e = m_wOff[v]; e < m_wOff[v + 1]; ++e) {
                     const size_t c = (size_t)m_wCp[e];
@@ -265,7 +281,8 @@ Vec LithologyModel::pullBackVertexGradient(const Vec& vg) const
             if (m_isCP[v]) continue;
             for (int64_t e = m_wOff[v]; e < m_wOff[v + 1]; ++e) {
                 const size_t c = (size_t)m_wCp[e];
-                for (int a = 0; a < 3; ++a) eff[3 * c + a] += m_w[e] * vg[3 * v + a];
+                for (int a = 0; a < 3; ++a)
+                    if (axisFree(m_freedom[v], a)) eff[3 * c + a] += m_w[e] * vg[3 * v + a];
             }
         }
     }
@@ -296,6 +313,7 @@ void LithologyModel::dofVertexMap(std::vector<int64_t>& off, std::vector<int32_t
         w.push_back(s);
         if (m_cpStride > 0)
             for (auto& pr : dep[(size_t)m_dofVertex[d]]) {
+                if (!axisFree(m_freedom[(size_t)pr.first], m_dofAxis[d])) continue;
                 vert.push_back(pr.first);
                 w.push_back(pr.second * s);
             }
@@ -569,6 +587,99 @@ Vec ReferenceModelRegularization::gradient(const Vec& params)
     Vec g(params.size(), 0.0);
     if (!m_captured) return g;
     for (size_t i = 0; i < params.size(); ++i) g[i] = m_weight * (params[i] - m_ref[i]);
+    return g;
+}
+
+// ================================================================ smoothness
+
+SurfaceSmoothness::SurfaceSmoothness(std::shared_ptr<LithologyModel> model)
+    : m_model(std::move(model))
+{
+    const int nu = m_model->groupCount();
+    const auto& F = m_model->faces();
+    const auto& fr = m_model->front();
+    const auto& bk = m_model->back();
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> edges(nu);
+    for (size_t f = 0; f < fr.size(); ++f) {
+        const int32_t a = (int32_t)F[3 * f], b = (int32_t)F[3 * f + 1], c = (int32_t)F[3 * f + 2];
+        for (int32_t u : {fr[f], bk[f]}) {
+            if (u < 0 || u >= nu) continue;
+            auto& E = edges[u];
+            E.insert(E.end(), {{a, b}, {a, c}, {b, a}, {b, c}, {c, a}, {c, b}});
+        }
+    }
+    m_unitVerts.resize(nu);
+    m_nbrOff.resize(nu);
+    m_nbr.resize(nu);
+    for (int u = 0; u < nu; ++u) {
+        auto& E = edges[u];
+        std::sort(E.begin(), E.end());
+        E.erase(std::unique(E.begin(), E.end()), E.end());
+        auto& V = m_unitVerts[u];
+        auto& off = m_nbrOff[u];
+        auto& nb = m_nbr[u];
+        for (size_t i = 0; i < E.size(); ++i) {
+            if (V.empty() || V.back() != E[i].first) {
+                V.push_back(E[i].first);
+                off.push_back((int64_t)nb.size());
+            }
+            nb.push_back(E[i].second);
+        }
+        off.push_back((int64_t)nb.size());
+    }
+}
+
+double SurfaceSmoothness::penalty(std::vector<double>* vgrad) const
+{
+    const auto& X = m_model->vertices();
+    if (vgrad) vgrad->assign(X.size(), 0.0);
+    double pen = 0.0;
+    for (size_t u = 0; u < m_unitVerts.size(); ++u) {
+        const auto& V = m_unitVerts[u];
+        const auto& off = m_nbrOff[u];
+        const auto& nb = m_nbr[u];
+        for (size_t i = 0; i < V.size(); ++i) {
+            const int32_t v = V[i];
+            const VertexFreedom fv = m_model->vertexFreedom(v);
+            if (fv == VertexFreedom::FIXED) continue;
+            double mean[3] = {0.0, 0.0, 0.0};
+            int n = 0;
+            for (int64_t e = off[i]; e < off[i + 1]; ++e)
+                if (m_model->vertexFreedom(nb[e]) == fv) {
+                    for (int a = 0; a < 3; ++a) mean[a] += X[3 * (size_t)nb[e] + a];
+                    ++n;
+                }
+            if (n == 0) continue;
+            double d[3];
+            for (int a = 0; a < 3; ++a) {
+                d[a] = X[3 * (size_t)v + a] - mean[a] / n;
+                if (!axisFree(fv, a)) d[a] = 0.0;
+                pen += 0.5 * d[a] * d[a];
+            }
+            if (!vgrad) continue;
+            // own term d, and -d/n to each same-freedom neighbour
+            for (int a = 0; a < 3; ++a) (*vgrad)[3 * (size_t)v + a] += d[a];
+            for (int64_t e = off[i]; e < off[i + 1]; ++e)
+                if (m_model->vertexFreedom(nb[e]) == fv)
+                    for (int a = 0; a < 3; ++a) (*vgrad)[3 * (size_t)nb[e] + a] -= d[a] / n;
+        }
+    }
+    return pen;
+}
+
+double SurfaceSmoothness::evaluate(const Vec& params)
+{
+    m_model->applyParameterVector(params);
+    return m_weight * penalty(nullptr);
+}
+
+Vec SurfaceSmoothness::gradient(const Vec& params)
+{
+    m_model->applyParameterVector(params);
+    std::vector<double> vg;
+    penalty(&vg);
+    Vec g = m_model->pullBackVertexGradient(vg);
+    for (auto& x : g) x *= m_weight;
     return g;
 }
 

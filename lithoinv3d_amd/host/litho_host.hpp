@@ -257,6 +257,28 @@ private:
     bool m_captured = false;
 };
 
+// modules/litho-regularization/src/regularization/smoothness.cpp:8-110 (penalty) and
+// :112-388 (gradient): for every unit's closed surface and every non-FIXED vertex on it,
+// 0.5 |v - mean of its neighbours on that surface with the same freedom|^2 over the
+// vertex's free axes, summed over units (a contact vertex counts once per unit it bounds).
+// The gradient is exact through the control-point map (the reference reads the
+// per-vertex gradient at control points only).
+class SurfaceSmoothness : public Regularization {
+public:
+    explicit SurfaceSmoothness(std::shared_ptr<LithologyModel> model);
+    double evaluate(const Vec& params) override;
+    Vec gradient(const Vec& params) override;
+    // unweighted penalty at the model's current geometry; vgrad (3 per vertex) if given
+    double penalty(std::vector<double>* vgrad) const;
+
+private:
+    std::shared_ptr<LithologyModel> m_model;
+    // per unit: its vertices and their unique neighbours on the unit's surface (CSR)
+    std::vector<std::vector<int32_t>> m_unitVerts;
+    std::vector<std::vector<int64_t>> m_nbrOff;
+    std::vector<std::vector<int32_t>> m_nbr;
+};
+
 // modules/litho-inversion/src/inversion/objective.cpp:6-103 (gravity data misfit +
 // regularization); the gradient uses computeJTr instead of forming J.
 class ObjectiveFunction {

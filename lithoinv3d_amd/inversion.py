@@ -59,6 +59,8 @@ def load_host():
     L.li_model_apply.argtypes = [_P, _P, _P]
     L.li_model_pullback.argtypes = [_P, _P, _P]
     L.li_model_bounds.argtypes = [_P, _P, ctypes.c_double, _P, _P]
+    L.li_model_smoothness.argtypes = [_P, _P, ctypes.c_double, ctypes.POINTER(ctypes.c_double), _P]
+    L.li_problem_add_smoothness.argtypes = [_P, ctypes.c_double]
     L.li_model_dof_map.argtypes = [_P, _P, _P, _P, ctypes.POINTER(i64)]
     L.li_problem_create.argtypes = [ctypes.POINTER(_P), _P, i64, _P, _P, _P, _P, _P,
                                     ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -173,6 +175,15 @@ class Model:
         _chk(L.li_model_dof_map(self._h, _p(off), _p(vert), _p(w), ctypes.byref(ne)), "dof_map")
         return off, vert, w
 
+    def smoothness(self, params, weight=1.0):
+        """SurfaceSmoothness at params: (weight * penalty, parameter gradient)."""
+        p = _a(params)
+        v = ctypes.c_double()
+        g = np.zeros(len(p))
+        _chk(load_host().li_model_smoothness(self._h, _p(p), weight, ctypes.byref(v), _p(g)),
+             "smoothness")
+        return v.value, g
+
     def bounds(self, params, margin_m):
         p = _a(params)
         lo, hi = np.zeros(len(p)), np.zeros(len(p))
@@ -237,11 +248,11 @@ def minimize(fun, grad, x0, lb, ub, max_iter=100, history=10, ftol=1e-6, c1=1e-4
 class Problem:
     """Joint gravity + magnetic objective (JointObjective) over the GPU forward and
     adjoint for one station set: Phi = sum (w_g r_g)^2/2 + alpha sum (w_t r_t)^2/2
-    (+ reference-model term), w = 1/std."""
+    (+ reference-model term, + surface smoothness), w = 1/std."""
 
     def __init__(self, model: Model, stations, gz_obs, gz_std=None, tmi_obs=None, tmi_std=None,
                  inc=-60.0, dec=0.0, field=50000.0, alpha_mag=1.0, precision="fp64",
-                 ref_weight=0.0, device=0):
+                 ref_weight=0.0, device=0, smooth_weight=0.0):
         L = load_host()
         self.model = model
         st = _a(stations).reshape(-1, 3)
@@ -253,6 +264,8 @@ class Problem:
         _chk(L.li_problem_create(ctypes.byref(self._h), model._h, self.n_s, *[_p(a) for a in arrs],
                                  inc, dec, field, alpha_mag, 1 if precision == "fp32c" else 0,
                                  ref_weight, device), "li_problem_create")
+        if smooth_weight > 0.0:
+            _chk(L.li_problem_add_smoothness(self._h, smooth_weight), "add_smoothness")
 
     def close(self):
         if self._h:

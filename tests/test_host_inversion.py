@@ -110,9 +110,69 @@ def test_control_points_map_and_transpose(stride):
     dV = M.apply(p0 + dp) - M.apply(p0)
     # every vertex moves by a convex combination of control-point displacements
     assert np.abs(dV[:, 2]).max() <= np.abs(dp).max() * 10.0 + 1e-9
+    # exterior (FIXED) vertices stay where they are
+    ext = np.zeros(m.n_vertices, bool)
+    ext[m.faces[m.back < 0].ravel()] = True
+    assert np.abs(dV[ext]).max() == 0.0
     vg = rng.normal(size=(m.n_vertices, 3))
     # <vg, A dp> == <A^T vg, dp>
     assert np.sum(vg * dV) == pytest.approx(M.pullback(vg) @ dp, rel=1e-12)
     lo, hi = M.bounds(p0, 200.0)
     np.testing.assert_allclose(hi - p0, 20.0)
     np.testing.assert_allclose(p0 - lo, 20.0)
+
+
+def reference_smoothness(m, freedom, V):
+    """Restatement of SurfaceSmoothness::evaluate (modules/litho-regularization/src/
+    regularization/smoothness.cpp:20-110) on the per-unit layout: each unit's closed mesh
+    (its facets, contacts included) with unique neighbour lists (surface_mesh.cpp:265-286),
+    same-freedom neighbours only, the free axes of the vertex's freedom."""
+    axes = {int(inversion.Z_ONLY): [2], int(inversion.X_ONLY): [0], int(inversion.Y_ONLY): [1],
+            int(inversion.XYZ_FREE): [0, 1, 2]}
+    pen = 0.0
+    for u in range(m.n_units):
+        tris = m.faces[(m.front == u) | (m.back == u)]
+        nbrs = {}
+        for a, b, c in tris:
+            for p, q in ((a, b), (a, c), (b, a), (b, c), (c, a), (c, b)):
+                nbrs.setdefault(int(p), set()).add(int(q))
+        for v, nb in nbrs.items():
+            if freedom[v] == inversion.FIXED:
+                continue
+            same = [n for n in sorted(nb) if freedom[n] == freedom[v]]
+            if not same:
+                continue
+            d = V[v] - V[same].mean(0)
+            pen += 0.5 * sum(d[a] ** 2 for a in axes[int(freedom[v])])
+    return pen
+
+
+@pytest.mark.parametrize("stride", [0, 2])
+def test_surface_smoothness(stride):
+    m = small_model()
+    M = inversion.Model(m)
+    M.fix_exterior(inversion.Z_ONLY)
+    M.set_axis_scale(10.0)
+    if stride:
+        M.set_cp_stride(stride, 25.0)
+    freedom = np.full(m.n_vertices, inversion.Z_ONLY)
+    freedom[m.faces[m.back < 0].ravel()] = inversion.FIXED
+    rng = np.random.default_rng(5 + stride)
+    p = M.params() + rng.normal(0, 1.0, M.n_dof)
+    w = 0.3
+    val, g = M.smoothness(p, w)
+    assert val == pytest.approx(w * reference_smoothness(m, freedom, M.apply(p)), rel=1e-12)
+    assert val > 0
+    # exact gradient through the parameter map: central differences of the value
+    h = 1e-4
+    for j in rng.choice(len(p), 12, replace=False):
+        e = np.zeros(len(p))
+        e[j] = h
+        fd = (M.smoothness(p + e, w)[0] - M.smoothness(p - e, w)[0]) / (2 * h)
+        assert fd == pytest.approx(g[j], rel=1e-6, abs=1e-9 * np.abs(g).max())
+    if stride == 0:
+        # every parameter + c shifts every free vertex by the same amount: each vertex's
+        # offset from its same-freedom neighbours is unchanged, so is the value, and the
+        # gradient sums to zero
+        assert M.smoothness(p + 3.0, w)[0] == pytest.approx(val, rel=1e-9)
+        assert abs(g.sum()) < 1e-9 * np.abs(g).sum()
