@@ -245,6 +245,58 @@ int li_problem_create(void** out, void* model, int64_t n_s, const double* statio
 
 void li_problem_destroy(void* h) { delete (Problem*)h; }
 
+namespace {
+std::vector<ContactPick> make_picks(int64_t n, const double* xyz, const int32_t* above,
+                                    const int32_t* below, const double* sigma)
+{
+    std::vector<ContactPick> picks((size_t)n);
+    for (int64_t k = 0; k < n; ++k)
+        picks[(size_t)k] = {xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2], above[k], below[k], sigma[k]};
+    return picks;
+}
+}  // namespace
+
+// BoreholeContactTerm of the model at params: weight * misfit, its parameter gradient (if
+// grad) and the model elevation of each pick (if zmodel; NaN where the line misses).
+int li_model_boreholes(void* h, int64_t n, const double* xyz, const int32_t* above,
+                       const int32_t* below, const double* sigma, double radius,
+                       const double* params, double weight, double* value, double* grad,
+                       double* zmodel)
+{
+    return guard([&] {
+        auto& m = ((ModelHandle*)h)->m;
+        BoreholeContactTerm b(m, make_picks(n, xyz, above, below, sigma), radius);
+        b.setWeight(weight);
+        const Vec p(params, params + m->totalDofCount());
+        *value = b.evaluate(p);
+        if (grad) {
+            const Vec g = b.gradient(p);
+            std::memcpy(grad, g.data(), sizeof(double) * g.size());
+        }
+        if (zmodel) {
+            std::vector<double> z;
+            b.misfit(nullptr, &z);
+            std::memcpy(zmodel, z.data(), sizeof(double) * z.size());
+        }
+    });
+}
+
+// Adds a BoreholeContactTerm with this weight to the problem's objectives.
+int li_problem_add_boreholes(void* h, int64_t n, const double* xyz, const int32_t* above,
+                             const int32_t* below, const double* sigma, double radius,
+                             double weight)
+{
+    return guard([&] {
+        auto* P = (Problem*)h;
+        auto b = std::make_shared<BoreholeContactTerm>(P->model,
+                                                       make_picks(n, xyz, above, below, sigma),
+                                                       radius);
+        b->setWeight(weight);
+        P->gobj->addRegularization(b);
+        if (P->fused) P->fused->addRegularization(b);
+    });
+}
+
 // Adds SurfaceSmoothness with this weight to the problem's objectives.
 int li_problem_add_smoothness(void* h, double weight)
 {
@@ -486,6 +538,39 @@ int li_stations_get(void* h, double* xyz, double* obs, double* stdev)
 const char* li_stations_warnings(void* h) { return ((StationsHandle*)h)->warnings.c_str(); }
 
 void li_stations_destroy(void* h) { delete (StationsHandle*)h; }
+
+struct ConstraintsHandle {
+    std::vector<ConstraintRecord> c;
+    std::string warnings;
+};
+
+int li_constraints_load(const char* path, void** out)
+{
+    return guard([&] {
+        auto h = std::make_unique<ConstraintsHandle>();
+        h->c = loadConstraintCSV(path, &h->warnings);
+        *out = h.release();
+    });
+}
+
+int64_t li_constraints_count(void* h) { return (int64_t)((ConstraintsHandle*)h)->c.size(); }
+
+int li_constraints_get(void* h, double* xy, double* z_top, double* z_bottom, int32_t* unit)
+{
+    const auto& C = ((ConstraintsHandle*)h)->c;
+    for (size_t i = 0; i < C.size(); ++i) {
+        xy[2 * i] = C[i].x;
+        xy[2 * i + 1] = C[i].y;
+        z_top[i] = C[i].z_top;
+        z_bottom[i] = C[i].z_bottom;
+        unit[i] = C[i].unit;
+    }
+    return 0;
+}
+
+const char* li_constraints_warnings(void* h) { return ((ConstraintsHandle*)h)->warnings.c_str(); }
+
+void li_constraints_destroy(void* h) { delete (ConstraintsHandle*)h; }
 
 int li_label_grid_load(const char* path, void** out)
 {

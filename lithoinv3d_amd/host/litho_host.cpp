@@ -683,6 +683,113 @@ Vec SurfaceSmoothness::gradient(const Vec& params)
     return g;
 }
 
+// ================================================================ boreholes
+
+BoreholeContactTerm::BoreholeContactTerm(std::shared_ptr<LithologyModel> model,
+                                         std::vector<ContactPick> picks, double searchRadius)
+    : m_model(std::move(model)), m_picks(std::move(picks))
+{
+    const auto& X = m_model->vertices();
+    const auto& F = m_model->faces();
+    const auto& fr = m_model->front();
+    const auto& bk = m_model->back();
+    m_cand.resize(m_picks.size());
+    for (size_t k = 0; k < m_picks.size(); ++k) {
+        const ContactPick& p = m_picks[k];
+        if (!(p.sigma > 0.0)) throw std::invalid_argument("BoreholeContactTerm: sigma must be > 0");
+        for (size_t f = 0; f < fr.size(); ++f) {
+            const bool pair = (fr[f] == p.above && bk[f] == p.below) ||
+                              (fr[f] == p.below && bk[f] == p.above);
+            if (!pair) continue;
+            double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300};
+            for (int c = 0; c < 3; ++c)
+                for (int a = 0; a < 2; ++a) {
+                    const double v = X[3 * (size_t)F[3 * f + c] + a];
+                    lo[a] = std::min(lo[a], v);
+                    hi[a] = std::max(hi[a], v);
+                }
+            if (p.x >= lo[0] - searchRadius && p.x <= hi[0] + searchRadius &&
+                p.y >= lo[1] - searchRadius && p.y <= hi[1] + searchRadius)
+                m_cand[k].push_back((int64_t)f);
+        }
+    }
+}
+
+double BoreholeContactTerm::misfit(std::vector<double>* vgrad, std::vector<double>* zmodel) const
+{
+    const auto& X = m_model->vertices();
+    const auto& F = m_model->faces();
+    if (vgrad) vgrad->assign(X.size(), 0.0);
+    if (zmodel) zmodel->assign(m_picks.size(), std::numeric_limits<double>::quiet_NaN());
+    double total = 0.0;
+    for (size_t k = 0; k < m_picks.size(); ++k) {
+        const ContactPick& p = m_picks[k];
+        double best = std::numeric_limits<double>::infinity(), zb = 0.0;
+        int64_t fb = -1;
+        double lb[3] = {0, 0, 0}, sxb = 0, syb = 0;
+        for (int64_t f : m_cand[k]) {
+            const double* P0 = &X[3 * (size_t)F[3 * f]];
+            const double* P1 = &X[3 * (size_t)F[3 * f + 1]];
+            const double* P2 = &X[3 * (size_t)F[3 * f + 2]];
+            const double x10 = P1[0] - P0[0], y10 = P1[1] - P0[1];
+            const double x20 = P2[0] - P0[0], y20 = P2[1] - P0[1];
+            const double d = x10 * y20 - x20 * y10;  // twice the signed horizontal area
+            const double scale = std::abs(x10 * y20) + std::abs(x20 * y10);
+            if (!(std::abs(d) > 1e-12 * scale)) continue;  // vertical facet
+            const double qx = p.x - P0[0], qy = p.y - P0[1];
+            const double l1 = (qx * y20 - x20 * qy) / d;
+            const double l2 = (x10 * qy - qx * y10) / d;
+            const double l0 = 1.0 - l1 - l2;
+            const double tol = -1e-12;
+            if (l0 < tol || l1 < tol || l2 < tol) continue;
+            const double z = l0 * P0[2] + l1 * P1[2] + l2 * P2[2];
+            if (std::abs(z - p.z) < best) {
+                best = std::abs(z - p.z);
+                zb = z;
+                fb = f;
+                lb[0] = l0;
+                lb[1] = l1;
+                lb[2] = l2;
+                // slope of the facet plane, dz/dx and dz/dy at fixed vertices
+                const double z10 = P1[2] - P0[2], z20 = P2[2] - P0[2];
+                sxb = (z10 * y20 - z20 * y10) / d;
+                syb = (x10 * z20 - x20 * z10) / d;
+            }
+        }
+        if (fb < 0) continue;
+        if (zmodel) (*zmodel)[k] = zb;
+        const double r = (zb - p.z) / p.sigma;
+        total += 0.5 * r * r;
+        if (!vgrad) continue;
+        // z = sum_j l_j z_j; moving vertex j horizontally by delta at fixed (x, y) shifts
+        // the barycentrics like moving the query point by -l_j delta
+        const double s = r / p.sigma;
+        for (int c = 0; c < 3; ++c) {
+            const size_t v = (size_t)F[3 * fb + c];
+            (*vgrad)[3 * v] += s * (-lb[c] * sxb);
+            (*vgrad)[3 * v + 1] += s * (-lb[c] * syb);
+            (*vgrad)[3 * v + 2] += s * lb[c];
+        }
+    }
+    return total;
+}
+
+double BoreholeContactTerm::evaluate(const Vec& params)
+{
+    m_model->applyParameterVector(params);
+    return m_weight * misfit(nullptr, nullptr);
+}
+
+Vec BoreholeContactTerm::gradient(const Vec& params)
+{
+    m_model->applyParameterVector(params);
+    std::vector<double> vg;
+    misfit(&vg, nullptr);
+    Vec g = m_model->pullBackVertexGradient(vg);
+    for (auto& x : g) x *= m_weight;
+    return g;
+}
+
 // ================================================================== objectives
 
 ObjectiveFunction::ObjectiveFunction(std::shared_ptr<ForwardModel> forward, const GravityData& data)

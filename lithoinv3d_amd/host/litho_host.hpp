@@ -279,6 +279,40 @@ private:
     std::vector<std::vector<int32_t>> m_nbr;
 };
 
+// A depth-to-contact pick from a borehole: at collar (x, y) the contact between unit
+// `above` and unit `below` lies at elevation z (z up, metres) with standard deviation
+// sigma.
+struct ContactPick {
+    double x, y, z;
+    int above, below;
+    double sigma;
+};
+
+// A differentiable stand-in for the reference's borehole penalty, which classifies the
+// interval midpoint through the label grid and differentiates that step function with
+// 1 m finite differences over every parameter (constraint_handler.cpp:15-66):
+// 0.5 sum ((z_model - z)/sigma)^2, z_model the elevation at (x, y) of the model's
+// above|below contact surface (a vertical line through the contact facets, the
+// intersection nearest z), with the exact gradient through the intersected facet's
+// vertices.  Picks whose line misses the contact add nothing.
+class BoreholeContactTerm : public Regularization {
+public:
+    // searchRadius: how far (m, horizontally) the candidate facets of a pick may lie from
+    // its collar at the starting geometry
+    BoreholeContactTerm(std::shared_ptr<LithologyModel> model, std::vector<ContactPick> picks,
+                        double searchRadius = 250.0);
+    double evaluate(const Vec& params) override;
+    Vec gradient(const Vec& params) override;
+    // unweighted misfit at the current geometry; vgrad (3 per vertex) and the model
+    // elevation of each pick (NaN where the line misses) if given
+    double misfit(std::vector<double>* vgrad, std::vector<double>* zmodel) const;
+
+private:
+    std::shared_ptr<LithologyModel> m_model;
+    std::vector<ContactPick> m_picks;
+    std::vector<std::vector<int64_t>> m_cand;  // candidate contact facets per pick
+};
+
 // modules/litho-inversion/src/inversion/objective.cpp:6-103 (gravity data misfit +
 // regularization); the gradient uses computeJTr instead of forming J.
 class ObjectiveFunction {

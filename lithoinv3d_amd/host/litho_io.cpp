@@ -241,6 +241,51 @@ std::vector<StationRecord> loadStationCSV(const std::string& path, std::string* 
     return out;
 }
 
+std::vector<ConstraintRecord> loadConstraintCSV(const std::string& path, std::string* warnings)
+{
+    std::ifstream f = open_in(path, "loadConstraintCSV");
+    std::vector<ConstraintRecord> out;
+    std::string line;
+    int lineNo = 0;
+    while (std::getline(f, line)) {
+        ++lineNo;
+        if (line.empty() || line[0] == '#') continue;
+        std::vector<double> vals;
+        bool skip = false;
+        size_t pos = 0;
+        while (pos <= line.size()) {
+            size_t comma = line.find(',', pos);
+            if (comma == std::string::npos) comma = line.size();
+            std::string tok = line.substr(pos, comma - pos);
+            pos = comma + 1;
+            const size_t a = tok.find_first_not_of(" \t\r");
+            if (a != std::string::npos) {
+                const size_t b = tok.find_last_not_of(" \t\r");
+                tok = tok.substr(a, b - a + 1);
+                char* end = nullptr;
+                errno = 0;
+                const double v = std::strtod(tok.c_str(), &end);
+                if (end == tok.c_str() || errno == ERANGE) {  // a header line lands here too
+                    warn(warnings, path + ":" + std::to_string(lineNo) + ": '" + tok +
+                                       "' is not a number");
+                    skip = true;
+                    break;
+                }
+                vals.push_back(v);
+            }
+            if (comma == line.size()) break;
+        }
+        if (skip) continue;
+        if (vals.size() < 5) {
+            warn(warnings, path + ":" + std::to_string(lineNo) +
+                               ": expected 5 values (x, y, z_top, z_bottom, unit)");
+            continue;
+        }
+        out.push_back({vals[0], vals[1], vals[2], vals[3], (int)vals[4]});
+    }
+    return out;
+}
+
 // ----------------------------------------------------------------- label grid
 
 LabelGridData loadLabelGrid(const std::string& path)

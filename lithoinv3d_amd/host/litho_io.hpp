@@ -35,6 +35,17 @@ struct StationRecord {
 // loaders parse identically: x, y, z, observation[, standard deviation])
 std::vector<StationRecord> loadStationCSV(const std::string& path, std::string* warnings = nullptr);
 
+// One borehole interval: unit `unit` between depths z_top and z_bottom (positive down)
+// at collar (x, y).
+struct ConstraintRecord {
+    double x, y, z_top, z_bottom;
+    int unit;
+};
+// modules/litho-io/src/io/constraint_loader.cpp:8-66: x, y, z_top, z_bottom, unit per
+// line; '#' comments; a line with a non-numeric field or fewer than 5 values is skipped.
+std::vector<ConstraintRecord> loadConstraintCSV(const std::string& path,
+                                                std::string* warnings = nullptr);
+
 // The label grid written by python/lithoseed/src/lithoseed/_surfaces.py:659-671: an
 // ASCII header "nx ny nz x0 y0 z0 dx dy dz\n" then nx*ny*nz int32 labels with i fastest
 // (index i + nx*(j + ny*k)).

@@ -61,6 +61,10 @@ def load_host():
     L.li_model_bounds.argtypes = [_P, _P, ctypes.c_double, _P, _P]
     L.li_model_smoothness.argtypes = [_P, _P, ctypes.c_double, ctypes.POINTER(ctypes.c_double), _P]
     L.li_problem_add_smoothness.argtypes = [_P, ctypes.c_double]
+    L.li_model_boreholes.argtypes = [_P, ctypes.c_int64, _P, _P, _P, _P, ctypes.c_double, _P,
+                                     ctypes.c_double, ctypes.POINTER(ctypes.c_double), _P, _P]
+    L.li_problem_add_boreholes.argtypes = [_P, ctypes.c_int64, _P, _P, _P, _P, ctypes.c_double,
+                                           ctypes.c_double]
     L.li_model_dof_map.argtypes = [_P, _P, _P, _P, ctypes.POINTER(i64)]
     L.li_problem_create.argtypes = [ctypes.POINTER(_P), _P, i64, _P, _P, _P, _P, _P,
                                     ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -184,6 +188,22 @@ class Model:
              "smoothness")
         return v.value, g
 
+    def boreholes(self, picks, params, weight=1.0, radius=250.0):
+        """BoreholeContactTerm at params for litho_io.ContactPicks: (weight * misfit,
+        parameter gradient, model elevation of each pick (NaN where it misses))."""
+        p = _a(params)
+        xyz = _a(picks.xyz).reshape(-1, 3)
+        ab = np.ascontiguousarray(picks.above, np.int32)
+        be = np.ascontiguousarray(picks.below, np.int32)
+        sg = _a(picks.sigma)
+        v = ctypes.c_double()
+        g = np.zeros(len(p))
+        z = np.zeros(len(xyz))
+        _chk(load_host().li_model_boreholes(self._h, len(xyz), _p(xyz), _p(ab), _p(be), _p(sg),
+                                            radius, _p(p), weight, ctypes.byref(v), _p(g), _p(z)),
+             "boreholes")
+        return v.value, g, z
+
     def bounds(self, params, margin_m):
         p = _a(params)
         lo, hi = np.zeros(len(p)), np.zeros(len(p))
@@ -248,11 +268,13 @@ def minimize(fun, grad, x0, lb, ub, max_iter=100, history=10, ftol=1e-6, c1=1e-4
 class Problem:
     """Joint gravity + magnetic objective (JointObjective) over the GPU forward and
     adjoint for one station set: Phi = sum (w_g r_g)^2/2 + alpha sum (w_t r_t)^2/2
-    (+ reference-model term, + surface smoothness), w = 1/std."""
+    (+ reference-model term, + surface smoothness, + borehole depth-to-contact misfit),
+    w = 1/std."""
 
     def __init__(self, model: Model, stations, gz_obs, gz_std=None, tmi_obs=None, tmi_std=None,
                  inc=-60.0, dec=0.0, field=50000.0, alpha_mag=1.0, precision="fp64",
-                 ref_weight=0.0, device=0, smooth_weight=0.0):
+                 ref_weight=0.0, device=0, smooth_weight=0.0, boreholes=None,
+                 borehole_weight=1.0, borehole_radius=250.0):
         L = load_host()
         self.model = model
         st = _a(stations).reshape(-1, 3)
@@ -266,6 +288,13 @@ class Problem:
                                  ref_weight, device), "li_problem_create")
         if smooth_weight > 0.0:
             _chk(L.li_problem_add_smoothness(self._h, smooth_weight), "add_smoothness")
+        if boreholes is not None and len(boreholes.xyz):
+            b = boreholes
+            arrs = [_a(b.xyz).reshape(-1, 3), np.ascontiguousarray(b.above, np.int32),
+                    np.ascontiguousarray(b.below, np.int32), _a(b.sigma)]
+            self._keep.extend(arrs)
+            _chk(L.li_problem_add_boreholes(self._h, len(arrs[0]), *[_p(a) for a in arrs],
+                                            borehole_radius, borehole_weight), "add_boreholes")
 
     def close(self):
         if self._h:

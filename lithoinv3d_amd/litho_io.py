@@ -39,6 +39,14 @@ def _lib():
         L.li_stations_warnings.restype = ctypes.c_char_p
         L.li_stations_destroy.argtypes = [_P]
         L.li_stations_destroy.restype = None
+        L.li_constraints_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(_P)]
+        L.li_constraints_count.argtypes = [_P]
+        L.li_constraints_count.restype = _i64
+        L.li_constraints_get.argtypes = [_P, _P, _P, _P, _P]
+        L.li_constraints_warnings.argtypes = [_P]
+        L.li_constraints_warnings.restype = ctypes.c_char_p
+        L.li_constraints_destroy.argtypes = [_P]
+        L.li_constraints_destroy.restype = None
         L.li_label_grid_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(_P)]
         L.li_label_grid_info.argtypes = [_P, _P, _P, _P]
         L.li_label_grid_get.argtypes = [_P, _P]
@@ -121,6 +129,59 @@ def read_stations(path) -> Stations:
     finally:
         L.li_stations_destroy(h)
     return Stations(xyz, obs, sd, w)
+
+
+@dataclasses.dataclass
+class Constraints:
+    xy: np.ndarray        # (n, 2) borehole collars
+    z_top: np.ndarray     # interval top depth (m, positive down)
+    z_bottom: np.ndarray  # interval bottom depth
+    unit: np.ndarray      # unit id of the interval
+    warnings: str = ""
+
+
+def read_constraints(path) -> Constraints:
+    """Borehole intervals: x, y, z_top, z_bottom, unit per line (depths positive down)."""
+    L = _lib()
+    h = _P()
+    _chk(L.li_constraints_load(_path(path), ctypes.byref(h)), f"read_constraints({path})")
+    try:
+        n = L.li_constraints_count(h)
+        xy, zt, zb = np.zeros((n, 2)), np.zeros(n), np.zeros(n)
+        u = np.zeros(n, np.int32)
+        L.li_constraints_get(h, _p(xy), _p(zt), _p(zb), _p(u))
+        w = L.li_constraints_warnings(h).decode()
+    finally:
+        L.li_constraints_destroy(h)
+    return Constraints(xy, zt, zb, u, w)
+
+
+@dataclasses.dataclass
+class ContactPicks:
+    xyz: np.ndarray    # (n, 3): collar x, y and the contact's elevation (z up)
+    above: np.ndarray  # unit above the contact
+    below: np.ndarray  # unit below it
+    sigma: np.ndarray  # standard deviation of the elevation (m)
+
+
+def contact_picks(c: Constraints, sigma=5.0, gap=1e-6) -> ContactPicks:
+    """Depth-to-contact picks from borehole intervals: wherever two intervals of one
+    borehole (same collar) touch (bottom of one = top of the next, within gap) and hold
+    different units, a contact at elevation -depth between the upper and the lower unit."""
+    xyz, above, below = [], [], []
+    keys = {}
+    for i, (x, y) in enumerate(c.xy):
+        keys.setdefault((float(x), float(y)), []).append(i)
+    for (x, y), idx in keys.items():
+        idx = sorted(idx, key=lambda i: c.z_top[i])
+        for i, j in zip(idx[:-1], idx[1:]):
+            if abs(c.z_bottom[i] - c.z_top[j]) <= gap and c.unit[i] != c.unit[j]:
+                xyz.append((x, y, -c.z_bottom[i]))
+                above.append(c.unit[i])
+                below.append(c.unit[j])
+    n = len(xyz)
+    return ContactPicks(np.array(xyz, np.float64).reshape(n, 3), np.array(above, np.int32),
+                        np.array(below, np.int32), np.full(n, float(sigma)))
 
 
 @dataclasses.dataclass

@@ -1,0 +1,94 @@
+"""Borehole depth-to-contact term (host C++, no GPU) and the borehole interval file
+(modules/litho-io/src/io/constraint_loader.cpp): picks from touching intervals, the
+model elevation of a contact along a vertical line, misfit value and its exact gradient
+against central differences, with and without control points, Z_ONLY and XYZ_FREE."""
+import numpy as np
+import pytest
+
+from lithoinv3d_amd import inversion, synth
+from lithoinv3d_amd import litho_io as IO
+
+CELL = 25.0
+TOP = -31.0
+
+
+def layered_model(seed=0):
+    """Unit 0 on top of unit 1 on top of unit 2, the 0|1 contact at 2 cells depth with a
+    bump of one cell in the middle."""
+    nx, ny, nz = 8, 7, 6
+    lab = np.zeros((nx, ny, nz), np.int32)
+    lab[:, :, 2:] = 1
+    lab[3:5, 3:5, 2] = 0          # the contact dips one cell over a 2x2 patch
+    lab[:, :, 4:] = 2
+    return synth.label_grid_model(lab, (-100.0, -87.5, TOP), (CELL, CELL, CELL),
+                                  [0.1, 0.2, 0.3], [0.01, 0.02, 0.03])
+
+
+def test_constraint_file_and_picks(tmp_path):
+    p = tmp_path / "bh.csv"
+    p.write_text("""x,y,z_top,z_bottom,unit
+# borehole A
+10.0, 20.0, 0.0, 81.0, 0
+10.0, 20.0, 81.0, 131.0, 1
+10.0, 20.0, 131.0, 200.0, 2
+# borehole B: a gap between intervals gives no pick
+50,60,0,70,0
+50,60,75,150,1
+50,60,150
+""")
+    c = IO.read_constraints(p)
+    assert len(c.unit) == 5 and "not a number" in c.warnings and "expected 5 values" in c.warnings
+    np.testing.assert_array_equal(c.unit, [0, 1, 2, 0, 1])
+    k = IO.contact_picks(c, sigma=5.0)
+    np.testing.assert_array_equal(k.xyz, [[10.0, 20.0, -81.0], [10.0, 20.0, -131.0]])
+    np.testing.assert_array_equal(k.above, [0, 1])
+    np.testing.assert_array_equal(k.below, [1, 2])
+    np.testing.assert_array_equal(k.sigma, [5.0, 5.0])
+
+
+def picks_for(z_obs, xy, above=0, below=1, sigma=5.0):
+    n = len(xy)
+    return IO.ContactPicks(np.column_stack([xy, np.full(n, z_obs)]), np.full(n, above, np.int32),
+                           np.full(n, below, np.int32), np.full(n, sigma))
+
+
+def test_contact_elevation_and_value():
+    m = layered_model()
+    M = inversion.Model(m)
+    M.fix_exterior(inversion.Z_ONLY)
+    p0 = M.params()
+    z01 = TOP - 2 * CELL          # flat part of the 0|1 contact
+    zbump = TOP - 3 * CELL        # over the dipping patch x in [-25, 25), y in [-12.5, 37.5)
+    xy = np.array([[-80.0, -60.0], [60.0, 70.0], [0.0, 10.0], [500.0, 500.0]])
+    k = picks_for(-90.0, xy)
+    val, g, z = M.boreholes(k, p0, weight=2.0)
+    np.testing.assert_allclose(z[:3], [z01, z01, zbump], atol=1e-9)
+    assert np.isnan(z[3])         # outside the model: the line misses the contact
+    r = (z[:3] + 90.0) / 5.0
+    assert val == pytest.approx(2.0 * 0.5 * np.sum(r ** 2), rel=1e-12)
+    # the 1|2 contact
+    _, _, z12 = M.boreholes(picks_for(-140.0, xy[:2], 1, 2), p0)
+    np.testing.assert_allclose(z12, TOP - 4 * CELL, atol=1e-9)
+
+
+@pytest.mark.parametrize("freedom,stride", [(inversion.Z_ONLY, 0), (inversion.XYZ_FREE, 0),
+                                            (inversion.Z_ONLY, 2)])
+def test_gradient_vs_central_differences(freedom, stride):
+    m = layered_model()
+    M = inversion.Model(m)
+    M.fix_exterior(freedom)
+    M.set_axis_scale(10.0)
+    if stride:
+        M.set_cp_stride(stride, CELL)
+    rng = np.random.default_rng(3)
+    xy = np.column_stack([rng.uniform(-70, 70, 12), rng.uniform(-60, 60, 12)])
+    k = picks_for(-85.0, xy, sigma=3.0)
+    p = M.params() + rng.normal(0, 0.3, M.n_dof)  # tilt the facets a little
+    val, g, z = M.boreholes(k, p, weight=0.7)
+    assert np.isfinite(z).all() and val > 0 and np.abs(g).max() > 0
+    h = 1e-6
+    for j in np.argsort(-np.abs(g))[:15]:
+        e = np.zeros(len(p))
+        e[j] = h
+        fd = (M.boreholes(k, p + e, 0.7)[0] - M.boreholes(k, p - e, 0.7)[0]) / (2 * h)
+        assert fd == pytest.approx(g[j], rel=1e-5, abs=1e-9 * np.abs(g).max())
