@@ -297,6 +297,36 @@ int li_problem_add_boreholes(void* h, int64_t n, const double* xyz, const int32_
     });
 }
 
+// The property phase (runner.cpp:708-935): which = 0 densities from gravity, 1
+// susceptibilities from TMI; the new unit properties go to out (n_units) and into the
+// model, and every forward of the problem takes them.
+int li_problem_invert_properties(void* h, int which, double damping, double lower, double upper,
+                                 int max_iter, double tol, double* out)
+{
+    return guard([&] {
+        auto* P = (Problem*)h;
+        PropertyInversionOptions o;
+        o.damping = damping;
+        o.lower = lower;
+        o.upper = upper;
+        o.maxIter = max_iter;
+        o.tolerance = tol;
+        Vec p;
+        if (which == 0) {
+            p = invertUnitDensities(*P->grav, P->gdata, *P->model, o);
+        } else if (which == 1) {
+            if (!P->mag) throw std::runtime_error("problem has no magnetic data");
+            p = invertUnitSusceptibilities(*P->mag, P->mdata, *P->model, o);
+        } else {
+            throw std::invalid_argument("li_problem_invert_properties: which must be 0 or 1");
+        }
+        if (P->mag) P->mag->refreshProperties();
+        P->grav->refreshProperties();
+        if (P->fused) P->fused->refreshProperties();
+        std::memcpy(out, p.data(), sizeof(double) * p.size());
+    });
+}
+
 // Adds SurfaceSmoothness with this weight to the problem's objectives.
 int li_problem_add_smoothness(void* h, double weight)
 {

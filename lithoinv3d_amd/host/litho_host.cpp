@@ -427,6 +427,12 @@ void GpuForwardBase::upload(const Vec& params)
     check(lf_set_vertices(c, m_model->vertices().data()), c, "lf_set_vertices");
 }
 
+void GpuForwardBase::syncGeometry()
+{
+    lf_ctx* c = ctx();
+    check(lf_set_vertices(c, m_model->vertices().data()), c, "lf_set_vertices");
+}
+
 Vec GpuForwardBase::forwardRaw(int which)
 {
     Vec out(m_nData);
@@ -912,6 +918,8 @@ FusedJointObjective::FusedJointObjective(std::shared_ptr<LithologyModel> model,
                 throw std::invalid_argument("FusedJointObjective: gravity and magnetic stations differ");
     double fh[3];
     fieldUnitVector(inc_deg, dec_deg, fh);
+    m_field = field_nT;
+    for (int k = 0; k < 3; ++k) m_fhat[k] = fh[k];
     std::vector<double> st;
     for (auto& p : gdata) st.insert(st.end(), p.position, p.position + 3);
     const int ng = m_model->groupCount();
@@ -945,6 +953,116 @@ FusedJointObjective::FusedJointObjective(std::shared_ptr<LithologyModel> model,
 FusedJointObjective::~FusedJointObjective()
 {
     if (m_ctx) lf_destroy(m_ctx);
+}
+
+void FusedJointObjective::refreshProperties()
+{
+    const int ng = m_model->groupCount();
+    std::vector<double> dens(ng), mag(3 * ng);
+    for (int g = 0; g < ng; ++g) {
+        dens[g] = m_model->group(g).density;
+        const double amp = m_model->group(g).susceptibility * m_field / 100.0;  // induced
+        for (int k = 0; k < 3; ++k) mag[3 * g + k] = amp * m_fhat[k];
+    }
+    check(lf_set_properties(m_ctx, dens.data(), mag.data()), m_ctx, "lf_set_properties");
+    m_cacheValid = false;
+}
+
+namespace {
+// min over p in [lo, hi] of 0.5 |w (obs - U p)|^2 / s^2 + 0.5 lambda |p - p0|^2 with the
+// reference's optimizer settings; U column-major (n_obs x n_p)
+Vec bounded_unit_fit(const std::vector<double>& U, size_t nObs, const Vec& obs, const Vec& sd,
+                     const Vec& p0, const PropertyInversionOptions& o)
+{
+    const size_t np = p0.size();
+    double var = 0.0;
+    for (double v : obs) var += v * v;
+    var /= std::max<size_t>(nObs, 1);
+    const double s2 = std::max(var, 1.0);  // rms floored at 1
+    auto pred = [&](const Vec& p) {
+        Vec r(nObs, 0.0);
+        for (size_t j = 0; j < np; ++j)
+            for (size_t i = 0; i < nObs; ++i) r[i] += U[j * nObs + i] * p[j];
+        return r;
+    };
+    auto wt = [&](size_t i) { return sd[i] > 0.0 ? 1.0 / sd[i] : 1.0; };
+    auto f = [&](const Vec& p) {
+        const Vec q = pred(p);
+        double m = 0.0;
+        for (size_t i = 0; i < nObs; ++i) {
+            const double r = wt(i) * (obs[i] - q[i]);
+            m += 0.5 * r * r;
+        }
+        m /= s2;
+        if (o.damping > 0.0)
+            for (size_t j = 0; j < np; ++j) m += 0.5 * o.damping * (p[j] - p0[j]) * (p[j] - p0[j]);
+        return m;
+    };
+    auto g = [&](const Vec& p) {
+        const Vec q = pred(p);
+        Vec gr(np, 0.0);
+        for (size_t i = 0; i < nObs; ++i) {
+            const double w = wt(i);
+            const double wr = w * w * (obs[i] - q[i]) / s2;
+            for (size_t j = 0; j < np; ++j) gr[j] -= U[j * nObs + i] * wr;
+        }
+        if (o.damping > 0.0)
+            for (size_t j = 0; j < np; ++j) gr[j] += o.damping * (p[j] - p0[j]);
+        return gr;
+    };
+    LBFGSBOptimizer opt;
+    opt.setMaxIterations(o.maxIter);
+    opt.setHistorySize(std::min<int>(5, (int)np));
+    opt.setTolerance(o.tolerance);
+    const Vec lo(np, o.lower), hi(np, o.upper);
+    return opt.minimize(f, g, p0, lo, hi).params;
+}
+}  // namespace
+
+Vec invertUnitDensities(GravityForward& fwd, const GravityData& data, LithologyModel& model,
+                        const PropertyInversionOptions& o)
+{
+    const int ng = model.groupCount();
+    const size_t n = data.size();
+    std::vector<double> U((size_t)ng * n);
+    fwd.syncGeometry();  // the responses of the current geometry
+    for (int g = 0; g < ng; ++g) {
+        const Vec col = fwd.computeGroupUnitResponse(g);
+        std::copy(col.begin(), col.end(), U.begin() + (size_t)g * n);
+    }
+    Vec obs(n), sd(n), p0(ng);
+    for (size_t i = 0; i < n; ++i) {
+        obs[i] = data[i].g_obs;
+        sd[i] = data[i].g_std;
+    }
+    for (int g = 0; g < ng; ++g) p0[g] = model.group(g).density;
+    const Vec p = bounded_unit_fit(U, n, obs, sd, p0, o);
+    for (int g = 0; g < ng; ++g) model.group(g).density = p[g];
+    fwd.refreshProperties();
+    return p;
+}
+
+Vec invertUnitSusceptibilities(MagneticForward& fwd, const MagneticData& data,
+                               LithologyModel& model, const PropertyInversionOptions& o)
+{
+    const int ng = model.groupCount();
+    const size_t n = data.size();
+    std::vector<double> U((size_t)ng * n);
+    fwd.syncGeometry();  // the responses of the current geometry
+    for (int g = 0; g < ng; ++g) {
+        const Vec col = fwd.computeGroupUnitResponse(g);
+        std::copy(col.begin(), col.end(), U.begin() + (size_t)g * n);
+    }
+    Vec obs(n), sd(n), p0(ng);
+    for (size_t i = 0; i < n; ++i) {
+        obs[i] = data[i].t_obs;
+        sd[i] = data[i].t_std;
+    }
+    for (int g = 0; g < ng; ++g) p0[g] = model.group(g).susceptibility;
+    const Vec p = bounded_unit_fit(U, n, obs, sd, p0, o);
+    for (int g = 0; g < ng; ++g) model.group(g).susceptibility = p[g];
+    fwd.refreshProperties();
+    return p;
 }
 
 void FusedJointObjective::ensureForward(const Vec& params)

@@ -173,6 +173,10 @@ public:
     Mat computeJacobian(const Vec& params) override;
     Vec computeJTr(const Vec& params, const Vec& w) override;
     const std::string& lastError() const { return m_err; }
+    // re-read the unit properties from the model (after they were changed)
+    void refreshProperties() { pushProperties(); }
+    // upload the model's current vertex positions
+    void syncGeometry();
 
 protected:
     GpuForwardBase(std::shared_ptr<LithologyModel> model, std::vector<double> stations,
@@ -375,9 +379,13 @@ public:
     void forward(const Vec& params, Vec& gz, Vec& tmi);
     int forwardPasses() const { return m_nForward; }
     int adjointPasses() const { return m_nAdjoint; }
+    // re-read the unit properties from the model (after they were changed)
+    void refreshProperties();
 
 private:
     void ensureForward(const Vec& params);
+    double m_field = 50000.0;
+    double m_fhat[3] = {0.0, 0.0, 1.0};
     std::shared_ptr<LithologyModel> m_model;
     const GravityData& m_g;
     const MagneticData& m_m;
@@ -388,6 +396,24 @@ private:
     bool m_cacheValid = false;
     int m_nForward = 0, m_nAdjoint = 0;
 };
+
+// The property phase of the reference's inversion loop, modules/litho-inversion/src/
+// inversion/runner.cpp:708-800 (densities from gravity) and :801-935 (susceptibilities
+// from TMI, induced magnetization): unit responses U (one forward per unit with a unit
+// property), then a bounded L-BFGS on
+//   0.5 |W (d_obs - U p)|^2 / rms(d_obs)^2 + 0.5 damping |p - p_start|^2
+// (rms floored at 1, W = 1/std or 1), history min(5, n_units).  The result is written into
+// the model's units.
+struct PropertyInversionOptions {
+    double damping = 0.0;
+    double lower = -1e300, upper = 1e300;
+    int maxIter = 50;
+    double tolerance = 1e-6;
+};
+Vec invertUnitDensities(GravityForward& fwd, const GravityData& data, LithologyModel& model,
+                        const PropertyInversionOptions& opt);
+Vec invertUnitSusceptibilities(MagneticForward& fwd, const MagneticData& data,
+                               LithologyModel& model, const PropertyInversionOptions& opt);
 
 // modules/litho-inversion/include/litho_invert/inversion/optimizer.h:7-36
 struct OptimizerResult {

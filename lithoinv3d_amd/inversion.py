@@ -61,6 +61,8 @@ def load_host():
     L.li_model_bounds.argtypes = [_P, _P, ctypes.c_double, _P, _P]
     L.li_model_smoothness.argtypes = [_P, _P, ctypes.c_double, ctypes.POINTER(ctypes.c_double), _P]
     L.li_problem_add_smoothness.argtypes = [_P, ctypes.c_double]
+    L.li_problem_invert_properties.argtypes = [_P, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                               ctypes.c_double, ctypes.c_int, ctypes.c_double, _P]
     L.li_model_boreholes.argtypes = [_P, ctypes.c_int64, _P, _P, _P, _P, ctypes.c_double, _P,
                                      ctypes.c_double, ctypes.POINTER(ctypes.c_double), _P, _P]
     L.li_problem_add_boreholes.argtypes = [_P, ctypes.c_int64, _P, _P, _P, _P, ctypes.c_double,
@@ -306,6 +308,19 @@ class Problem:
             self.close()
         except Exception:
             pass
+
+    def invert_properties(self, kind="density", damping=0.0, lower=-1e300, upper=1e300,
+                          max_iter=50, tol=1e-6):
+        """The property phase of the inversion loop at the model's current geometry:
+        per-unit densities from gravity ("density") or susceptibilities from TMI
+        ("susceptibility"), bounded, with optional damping toward the current values.
+        Returns the new values; the model and every forward of the problem take them."""
+        which = {"density": 0, "susceptibility": 1}[kind]
+        out = np.zeros(self.model.src.n_units)
+        _chk(load_host().li_problem_invert_properties(self._h, which, damping, lower, upper,
+                                                      max_iter, tol, _p(out)),
+             "invert_properties")
+        return out
 
     def set_fused(self, on: bool):
         """True (default with magnetic data): one GPU context for gz and TMI, one adjoint

@@ -151,3 +151,32 @@ def test_fused_objective_matches_reference_shaped():
     f, a = P.pass_counts()
     assert (f, a) == (1, 1)  # the gradient reused the forward of the evaluation
     P.close()
+
+
+def test_property_inversion_recovers_units():
+    """The property phase (runner.cpp:708-935): unit densities from gravity and unit
+    susceptibilities from TMI, started from wrong values, recover the values that made the
+    noise-free data (relative 1e-5), and the problem's forward takes them."""
+    m, st = model_and_stations(11)
+    M0 = inversion.Model(m)
+    P = inversion.Problem(M0, st, np.zeros(len(st)), tmi_obs=np.zeros(len(st)))
+    gz, tm = P.forward(M0.params())
+    P.close()
+    wrong = synth.LithoModel(m.vertices, m.faces, m.front, m.back, 0.5 * m.density + 0.05,
+                             2.0 * m.susceptibility)
+    M = inversion.Model(wrong)
+    P = inversion.Problem(M, st, gz, np.full(len(st), 0.05), tm, np.full(len(st), 1.0))
+    say("property inversion: densities")
+    d = P.invert_properties("density", lower=-1.0, upper=1.0, max_iter=200, tol=1e-14)
+    print("densities", d, "true", m.density)
+    np.testing.assert_allclose(d, m.density, rtol=1e-5)
+    say("property inversion: susceptibilities")
+    s = P.invert_properties("susceptibility", lower=0.0, upper=1.0, max_iter=200, tol=1e-14)
+    print("susceptibilities", s, "true", m.susceptibility)
+    np.testing.assert_allclose(s, m.susceptibility, rtol=1e-5)
+    g2, t2 = P.forward(M.params())
+    np.testing.assert_allclose(g2, gz, rtol=0, atol=1e-6 * np.abs(gz).max())
+    np.testing.assert_allclose(t2, tm, rtol=0, atol=1e-5 * np.abs(tm).max())
+    comp = P.evaluate(M.params())
+    print("objective components after the property phase", comp)
+    P.close()
