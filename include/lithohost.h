@@ -1,0 +1,181 @@
+/*
+ * lithohost.h -- C ABI of the host layer (liblithohost.so) that sits above lithofwd.h: the
+ * lithology model with its parameter vector, the reference's ForwardModel /
+ * ObjectiveFunction / JointObjective / L-BFGS-B surface over the GPU forward and adjoint,
+ * the regularizers, the property phase, and the file formats around the forward path.
+ *
+ * Every handle is an opaque pointer made by a *_create / *_load / li_assemble call and
+ * released by the matching *_destroy.  Entry points returning int give 0 on success and
+ * -1 on failure, li_last_error() then describes it (the message of the C++ exception,
+ * thread-local).  Arrays are host pointers, row-major, 0-based indices; xyz triples are
+ * metres with z positive up.  A handle is not thread-safe.
+ */
+#ifndef LITHOHOST_H
+#define LITHOHOST_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char* li_last_error(void);
+
+/* ---- lithology model (modules/litho-model/src/litho/lithology_model.cpp) ---- */
+
+/* LithologyModel from a shared-vertex facet set: faces oriented outward for `front`, `back`
+ * the unit on the other side (-1 outside), per-unit density contrast (g/cc) and
+ * susceptibility (SI); litho_group.h:7-34 holds the same unit properties. */
+int li_model_create(void** out, int64_t n_v, const double* verts, int64_t n_f,
+                    const uint32_t* faces, const int32_t* front, const int32_t* back,
+                    int32_t n_u, const double* density, const double* susceptibility);
+void li_model_destroy(void* h);
+/* fixExteriorFaces (lithology_model.cpp:369-579): vertices of model-bounding facets FIXED,
+ * the rest `interior_freedom` (0 FIXED, 1 Z_ONLY, 2 XY_ONLY, 3 FREE; common.h:16-38). */
+int li_model_fix_exterior(void* h, int interior_freedom);
+/* setVertexFreedom per vertex (n_v values, same codes) */
+int li_model_set_freedom(void* h, const uint8_t* freedom);
+/* axis scale of the parameter vector (runner.cpp:241-248: params in units of s metres) */
+int li_model_set_axis_scale(void* h, int axis, double s);
+/* control points on a stride*cell grid (lithology_model.cpp:864-1040, IDW K = 4 of
+ * surface_mesh.cpp:544-601); stride <= 0 turns them off; *n_dof = new parameter count */
+int li_model_set_cp_stride(void* h, int stride, double cell, int64_t* n_dof);
+int64_t li_model_dof_count(void* h);
+/* per DOF: its vertex and axis; per vertex: is it a control point */
+int li_model_dofs(void* h, int32_t* vertex, int32_t* axis, uint8_t* is_cp);
+/* assembleParameterVector (lithology_model.cpp:194-209) */
+int li_model_params(void* h, double* out);
+/* applyParameterVector (lithology_model.cpp:210-248); vertices_out (3*n_v) may be NULL */
+int li_model_apply(void* h, const double* params, double* vertices_out);
+/* vertex gradient (3*n_v) -> parameter gradient, the transpose of li_model_apply's map */
+int li_model_pullback(void* h, const double* vgrad, double* out);
+/* that map as CSR per parameter; call with vert and w NULL for the entry count first */
+int li_model_dof_map(void* h, int64_t* off, int32_t* vert, double* w, int64_t* n_entries);
+/* geometry bounds params -/+ margin/axisScale (runner.cpp:241-248) */
+int li_model_bounds(void* h, const double* params, double margin_m, double* lo, double* hi);
+/* SmoothnessRegularization (smoothness.cpp:8-388): weight * penalty at params and its
+ * parameter gradient (grad may be NULL) */
+int li_model_smoothness(void* h, const double* params, double weight, double* value,
+                        double* grad);
+/* Depth-to-contact picks (constraint_handler.cpp:15-66, made differentiable): weight *
+ * 0.5 sum ((z_model - z)/sigma)^2 at params, its gradient and each pick's model
+ * elevation (NaN where the vertical line misses the contact); grad, zmodel may be NULL */
+int li_model_boreholes(void* h, int64_t n, const double* xyz, const int32_t* above,
+                       const int32_t* below, const double* sigma, double radius,
+                       const double* params, double weight, double* value, double* grad,
+                       double* zmodel);
+
+/* ---- joint gravity + magnetic problem (modules/litho-inversion) ---- */
+
+/* GravityForward + ObjectiveFunction (objective.cpp:6-103), MagneticForward and
+ * JointObjective (joint_objective.cpp:42-173) when tmi_obs is given and alpha_mag > 0, on
+ * one station set (n_s xyz triples); *_std <= 0 means unweighted; inc/dec in degrees,
+ * field in nT; precision 0 FP64, 1 FP32C; ref_weight > 0 adds ReferenceModelRegularization
+ * (reference_model.cpp:16-25) at the model's current parameters. */
+int li_problem_create(void** out, void* model, int64_t n_s, const double* stations,
+                      const double* gz_obs, const double* gz_std, const double* tmi_obs,
+                      const double* tmi_std, double inc, double dec, double field,
+                      double alpha_mag, int precision, double ref_weight, int device);
+void li_problem_destroy(void* h);
+/* adds SmoothnessRegularization / the depth-to-contact term to the objective */
+int li_problem_add_smoothness(void* h, double weight);
+int li_problem_add_boreholes(void* h, int64_t n, const double* xyz, const int32_t* above,
+                             const int32_t* below, const double* sigma, double radius,
+                             double weight);
+/* the property phase (runner.cpp:708-800 densities, :801-935 susceptibilities): which 0
+ * densities from gravity, 1 susceptibilities from TMI; out = n_units new values, which the
+ * model and every forward of the problem take */
+int li_problem_invert_properties(void* h, int which, double damping, double lower, double upper,
+                                 int max_iter, double tol, double* out);
+/* JointObjective::calibrateDataWeights (joint_objective.cpp:281-393), matrix-free: sets
+ * alpha_mag from the ratio of the largest eigenvalues of J^T W^2 J of gravity and of
+ * magnetics at params.  out[7]: both eigenvalues, their ratio, the new alpha_mag, the two
+ * power-iteration counts, 1 if the gradient-norm fallback ran.  eps: central-difference
+ * step in parameter units; seed: the power iterations' start vectors. */
+int li_problem_calibrate(void* h, const double* params, uint64_t seed, double eps, double* out);
+/* 1 (default with magnetic data): one GPU context for gz and TMI, one adjoint pass and a
+ * cached forward; 0: the reference-shaped JointObjective over separate forwards */
+int li_problem_set_fused(void* h, int on);
+/* forward and adjoint passes run so far */
+int li_problem_pass_counts(void* h, int64_t* fwd, int64_t* adj);
+/* JointObjective::evaluateComponents: comp[4] = gravity misfit, magnetic misfit (before
+ * alpha_mag), regularization, total */
+int li_problem_evaluate(void* h, const double* params, double* comp);
+/* JointObjective::gradient (joint_objective.cpp:113-173) */
+int li_problem_gradient(void* h, const double* params, double* out);
+/* ForwardModel::compute (forward_model.h:6-33): gz (mGal) and TMI (nT), tmi may be NULL */
+int li_problem_forward(void* h, const double* params, double* gz, double* tmi);
+/* ForwardModel::computeJacobian (gravity_forward.cpp:269-288): which 0 gz, 1 TMI;
+ * n_s x n_dof, column-major */
+int li_problem_jacobian(void* h, const double* params, int which, double* J_colmajor);
+/* unit-property response of one group at the current geometry (gravity_forward.cpp:132-145) */
+int li_problem_unit_response(void* h, int which, int group, double* out);
+
+/* ---- L-BFGS-B (lbfgsb_optimizer.cpp:13-263, optimizer.h:7-36) ---- */
+
+typedef double (*li_obj_cb)(const double* x, int64_t n, void* user);
+typedef void (*li_grad_cb)(const double* x, int64_t n, double* g, void* user);
+typedef struct {
+    int32_t max_iter;
+    int32_t history;
+    double ftol;
+    double armijo_c1;
+    double target;  /* stop when f <= target; NaN: none */
+    int32_t has_h0; /* h0 holds a diagonal initial inverse Hessian */
+    int32_t active_set;
+} LiOptions;
+/* hist rows of 7: iteration, f, |projected gradient|, step, seconds, objective calls,
+ * gradient calls; msg: the stop reason */
+int li_minimize_callbacks(int64_t n, const double* x0, const double* lb, const double* ub,
+                          li_obj_cb f, li_grad_cb g, void* user, const LiOptions* o,
+                          const double* h0, double* x_out, double* hist, int64_t hist_cap,
+                          int64_t* n_hist, int32_t* iterations, char* msg, int64_t msg_cap);
+int li_problem_minimize(void* h, const double* x0, const double* lb, const double* ub,
+                        const LiOptions* o, const double* h0, double* x_out, double* hist,
+                        int64_t hist_cap, int64_t* n_hist, int32_t* iterations, char* msg,
+                        int64_t msg_cap);
+
+/* ---- file formats (modules/litho-io) ---- */
+
+/* .ts (surface_loader.cpp:93-148) or .obj (:9-91) by extension (:150-168) */
+int li_mesh_load(const char* path, void** out);
+int li_mesh_counts(void* h, int64_t* n_vertices, int64_t* n_triangles);
+int li_mesh_get(void* h, double* vertices, uint32_t* triangles);
+const char* li_mesh_warnings(void* h);
+void li_mesh_destroy(void* h);
+/* InversionExporter::exportTS (exporters.cpp:116-150) */
+int li_mesh_write_ts(const char* path, const char* name, int64_t n_vertices,
+                     const double* vertices, int64_t n_triangles, const uint32_t* triangles);
+/* gravity_loader.cpp:8-69 / magnetic_loader.cpp:8-69: x, y, z, observation[, std] */
+int li_stations_load(const char* path, void** out);
+int64_t li_stations_count(void* h);
+int li_stations_get(void* h, double* xyz, double* obs, double* stdev);
+const char* li_stations_warnings(void* h);
+void li_stations_destroy(void* h);
+/* constraint_loader.cpp:8-66: x, y, z_top, z_bottom, unit */
+int li_constraints_load(const char* path, void** out);
+int64_t li_constraints_count(void* h);
+int li_constraints_get(void* h, double* xy, double* z_top, double* z_bottom, int32_t* unit);
+const char* li_constraints_warnings(void* h);
+void li_constraints_destroy(void* h);
+/* label grid of python/lithoseed/src/lithoseed/_surfaces.py:659-671 */
+int li_label_grid_load(const char* path, void** out);
+int li_label_grid_info(void* h, int32_t* dims, double* origin, double* cell);
+int li_label_grid_get(void* h, int32_t* labels);
+void li_label_grid_destroy(void* h);
+int li_label_grid_write(const char* path, const int32_t* dims, const double* origin,
+                        const double* cell, const int32_t* labels);
+/* per-unit closed meshes (unit u: vertices v_off[u]..v_off[u+1], triangles t_off[u]..) ->
+ * one shared-vertex facet set, vertices merged on the 1 um keys of rebuildGlobalDofs
+ * (lithology_model.cpp:113-168), a triangle whose reversed copy is in another unit one
+ * facet with (front, back) */
+int li_assemble(int32_t n_units, const int64_t* v_off, const double* verts, const int64_t* t_off,
+                const uint32_t* tris, void** out);
+int li_assembled_counts(void* h, int64_t* n_vertices, int64_t* n_faces, int64_t* n_contacts,
+                        int64_t* n_merged);
+int li_assembled_get(void* h, double* vertices, uint32_t* faces, int32_t* front, int32_t* back);
+void li_assembled_destroy(void* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LITHOHOST_H */

@@ -1,12 +1,13 @@
-// litho_capi.cpp -- C entry points over the C++ host layer, for the Python drivers
-// and tests (ctypes).  Handles are opaque; every call returns 0 on success and a
-// negative code on failure with li_last_error() describing it.
+// litho_capi.cpp -- the C entry points of include/lithohost.h over the C++ host layer,
+// for the Python drivers and tests (ctypes).  Handles are opaque; every call returns 0
+// on success and a negative code on failure with li_last_error() describing it.
 #include <cmath>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
 #include <string>
 
+#include "../../include/lithohost.h"
 #include "litho_host.hpp"
 #include "litho_io.hpp"
 
@@ -38,6 +39,8 @@ struct Problem {
     std::unique_ptr<JointObjective> joint;
     std::unique_ptr<FusedJointObjective> fused;  // used when present and enabled
     bool useFused = true;
+    double inc = 0.0, dec = 0.0, field = 0.0;
+    int device = 0;
 
     JointObjective::JointComponents components(const Vec& p)
     {
@@ -223,6 +226,10 @@ int li_problem_create(void** out, void* model, int64_t n_s, const double* statio
             P->mdata[i].t_std = tmi_std ? tmi_std[i] : 0.0;
         }
         const Precision prec = precision == 1 ? Precision::FP32C : Precision::FP64;
+        P->inc = inc;
+        P->dec = dec;
+        P->field = field;
+        P->device = device;
         P->grav = std::make_shared<GravityForward>(P->model, P->gdata, prec, device);
         P->gobj = std::make_shared<ObjectiveFunction>(P->grav, P->gdata);
         if (ref_weight > 0.0) {
@@ -327,6 +334,29 @@ int li_problem_invert_properties(void* h, int which, double damping, double lowe
     });
 }
 
+// JointObjective::calibrateDataWeights (joint_objective.cpp:281-393): sets alpha_mag of the
+// problem from the ratio of the largest eigenvalues of J^T W^2 J of gravity and magnetics
+// at params.  out[0..6] = gravity eigenvalue, magnetic eigenvalue, their ratio, the new
+// alpha_mag, gravity and magnetic power iterations, 1 if the gradient-norm fallback was used.
+int li_problem_calibrate(void* h, const double* params, uint64_t seed, double eps, double* out)
+{
+    return guard([&] {
+        auto* P = (Problem*)h;
+        if (!P->mag) throw std::runtime_error("problem has no magnetic data");
+        const Vec p(params, params + P->model->totalDofCount());
+        const double a0 = P->joint->alphaMag();
+        const CalibrationResult r = calibrateMagneticWeight(P->model, P->gdata, P->mdata, P->inc,
+                                                            P->dec, P->field, p, a0, seed, eps,
+                                                            P->device);
+        P->joint->setAlphaMag(r.alpha);
+        if (P->fused) P->fused->setAlphaMag(r.alpha);
+        const double o[7] = {r.eigGravity, r.eigMagnetic, r.ratio, r.alpha,
+                             (double)r.iterationsGravity, (double)r.iterationsMagnetic,
+                             r.fallback ? 1.0 : 0.0};
+        std::memcpy(out, o, sizeof(o));
+    });
+}
+
 // Adds SurfaceSmoothness with this weight to the problem's objectives.
 int li_problem_add_smoothness(void* h, double weight)
 {
@@ -415,19 +445,6 @@ int li_problem_unit_response(void* h, int which, int group, double* out)
         std::memcpy(out, r.data(), sizeof(double) * r.size());
     });
 }
-
-typedef double (*li_obj_cb)(const double* x, int64_t n, void* user);
-typedef void (*li_grad_cb)(const double* x, int64_t n, double* g, void* user);
-
-struct LiOptions {
-    int32_t max_iter;
-    int32_t history;
-    double ftol;
-    double armijo_c1;
-    double target;  // NaN: none
-    int32_t has_h0;
-    int32_t active_set;
-};
 
 // hist rows: iteration, f, |proj grad|, alpha, seconds, objective calls, gradient calls
 static int run_minimize(LBFGSBOptimizer::Objective obj, LBFGSBOptimizer::Gradient grad,

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <limits>
 #include <map>
+#include <random>
 #include <stdexcept>
 #include <unordered_map>
 
@@ -966,6 +967,97 @@ void FusedJointObjective::refreshProperties()
     }
     check(lf_set_properties(m_ctx, dens.data(), mag.data()), m_ctx, "lf_set_properties");
     m_cacheValid = false;
+}
+
+Vec FusedJointObjective::jtw(const Vec& params, const Vec& wg, const Vec& wt)
+{
+    m_model->applyParameterVector(params);
+    check(lf_set_vertices(m_ctx, m_model->vertices().data()), m_ctx, "lf_set_vertices");
+    m_cacheValid = false;
+    std::vector<double> vg(3 * m_model->vertexCount());
+    check(lf_adjoint(m_ctx, wg.empty() ? nullptr : wg.data(), wt.empty() ? nullptr : wt.data(),
+                     vg.data()),
+          m_ctx, "lf_adjoint");
+    ++m_nAdjoint;
+    return m_model->pullBackVertexGradient(vg);
+}
+
+CalibrationResult calibrateMagneticWeight(std::shared_ptr<LithologyModel> model,
+                                          const GravityData& gdata, const MagneticData& mdata,
+                                          double inc_deg, double dec_deg, double field_nT,
+                                          const Vec& params, double alpha_mag, uint64_t seed,
+                                          double eps, int device)
+{
+    CalibrationResult out;
+    out.alpha = alpha_mag;
+    if (mdata.empty() || alpha_mag <= 0.0) return out;
+    FusedJointObjective f(model, gdata, mdata, inc_deg, dec_deg, field_nT, alpha_mag,
+                          Precision::FP64, device);
+    const size_t np = params.size(), ns = gdata.size();
+    Vec wgt_g(ns), wgt_m(ns);
+    for (size_t i = 0; i < ns; ++i) {
+        wgt_g[i] = gdata[i].g_std > 0 ? 1.0 / (gdata[i].g_std * gdata[i].g_std) : 1.0;
+        wgt_m[i] = mdata[i].t_std > 0 ? 1.0 / (mdata[i].t_std * mdata[i].t_std) : 1.0;
+    }
+    // W^2 (J v) for one data type, J v by central differences along v
+    auto wjv = [&](const Vec& v, bool mag) {
+        Vec pp(np), pm(np), gp, tp, gm, tm;
+        for (size_t j = 0; j < np; ++j) {
+            pp[j] = params[j] + eps * v[j];
+            pm[j] = params[j] - eps * v[j];
+        }
+        f.forward(pp, gp, tp);
+        f.forward(pm, gm, tm);
+        Vec w(ns);
+        for (size_t i = 0; i < ns; ++i)
+            w[i] = (mag ? (tp[i] - tm[i]) * wgt_m[i] : (gp[i] - gm[i]) * wgt_g[i]) / (2.0 * eps);
+        return w;
+    };
+    auto jt = [&](const Vec& w, bool mag) {
+        return mag ? f.jtw(params, Vec(), w) : f.jtw(params, w, Vec());
+    };
+    std::mt19937_64 rng(seed);
+    std::uniform_real_distribution<double> uni(-1.0, 1.0);
+    auto power = [&](bool mag, int& iters) {
+        Vec v(np);
+        for (auto& x : v) x = uni(rng);
+        double nv = std::sqrt(dot(v, v));
+        for (auto& x : v) x /= nv;
+        double lamOld = 0.0;
+        iters = 0;
+        for (int it = 0; it < 50; ++it) {
+            iters = it + 1;
+            v = jt(wjv(v, mag), mag);
+            const double lam = std::sqrt(dot(v, v));
+            if (lam < 1e-60) return 0.0;
+            for (auto& x : v) x /= lam;
+            if (std::abs(lam - lamOld) / lam < 1e-8) break;
+            lamOld = lam;
+        }
+        const Vec hv = jt(wjv(v, mag), mag);
+        return std::max(dot(v, hv), 0.0);
+    };
+    out.eigGravity = power(false, out.iterationsGravity);
+    out.eigMagnetic = power(true, out.iterationsMagnetic);
+    out.ratio = out.eigMagnetic > 1e-60 ? out.eigGravity / out.eigMagnetic : 0.0;
+    if (std::isfinite(out.ratio) && out.ratio > 0.0) {
+        out.alpha = alpha_mag * std::max(1e-6, std::min(out.ratio, 1e6));
+    } else {
+        // gradient-norm ratio of the weighted data misfits
+        Vec gz, tmi;
+        f.forward(params, gz, tmi);
+        Vec rg(ns), rm(ns);
+        for (size_t i = 0; i < ns; ++i) {
+            rg[i] = wgt_g[i] * (gdata[i].g_obs - gz[i]);
+            rm[i] = wgt_m[i] * (mdata[i].t_obs - tmi[i]);
+        }
+        const Vec gg = jt(rg, false), gm = jt(rm, true);
+        const double ng = std::sqrt(dot(gg, gg)), nm = std::sqrt(dot(gm, gm));
+        out.fallback = true;
+        if (ng >= 1e-60 && nm > 1e-60) out.alpha = alpha_mag * ng / nm;
+    }
+    model->applyParameterVector(params);
+    return out;
 }
 
 namespace {

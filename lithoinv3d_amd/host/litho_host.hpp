@@ -353,6 +353,7 @@ public:
     Vec gradient(const Vec& params);
     JointComponents evaluateComponents(const Vec& params);
     double alphaMag() const { return m_alpha; }
+    void setAlphaMag(double a) { m_alpha = a; }
 
 private:
     std::shared_ptr<ObjectiveFunction> m_grav;
@@ -381,6 +382,10 @@ public:
     int adjointPasses() const { return m_nAdjoint; }
     // re-read the unit properties from the model (after they were changed)
     void refreshProperties();
+    double alphaMag() const { return m_alpha; }
+    void setAlphaMag(double a) { m_alpha = a; }
+    // sum_i wg[i] dgz_i/dp + wt[i] dTMI_i/dp at params (either weight vector may be empty)
+    Vec jtw(const Vec& params, const Vec& wg, const Vec& wt);
 
 private:
     void ensureForward(const Vec& params);
@@ -414,6 +419,25 @@ Vec invertUnitDensities(GravityForward& fwd, const GravityData& data, LithologyM
                         const PropertyInversionOptions& opt);
 Vec invertUnitSusceptibilities(MagneticForward& fwd, const MagneticData& data,
                                LithologyModel& model, const PropertyInversionOptions& opt);
+
+// JointObjective::calibrateDataWeights (joint_objective.cpp:281-393) without forming J:
+// power iteration (at most 50 steps, relative tolerance 1e-8) for the largest eigenvalue
+// of J^T W^2 J of gravity and of magnetics (W = 1/std, or 1 where std is 0), J v from
+// central differences of an FP64 forward (step eps in parameter units along the unit
+// vector v) and J^T u from the adjoint; alpha_mag is multiplied by the eigenvalue ratio
+// clipped to [1e-6, 1e6], or, if the ratio is unusable, by the ratio of the two data-misfit
+// gradient norms.  The power iterations start from a seeded random unit vector (the
+// reference uses Eigen's unseeded Random()).
+struct CalibrationResult {
+    double eigGravity = 0.0, eigMagnetic = 0.0, ratio = 0.0, alpha = 0.0;
+    int iterationsGravity = 0, iterationsMagnetic = 0;
+    bool fallback = false;
+};
+CalibrationResult calibrateMagneticWeight(std::shared_ptr<LithologyModel> model,
+                                          const GravityData& gdata, const MagneticData& mdata,
+                                          double inc_deg, double dec_deg, double field_nT,
+                                          const Vec& params, double alpha_mag,
+                                          uint64_t seed = 1, double eps = 1e-3, int device = 0);
 
 // modules/litho-inversion/include/litho_invert/inversion/optimizer.h:7-36
 struct OptimizerResult {

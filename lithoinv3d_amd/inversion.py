@@ -63,6 +63,7 @@ def load_host():
     L.li_problem_add_smoothness.argtypes = [_P, ctypes.c_double]
     L.li_problem_invert_properties.argtypes = [_P, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                                ctypes.c_double, ctypes.c_int, ctypes.c_double, _P]
+    L.li_problem_calibrate.argtypes = [_P, _P, ctypes.c_uint64, ctypes.c_double, _P]
     L.li_model_boreholes.argtypes = [_P, ctypes.c_int64, _P, _P, _P, _P, ctypes.c_double, _P,
                                      ctypes.c_double, ctypes.POINTER(ctypes.c_double), _P, _P]
     L.li_problem_add_boreholes.argtypes = [_P, ctypes.c_int64, _P, _P, _P, _P, ctypes.c_double,
@@ -321,6 +322,20 @@ class Problem:
                                                       max_iter, tol, _p(out)),
              "invert_properties")
         return out
+
+    def calibrate(self, params=None, seed=1, eps=1e-3):
+        """Sets alpha_mag from the ratio of the largest eigenvalues of J^T W^2 J of gravity
+        and of magnetics at params (default: the model's current parameters), W = 1/std,
+        found matrix-free by power iteration (J v by central differences of an FP64
+        forward, J^T u by the adjoint).  Returns a dict with both eigenvalues, their ratio,
+        the new alpha_mag, the iteration counts and whether the gradient-norm fallback ran."""
+        p = self.model.params() if params is None else np.ascontiguousarray(params, np.float64)
+        out = np.zeros(7)
+        _chk(load_host().li_problem_calibrate(self._h, _p(p), int(seed), float(eps), _p(out)),
+             "calibrate")
+        return dict(eig_gravity=out[0], eig_magnetic=out[1], ratio=out[2], alpha_mag=out[3],
+                    iterations_gravity=int(out[4]), iterations_magnetic=int(out[5]),
+                    fallback=bool(out[6]))
 
     def set_fused(self, on: bool):
         """True (default with magnetic data): one GPU context for gz and TMI, one adjoint

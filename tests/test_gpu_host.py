@@ -180,3 +180,54 @@ def test_property_inversion_recovers_units():
     comp = P.evaluate(M.params())
     print("objective components after the property phase", comp)
     P.close()
+
+
+def test_calibrate_magnetic_weight_vs_dense_eigenvalues(oracle):
+    """JointObjective::calibrateDataWeights (joint_objective.cpp:281-393) done matrix-free
+    (power iteration, J v by central differences of the FP64 forward, J^T u by the
+    adjoint) against numpy's largest eigenvalue of J^T diag(1/std^2) J with J the
+    reference's per-face FD Jacobian (oracle) on Z-only DOFs x axisScale: rel. 1e-3.  The
+    new alpha_mag is alpha_mag x the clipped ratio and the objective takes it; a gravity
+    std that makes the ratio exceed 1e6 is clipped to 1e6."""
+    m, st = model_and_stations(12)
+    fh = synth.field_unit_vector(-60.0, 0.0)
+    M = inversion.Model(m)
+    M.fix_exterior(inversion.Z_ONLY)
+    M.set_axis_scale(AXIS_SCALE)
+    v, _, _ = M.dofs()
+    p0 = M.params()
+    om = oracle.OracleModel.from_litho(m, 50000.0, fh)
+    Jg = om.gravity_vertex_jacobian(st)[:, v, 2] * AXIS_SCALE
+    Jm = om.magnetic_vertex_jacobian(fh, st)[:, v, 2] * AXIS_SCALE
+    rng = np.random.default_rng(9)
+    gstd = rng.uniform(0.02, 0.08, len(st))
+    tstd = rng.uniform(0.5, 2.0, len(st))
+    eg = np.linalg.eigvalsh(Jg.T @ (Jg / gstd[:, None] ** 2))
+    em = np.linalg.eigvalsh(Jm.T @ (Jm / tstd[:, None] ** 2))
+    print(f"dense: lambda_g {eg[-1]:.6e} (next/largest {eg[-2] / eg[-1]:.3f}), "
+          f"lambda_m {em[-1]:.6e} (next/largest {em[-2] / em[-1]:.3f})")
+    alpha0 = 0.3
+    say("calibrate alpha_mag")
+    P = inversion.Problem(M, st, rng.normal(0, 0.5, len(st)), gstd, rng.normal(0, 30, len(st)),
+                          tstd, alpha_mag=alpha0)
+    c0 = P.evaluate(p0)
+    r = P.calibrate(p0, seed=3)
+    print(r)
+    assert not r["fallback"]
+    assert abs(r["eig_gravity"] / eg[-1] - 1) < 1e-3
+    assert abs(r["eig_magnetic"] / em[-1] - 1) < 1e-3
+    ratio = r["eig_gravity"] / r["eig_magnetic"]
+    assert abs(r["ratio"] / ratio - 1) < 1e-12
+    assert abs(r["alpha_mag"] / (alpha0 * np.clip(ratio, 1e-6, 1e6)) - 1) < 1e-12
+    np.testing.assert_allclose(M.params(), p0, rtol=0, atol=0)  # the model is left at p0
+    c1 = P.evaluate(p0)
+    mag0, mag1 = c0[3] - c0[0] - c0[2], c1[3] - c1[0] - c1[2]
+    np.testing.assert_allclose(mag1, mag0 * r["alpha_mag"] / alpha0, rtol=1e-10)
+    P.close()
+    say("calibrate alpha_mag, clipped ratio")
+    P = inversion.Problem(M, st, np.zeros(len(st)), np.full(len(st), 1e-6), np.zeros(len(st)),
+                          np.full(len(st), 1e3), alpha_mag=alpha0)
+    r = P.calibrate(p0, seed=4)
+    print(r)
+    assert r["ratio"] > 1e6 and abs(r["alpha_mag"] / (alpha0 * 1e6) - 1) < 1e-12
+    P.close()

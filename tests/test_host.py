@@ -13,12 +13,12 @@ from lithoinv3d_amd import lithofwd
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_symbols():
+def declared_symbols(prefix="lf_"):
     names = set()
     for fn in os.listdir(os.path.join(ROOT, "include")):
         if fn.endswith(".h"):
             txt = open(os.path.join(ROOT, "include", fn)).read()
-            names |= set(re.findall(r"^\w[\w\s\*]*?\b(lf_\w+)\s*\(", txt, re.M))
+            names |= set(re.findall(r"^\w[\w\s\*]*?\b(" + prefix + r"\w+)\s*\(", txt, re.M))
     return names
 
 
@@ -29,6 +29,21 @@ def test_library_exports_declared_symbols():
     missing = [n for n in sorted(names) if not hasattr(L, n)]
     assert not missing, missing
     assert set(lithofwd.EXPORTED) | set(lithofwd.EXPORTED_MLP) == names
+
+
+def test_host_library_exports_declared_symbols():
+    """liblithohost.so exports every li_ entry point include/lithohost.h declares, and
+    declares every li_ function its C source defines."""
+    from lithoinv3d_amd import inversion
+    L = inversion.load_host()
+    names = declared_symbols("li_")
+    assert {"li_model_create", "li_problem_create", "li_problem_minimize",
+            "li_problem_calibrate", "li_assemble"} <= names
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+    src = open(os.path.join(ROOT, "lithoinv3d_amd", "host", "litho_capi.cpp")).read()
+    defined = set(re.findall(r"^(?:int|void|int64_t|const char\*) (li_\w+)\(", src, re.M))
+    assert defined == names, sorted(defined ^ names)
 
 
 def directed_edges(F):
