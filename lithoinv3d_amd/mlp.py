@@ -1,17 +1,22 @@
 """Learned diagonal initial inverse Hessian for the projected L-BFGS.
 
-A 16 -> 64 -> 64 -> 1 GELU MLP (trained and evaluated by the lf_mlp_* kernels of
-liblithofwd.so: BF16 MFMA, FP32 accumulation) predicts, per control-point parameter, the
-log of the Gauss-Newton diagonal of the joint misfit,
+The optimizer starts from H0 = gamma * diag(exp(-L/2)), L an estimate of the log of the
+Gauss-Newton diagonal of the joint misfit per control-point parameter,
 
-    d_j = sum_i (dgz_i/dp_j / sigma_gz)^2 + alpha * sum_i (dTMI_i/dp_j / sigma_tmi)^2,
+    d_j = sum_i (dgz_i/dp_j / sigma_gz)^2 + alpha * sum_i (dTMI_i/dp_j / sigma_tmi)^2
 
-from local geometry; the optimizer then starts from H0 = gamma * diag(exp(-prediction/2))
-(one over the predicted column norm; full Jacobi scaling, exp(-prediction), is slower on
-these problems).
+(one over the column norm; full Jacobi scaling, exp(-L), is slower on these problems).
+L = B + r: B is the same sum over a sparse subset of the stations, one per cell of a
+grid `stride` station spacings wide (1/1024 of a regular grid at stride 32), scaled by
+n_stations / n_subset and evaluated by the GPU column-norm kernel (lf_colnorm2) at a
+small fraction of the full cost; r is a correction predicted by a 16 -> 64 -> 64 -> 1
+GELU MLP (trained and evaluated by the lf_mlp_* kernels of liblithofwd.so: BF16 MFMA,
+FP32 accumulation) from local geometry and B itself.  The subset undersamples the
+response of shallow parameters, whose footprint is narrower than the subset's spacing;
+the network learns that, halving the error of B alone.
 The objective stays exact, so the plain solve (H0 = gamma * I) remains both the fallback
-and the judge of the gain.  Training targets are the exact column norms produced by the
-GPU forward machinery (lf_colnorm2).
+and the judge of the gain.  Training targets are the exact column norms over all
+stations (lf_colnorm2).
 """
 from __future__ import annotations
 
@@ -25,10 +30,11 @@ from . import lithofwd
 N_FEATURES = 16
 FEATURE_NAMES = (
     "depth_km", "log_area", "log_chi_sum_area", "log_chi_sum_area_fpar", "n_z",
-    "log_stations_within_2_depths", "abs_drho", "log_abs_dchi", "log_support", "weight_sum",
-    "log_depth", "log_area_over_depth2", "log_chi_sum_area_nz", "abs_drho_nz",
-    "abs_dchi_nz_scaled", "edge_distance_over_depth",
+    "log_stations_within_2_depths", "abs_drho", "log_abs_dchi", "log_subset_colnorm_gz",
+    "log_subset_colnorm_tmi", "log_subset_spacing_over_depth", "log_area_over_depth2",
+    "log_chi_sum_area_nz", "abs_drho_nz", "abs_dchi_nz_scaled", "edge_distance_over_depth",
 )
+SUBSET_STRIDE = 32
 
 _lib_ready = False
 
@@ -159,10 +165,61 @@ def vertex_areas_normals(m, field_dir=(0.0, 0.0, 1.0)):
                 S_par=acc[:, 7], S_z=acc[:, 8])
 
 
-def dof_features(m, dof_map, dof_vertex, stations, axis_scale, field_dir=(0.0, 0.0, 1.0)):
+def station_subset(stations, stride=SUBSET_STRIDE):
+    """Indices of a sparse station subset: the station nearest the centre of each occupied
+    cell of a horizontal grid `stride` mean station spacings wide, and that cell width.
+    The spacing s solves (W + s)(H + s) = n s^2 for the bounding box W x H (s is the
+    pitch of a regular grid); the cells start half a spacing before the box, so a regular
+    grid of k*stride stations per side gives k cells per side."""
+    xy = np.asarray(stations, np.float64)[:, :2]
+    n = len(xy)
+    if n < 2:
+        return np.arange(n), 0.0
+    lo, hi = xy.min(0), xy.max(0)
+    W, H = np.maximum(hi - lo, 0.0)
+    sp = ((W + H) + np.sqrt((W + H) ** 2 + 4.0 * (n - 1) * W * H)) / (2.0 * (n - 1))
+    if sp <= 0.0:
+        return np.arange(n), 0.0
+    cell = stride * sp
+    org = lo - 0.5 * sp
+    ij = np.floor((xy - org) / cell).astype(np.int64)
+    centre = org + (ij + 0.5) * cell
+    d2 = ((xy - centre) ** 2).sum(1)
+    key = ij[:, 0] * (1 << 31) + ij[:, 1]
+    order = np.lexsort((d2, key))
+    first = np.ones(len(order), bool)
+    first[1:] = key[order][1:] != key[order][:-1]
+    return np.sort(order[first]), float(cell)
+
+
+@dataclasses.dataclass
+class SubsetDiagonal:
+    gz: np.ndarray    # sum over the subset of (dgz/dp_j / sigma_gz)^2, scaled to all stations
+    tmi: np.ndarray   # alpha * the same for TMI
+    spacing: float    # subset spacing (m)
+
+    def log_diag(self):
+        return np.log(np.maximum(self.gz + self.tmi, 1e-300))
+
+
+def subset_gn_diagonal(m, stations, dof_map, field_dir, field_nT=50000.0, sigma_gz=0.05,
+                       sigma_tmi=1.0, alpha=1.0, axis=2, stride=SUBSET_STRIDE,
+                       precision="fp32c", device=0):
+    """The Gauss-Newton diagonal over station_subset(stations, stride), each part scaled
+    by n_stations / n_subset (GPU column-norm kernel)."""
+    idx, cell = station_subset(stations, stride)
+    with lithofwd.ForwardContext(m, np.asarray(stations)[idx], field_dir, field_nT,
+                                 precision=precision, device=device) as ctx:
+        cg, ct = ctx.colnorm2(axis, *dof_map)
+    scale = len(stations) / len(idx)
+    return SubsetDiagonal(cg * scale / sigma_gz ** 2, alpha * ct * scale / sigma_tmi ** 2, cell)
+
+
+def dof_features(m, dof_map, dof_vertex, stations, axis_scale, sub: SubsetDiagonal,
+                 field_dir=(0.0, 0.0, 1.0)):
     """(n_dof, 16) features per control-point parameter (FEATURE_NAMES), from the
-    parameter's vertices and their interpolation weights (dof_map), the stations and
-    the inducing-field direction."""
+    parameter's vertices and their interpolation weights (dof_map), the stations, the
+    inducing-field direction and the subset Gauss-Newton diagonal."""
     from scipy.spatial import cKDTree
 
     off, vert, w = dof_map
@@ -192,8 +249,6 @@ def dof_features(m, dof_map, dof_vertex, stations, axis_scale, field_dir=(0.0, 0
     xmin, ymin = sxy.min(0)
     xmax, ymax = sxy.max(0)
     edge = np.minimum.reduce([P[:, 0] - xmin, xmax - P[:, 0], P[:, 1] - ymin, ymax - P[:, 1]])
-    support = np.diff(off).astype(np.float64)
-    wsum = np.bincount(seg, weights=ww, minlength=n)
     tiny = 1e-12
     X = np.stack([
         depth / 1000.0,
@@ -204,9 +259,9 @@ def dof_features(m, dof_map, dof_vertex, stations, axis_scale, field_dir=(0.0, 0
         np.log1p(cnt),
         Reff,
         np.log10(np.maximum(Ceff, 1e-6)),
-        np.log(support),
-        wsum,
-        np.log(depth),
+        np.log(np.maximum(sub.gz, 1e-300)),
+        np.log(np.maximum(sub.tmi, 1e-300)),
+        np.full(n, np.log(sub.spacing)) - np.log(depth),
         np.log(np.maximum(Aeff, 1e-6) / depth ** 2),
         np.log(np.abs(dofsum(va["S_z"])) + tiny),
         Reff * np.abs(nn[:, 2]),

@@ -82,35 +82,41 @@ def test_training_reduces_loss(net):
 
 
 def test_learned_h0_gain_over_plain_solve():
-    """What the trained network gains: trained (on the GPU) on 24 seeded small contact
-    models (32x32x16 cells, 64x64 stations) and used as H0 on 4 held-out ones.  Each
-    held-out model: the plain FP32C L-BFGS-B runs 30 iterations; its final misfit is the
-    target for the learned-H0 run.  Passes when the learned runs reach the target in
-    fewer iterations than the plain runs needed (median over the 4 models); every
-    number is printed."""
+    """What the trained network gains: trained (on the GPU) on 48 seeded small contact
+    models (32x32x16 cells, 64x64 stations) to correct the 1/1024-station-subset estimate
+    of the log Gauss-Newton diagonal, and used as H0 on 4 held-out ones.  Passes when
+    (1) on the held-out models the corrected estimate is closer to the exact log diagonal
+    than the subset estimate alone (RMS after removing the constant offset, which the
+    solver's gamma absorbs), and (2) the learned-H0 FP32C runs reach the plain FP32C run's
+    30-iteration misfit in fewer iterations than the plain runs needed (median over the 4
+    models).  Every number is printed."""
     from lithoinv3d_amd import experiments as E
 
     shape, n_st = (32, 32, 16), 64
     say("learned H0: training set")
-    X, t = E.training_set(range(5000, 5024), shape, n_st)
+    X, t = E.training_set(range(5000, 5048), shape, n_st)
     net, norm, losses = E.train_mlp(X, t, steps=1500)
     print(f"trained on {len(t)} parameters: loss {losses[0]:.3f} -> {losses[-1]:.3f}")
-    plain_it, learned_it = [], []
+    plain_it, learned_it, err_l, err_b = [], [], [], []
     for s in range(6000, 6004):
         say(f"learned H0: held-out seed {s}")
         prob = E.contact_problem(s, shape, n_st)
         E.synth_data(prob)
         rp, _ = E.run_solver(prob, "fp32c", 30)
-        Xh, _ = E.features_targets(prob, with_targets=False)
-        pred = norm.t_inv(net.forward(norm.x(Xh))).astype(np.float64)
+        _, exact, _ = E.features_targets(prob)
+        pred, base = E.predict_log_diag(prob, net, norm)
+        err_l.append(np.std(pred - exact))
+        err_b.append(np.std(base - exact))
         rl, _ = E.run_solver(prob, "fp32c", 90, target=rp.f, h0=np.exp(-E.H0_POWER * pred))
         hit = E.time_to(rl, rp.f)
         plain_it.append(rp.iterations)
         learned_it.append(hit[0] if hit else 90)
-        print(f"seed {s}: plain {rp.iterations} iterations to f={rp.f:.4e}; learned H0 "
+        print(f"seed {s}: log-diagonal RMS error learned {err_l[-1]:.3f}, subset only "
+              f"{err_b[-1]:.3f}; plain {rp.iterations} iterations to f={rp.f:.4e}; learned H0 "
               f"{learned_it[-1]} ({rl.message})")
     net_gain = 1.0 - np.median(learned_it) / np.median(plain_it)
     print(f"median iterations: plain {np.median(plain_it)}, learned {np.median(learned_it)} "
           f"({100 * net_gain:.0f} % fewer)")
     net.close()
+    assert np.median(err_l) < np.median(err_b)
     assert np.median(learned_it) < np.median(plain_it)

@@ -125,3 +125,28 @@ def test_forward_context_rejects_bad_input():
     bad2.front[0] = 5
     with pytest.raises(lithofwd.LithoFwdError, match="invalid unit id"):
         lithofwd.ForwardContext(bad2, np.zeros((1, 3)))
+
+
+def test_station_subset():
+    """One station per cell of a grid `stride` spacings wide, the one nearest the cell
+    centre: a regular 64 x 64 grid at stride 32 keeps 4 stations, one per quadrant, each
+    within one spacing of its quadrant's centre; stride 1 keeps every station of the
+    grid; an irregular set keeps one station per occupied cell."""
+    from lithoinv3d_amd import mlp
+    st = synth.topo_stations(64, 1600.0)
+    idx, cell = mlp.station_subset(st, 32)
+    assert len(idx) == 4 and len(set(idx)) == 4
+    xy = st[idx, :2]
+    lo = st[:, :2].min(0)
+    d = (st[1, 0] - st[0, 0]) if st[1, 0] != st[0, 0] else (st[1, 1] - st[0, 1])
+    q = np.floor((xy - lo + abs(d) / 2) / cell).astype(int)
+    assert sorted(map(tuple, q)) == [(0, 0), (0, 1), (1, 0), (1, 1)]
+    np.testing.assert_array_less(np.abs(xy - (lo - abs(d) / 2 + (q + 0.5) * cell)), abs(d) + 1e-9)
+    idx1, _ = mlp.station_subset(st, 1)
+    assert len(idx1) == len(st)
+    rng = np.random.default_rng(1)
+    pts = rng.uniform(0, 1000, (500, 3))
+    idx, cell = mlp.station_subset(pts, 4)
+    org = pts[:, :2].min(0) - cell / 8
+    occupied = {tuple(c) for c in np.floor((pts[:, :2] - org) / cell).astype(int)}
+    assert len(idx) == len(occupied)
