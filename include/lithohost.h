@@ -29,8 +29,11 @@ int li_model_create(void** out, int64_t n_v, const double* verts, int64_t n_f,
                     const uint32_t* faces, const int32_t* front, const int32_t* back,
                     int32_t n_u, const double* density, const double* susceptibility);
 void li_model_destroy(void* h);
+/* current per-unit density and susceptibility (n_units each; LithoGroup, litho_group.h) */
+int li_model_properties(void* h, double* density, double* susceptibility);
 /* fixExteriorFaces (lithology_model.cpp:369-579): vertices of model-bounding facets FIXED,
- * the rest `interior_freedom` (0 FIXED, 1 Z_ONLY, 2 XY_ONLY, 3 FREE; common.h:16-38). */
+ * the rest `interior_freedom` (VertexFreedom of surface_mesh.h: 0 FIXED, 1 Z_ONLY,
+ * 2 X_ONLY, 3 Y_ONLY, 8 XYZ_FREE). */
 int li_model_fix_exterior(void* h, int interior_freedom);
 /* setVertexFreedom per vertex (n_v values, same codes) */
 int li_model_set_freedom(void* h, const uint8_t* freedom);
@@ -110,6 +113,52 @@ int li_problem_jacobian(void* h, const double* params, int which, double* J_colm
 /* unit-property response of one group at the current geometry (gravity_forward.cpp:132-145) */
 int li_problem_unit_response(void* h, int which, int group, double* out);
 
+/* ---- the inversion loop (runner.cpp:217-641) ---- */
+
+/* InversionConfig (runner.h:27-170), the fields the loop uses */
+typedef struct {
+    int32_t max_iterations;      /* geometry iterations in total */
+    int32_t property_interval;   /* geometry iterations between property phases */
+    int32_t geometry;            /* 0: property phases only, one round */
+    int32_t properties;          /* alternate with density / susceptibility phases */
+    int32_t eigenvalue_scaling;  /* calibrate alpha_mag first (li_problem_calibrate) */
+    int32_t validate;            /* validation gate after each geometry phase */
+    int32_t depth_bounds;        /* also bound each parameter to start -/+ depth_margin */
+    int32_t lbfgs_history;
+    int32_t line_search_max_iter;
+    int32_t property_max_iter;
+    double depth_margin;         /* parameter units */
+    double bounds_margin;        /* metres (li_model_bounds) */
+    double gradient_smoothing;   /* smoothGradient weight (runner.cpp:643-706), 0 = off */
+    double tolerance;
+    double armijo_c1;
+    double density_lower, density_upper, density_damping;
+    double susceptibility_lower, susceptibility_upper, susceptibility_damping;
+    double property_tolerance;
+} LiRunnerOptions;
+/* hist rows of 10: iteration, gravity misfit, magnetic misfit, regularization, total,
+ * gravity RMS, Durbin-Watson gravity x, y, magnetic x, y (InversionIteration,
+ * inversion_result.h:10-28), one per objective evaluation at new parameters;
+ * summary[8]: converged, geometry iterations, final objective, final gravity RMS,
+ * calibrated alpha_mag (0: none), validation retries, rejections, history entries in
+ * total; params_out: the final parameters; log: one line per phase and event.
+ * label_grid (a li_label_grid_load handle) may be NULL. */
+int li_problem_run(void* h, const LiRunnerOptions* o, void* label_grid, double* params_out,
+                   double* hist, int64_t hist_cap, int64_t* n_hist, double* summary, char* log,
+                   int64_t log_cap);
+/* classifyPoint (lithology_model.cpp:288-362): the label grid's unit where it holds one,
+ * else the lowest unit whose surface a +x ray crosses an odd number of times, else -1
+ * (and -1 below bottom_depth, the reference's default -5000) */
+int li_model_classify(void* h, void* label_grid, double bottom_depth, int64_t n,
+                      const double* xyz, int32_t* out);
+/* LithologyModel::validate (lithology_model.cpp:624-798) at the current geometry:
+ * counts[2] = adjacent and non-adjacent overlaps; negative baselines: counts only */
+int li_model_validate(void* h, void* label_grid, double bottom_depth, int64_t overlap_baseline,
+                      int64_t adjacent_baseline, int32_t* passed, double* worst,
+                      int64_t* counts, char* reason, int64_t reason_cap);
+/* computeDurbinWatson (modules/litho-core/src/core/stats.cpp:31-85): out[2] = x, y */
+int li_durbin_watson(int64_t n, const double* xyz, const double* residuals, double* out);
+
 /* ---- L-BFGS-B (lbfgsb_optimizer.cpp:13-263, optimizer.h:7-36) ---- */
 
 typedef double (*li_obj_cb)(const double* x, int64_t n, void* user);
@@ -159,6 +208,8 @@ const char* li_constraints_warnings(void* h);
 void li_constraints_destroy(void* h);
 /* label grid of python/lithoseed/src/lithoseed/_surfaces.py:659-671 */
 int li_label_grid_load(const char* path, void** out);
+int li_label_grid_create(const int32_t* dims, const double* origin, const double* cell,
+                         const int32_t* labels, void** out);
 int li_label_grid_info(void* h, int32_t* dims, double* origin, double* cell);
 int li_label_grid_get(void* h, int32_t* labels);
 void li_label_grid_destroy(void* h);

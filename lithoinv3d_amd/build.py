@@ -46,7 +46,8 @@ def sources():
 def build_host(verbose: bool = False) -> str:
     """The C++ host layer (ForwardModel / objectives / L-BFGS-B) over the C ABI,
     linked against liblithofwd.so."""
-    srcs = [os.path.join(HOST, f) for f in ("litho_host.cpp", "litho_io.cpp", "litho_capi.cpp")]
+    srcs = [os.path.join(HOST, f) for f in ("litho_host.cpp", "litho_io.cpp", "litho_runner.cpp",
+                                            "litho_capi.cpp")]
     cmd = [CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra",
            "-Wno-unused-parameter"] + srcs + [
         "-L" + HERE, "-llithofwd", "-Wl,-rpath,$ORIGIN", "-o", HOST_LIB + ".tmp"]

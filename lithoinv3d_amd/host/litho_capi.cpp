@@ -10,6 +10,7 @@
 #include "../../include/lithohost.h"
 #include "litho_host.hpp"
 #include "litho_io.hpp"
+#include "litho_runner.hpp"
 
 using namespace lithofwd;
 
@@ -95,6 +96,17 @@ int li_model_create(void** out, int64_t n_v, const double* verts, int64_t n_f,
 }
 
 void li_model_destroy(void* h) { delete (ModelHandle*)h; }
+
+int li_model_properties(void* h, double* density, double* susceptibility)
+{
+    return guard([&] {
+        const auto& m = *((ModelHandle*)h)->m;
+        for (int g = 0; g < m.groupCount(); ++g) {
+            density[g] = m.group(g).density;
+            susceptibility[g] = m.group(g).susceptibility;
+        }
+    });
+}
 
 int li_model_fix_exterior(void* h, int interior_freedom)
 {
@@ -354,6 +366,152 @@ int li_problem_calibrate(void* h, const double* params, uint64_t seed, double ep
                              (double)r.iterationsGravity, (double)r.iterationsMagnetic,
                              r.fallback ? 1.0 : 0.0};
         std::memcpy(out, o, sizeof(o));
+    });
+}
+
+namespace {
+void copy_text(const std::string& t, char* out, int64_t cap)
+{
+    if (!out || cap <= 0) return;
+    const size_t n = std::min<size_t>(t.size(), (size_t)cap - 1);
+    std::memcpy(out, t.data(), n);
+    out[n] = 0;
+}
+}  // namespace
+
+// InversionRunner::run (runner.cpp:217-641) over this problem: calibration, geometry
+// phases with the validation gate, property phases.  hist rows of 10: iteration,
+// gravity misfit, magnetic misfit, regularization, total, gravity RMS, DW gravity x/y,
+// DW magnetic x/y.  summary[8]: converged, geometry iterations, final objective, final
+// gravity RMS, alpha_mag after calibration (0 if none), validation retries, rejections,
+// history entries in total.
+int li_problem_run(void* h, const LiRunnerOptions* o, void* label_grid, double* params_out,
+                   double* hist, int64_t hist_cap, int64_t* n_hist, double* summary, char* log,
+                   int64_t log_cap)
+{
+    return guard([&] {
+        auto* P = (Problem*)h;
+        RunnerOptions ro;
+        ro.maxIterations = o->max_iterations;
+        ro.propertyInterval = o->property_interval;
+        ro.geometry = o->geometry != 0;
+        ro.properties = o->properties != 0;
+        ro.eigenvalueScaling = o->eigenvalue_scaling != 0;
+        ro.validate = o->validate != 0;
+        ro.depthBounds = o->depth_bounds != 0;
+        ro.depthMargin = o->depth_margin;
+        ro.boundsMargin = o->bounds_margin;
+        ro.gradientSmoothing = o->gradient_smoothing;
+        ro.lbfgsHistory = o->lbfgs_history;
+        ro.tolerance = o->tolerance;
+        ro.armijoC1 = o->armijo_c1;
+        ro.lineSearchMaxIter = o->line_search_max_iter;
+        ro.densityOptions = {o->density_damping, o->density_lower, o->density_upper,
+                             o->property_max_iter, o->property_tolerance};
+        ro.susceptibilityOptions = {o->susceptibility_damping, o->susceptibility_lower,
+                                    o->susceptibility_upper, o->property_max_iter,
+                                    o->property_tolerance};
+        RunnerProblem rp;
+        rp.gdata = &P->gdata;
+        rp.mdata = P->mag ? &P->mdata : nullptr;
+        rp.components = [P](const Vec& p) { return P->components(p); };
+        rp.gradient = [P](const Vec& p) { return P->grad(p); };
+        rp.predict = [P](const Vec& p, Vec& gz, Vec& tmi) {
+            if (P->fused && P->useFused) {
+                P->fused->forward(p, gz, tmi);
+                return;
+            }
+            gz = P->grav->compute(p);
+            tmi.clear();
+            if (P->mag) tmi = P->mag->compute(p);
+        };
+        rp.invertProperties = [P](int which, const PropertyInversionOptions& po) {
+            if (which == 0)
+                invertUnitDensities(*P->grav, P->gdata, *P->model, po);
+            else if (P->mag)
+                invertUnitSusceptibilities(*P->mag, P->mdata, *P->model, po);
+            if (P->mag) P->mag->refreshProperties();
+            P->grav->refreshProperties();
+            if (P->fused) P->fused->refreshProperties();
+        };
+        rp.calibrate = [P](const Vec& p) {
+            const CalibrationResult r = calibrateMagneticWeight(
+                P->model, P->gdata, P->mdata, P->inc, P->dec, P->field, p, P->joint->alphaMag(),
+                1, 1e-3, P->device);
+            P->joint->setAlphaMag(r.alpha);
+            if (P->fused) P->fused->setAlphaMag(r.alpha);
+            return r.alpha;
+        };
+        const LabelGridData* grid = label_grid ? &((GridHandle*)label_grid)->g : nullptr;
+        const RunnerResult r = runInversion(P->model, rp, ro, grid);
+        std::memcpy(params_out, r.params.data(), sizeof(double) * r.params.size());
+        const int64_t nh = std::min<int64_t>((int64_t)r.history.size(), hist_cap);
+        for (int64_t k = 0; k < nh; ++k) {
+            const auto& it = r.history[(size_t)k];
+            const double row[10] = {(double)it.iteration, it.gravityMisfit, it.magneticMisfit,
+                                    it.regularization, it.total, it.rms, it.dwGravityX,
+                                    it.dwGravityY, it.dwMagneticX, it.dwMagneticY};
+            std::memcpy(hist + 10 * k, row, sizeof(row));
+        }
+        *n_hist = nh;
+        const double sm[8] = {r.converged ? 1.0 : 0.0, (double)r.totalIterations, r.finalMisfit,
+                              r.finalRMS, r.alphaMag, (double)r.validationRetries,
+                              (double)r.validationRejections, (double)r.history.size()};
+        std::memcpy(summary, sm, sizeof(sm));
+        copy_text(r.log, log, log_cap);
+    });
+}
+
+// classifyPoint (lithology_model.cpp:288-362) for n points; label_grid may be NULL
+int li_model_classify(void* h, void* label_grid, double bottom_depth, int64_t n,
+                      const double* xyz, int32_t* out)
+{
+    return guard([&] {
+        const auto& m = *((ModelHandle*)h)->m;
+        const LabelGridData* grid = label_grid ? &((GridHandle*)label_grid)->g : nullptr;
+        PointClassifier cls(m, grid, bottom_depth);
+        for (int64_t i = 0; i < n; ++i)
+            out[i] = cls.classify(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+    });
+}
+
+// LithologyModel::validate (lithology_model.cpp:624-798) at the model's current geometry.
+// counts[2] = adjacent and non-adjacent overlaps; with baselines < 0 only the counts
+// are computed (the baselines of the starting model, runner.cpp:493-516).
+int li_model_validate(void* h, void* label_grid, double bottom_depth, int64_t overlap_baseline,
+                      int64_t adjacent_baseline, int32_t* passed, double* worst,
+                      int64_t* counts, char* reason, int64_t reason_cap)
+{
+    return guard([&] {
+        const auto& m = *((ModelHandle*)h)->m;
+        const LabelGridData* grid = label_grid ? &((GridHandle*)label_grid)->g : nullptr;
+        PointClassifier cls(m, grid, bottom_depth);
+        if (overlap_baseline < 0 || adjacent_baseline < 0) {
+            const OverlapCounts c = countOverlaps(m, cls);
+            counts[0] = c.adjacent;
+            counts[1] = c.nonAdjacent;
+            *passed = 1;
+            *worst = 0.0;
+            copy_text("", reason, reason_cap);
+            return;
+        }
+        const ValidationResult v = validateModel(m, cls, overlap_baseline, adjacent_baseline);
+        counts[0] = v.adjacentOverlaps;
+        counts[1] = v.nonAdjacentOverlaps;
+        *passed = v.passed ? 1 : 0;
+        *worst = v.worstViolation;
+        copy_text(v.failureReason, reason, reason_cap);
+    });
+}
+
+// computeDurbinWatson (stats.cpp:31-85): out[2] = DW along x and along y
+int li_durbin_watson(int64_t n, const double* xyz, const double* residuals, double* out)
+{
+    return guard([&] {
+        const DurbinWatson d = durbinWatson(std::vector<double>(xyz, xyz + 3 * n),
+                                            Vec(residuals, residuals + n));
+        out[0] = d.x;
+        out[1] = d.y;
     });
 }
 
@@ -624,6 +782,29 @@ int li_label_grid_load(const char* path, void** out)
     return guard([&] {
         auto h = std::make_unique<GridHandle>();
         h->g = loadLabelGrid(path);
+        *out = h.release();
+    });
+}
+
+// a label grid from memory: dims nx, ny, nz; labels with i fastest
+int li_label_grid_create(const int32_t* dims, const double* origin, const double* cell,
+                         const int32_t* labels, void** out)
+{
+    return guard([&] {
+        if (dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0 || cell[0] <= 0 || cell[1] <= 0 ||
+            cell[2] <= 0)
+            throw std::invalid_argument("li_label_grid_create: dims and cell sizes must be > 0");
+        auto h = std::make_unique<GridHandle>();
+        h->g.nx = dims[0];
+        h->g.ny = dims[1];
+        h->g.nz = dims[2];
+        h->g.x0 = origin[0];
+        h->g.y0 = origin[1];
+        h->g.z0 = origin[2];
+        h->g.dx = cell[0];
+        h->g.dy = cell[1];
+        h->g.dz = cell[2];
+        h->g.labels.assign(labels, labels + (size_t)dims[0] * dims[1] * dims[2]);
         *out = h.release();
     });
 }

@@ -31,6 +31,16 @@ class _Options(ctypes.Structure):
                 ("active_set", ctypes.c_int32)]
 
 
+class _RunnerOptions(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "max_iterations", "property_interval", "geometry", "properties", "eigenvalue_scaling",
+        "validate", "depth_bounds", "lbfgs_history", "line_search_max_iter",
+        "property_max_iter")] + [(n, ctypes.c_double) for n in (
+            "depth_margin", "bounds_margin", "gradient_smoothing", "tolerance", "armijo_c1",
+            "density_lower", "density_upper", "density_damping", "susceptibility_lower",
+            "susceptibility_upper", "susceptibility_damping", "property_tolerance")]
+
+
 OBJ_CB = ctypes.CFUNCTYPE(ctypes.c_double, _D, ctypes.c_int64, ctypes.c_void_p)
 GRAD_CB = ctypes.CFUNCTYPE(None, _D, ctypes.c_int64, _D, ctypes.c_void_p)
 
@@ -64,6 +74,16 @@ def load_host():
     L.li_problem_invert_properties.argtypes = [_P, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                                ctypes.c_double, ctypes.c_int, ctypes.c_double, _P]
     L.li_problem_calibrate.argtypes = [_P, _P, ctypes.c_uint64, ctypes.c_double, _P]
+    L.li_problem_run.argtypes = [_P, ctypes.POINTER(_RunnerOptions), _P, _P, _P, i64,
+                                 ctypes.POINTER(i64), _P, ctypes.c_char_p, i64]
+    L.li_model_classify.argtypes = [_P, _P, ctypes.c_double, i64, _P, _P]
+    L.li_model_validate.argtypes = [_P, _P, ctypes.c_double, i64, i64, _P, _P, _P,
+                                    ctypes.c_char_p, i64]
+    L.li_durbin_watson.argtypes = [i64, _P, _P, _P]
+    L.li_model_properties.argtypes = [_P, _P, _P]
+    L.li_label_grid_create.argtypes = [_P, _P, _P, _P, ctypes.POINTER(_P)]
+    L.li_label_grid_destroy.argtypes = [_P]
+    L.li_label_grid_destroy.restype = None
     L.li_model_boreholes.argtypes = [_P, ctypes.c_int64, _P, _P, _P, _P, ctypes.c_double, _P,
                                      ctypes.c_double, ctypes.POINTER(ctypes.c_double), _P, _P]
     L.li_problem_add_boreholes.argtypes = [_P, ctypes.c_int64, _P, _P, _P, _P, ctypes.c_double,
@@ -213,6 +233,71 @@ class Model:
         _chk(load_host().li_model_bounds(self._h, _p(p), margin_m, _p(lo), _p(hi)), "bounds")
         return lo, hi
 
+    def properties(self):
+        """Current unit densities and susceptibilities of the C++ model."""
+        n = self.src.n_units
+        d, c = np.zeros(n), np.zeros(n)
+        _chk(load_host().li_model_properties(self._h, _p(d), _p(c)), "properties")
+        return d, c
+
+    def classify(self, points, grid=None, bottom_depth=-5000.0):
+        """Unit containing each point (classifyPoint): the label grid's where it holds a
+        unit, else by +x ray casting through the unit surfaces; -1 outside."""
+        pts = _a(points).reshape(-1, 3)
+        out = np.zeros(len(pts), np.int32)
+        _chk(load_host().li_model_classify(self._h, None if grid is None else grid._h,
+                                           bottom_depth, len(pts), _p(pts), _p(out)), "classify")
+        return out
+
+    def validate(self, baselines=None, grid=None, bottom_depth=-5000.0):
+        """The validation gate at the current geometry.  baselines = (non-adjacent,
+        adjacent) overlap counts of the starting model; None: only count the overlaps."""
+        non, adj = (-1, -1) if baselines is None else baselines
+        passed = np.zeros(1, np.int32)
+        worst = np.zeros(1)
+        counts = np.zeros(2, np.int64)
+        reason = ctypes.create_string_buffer(512)
+        _chk(load_host().li_model_validate(self._h, None if grid is None else grid._h,
+                                           bottom_depth, int(non), int(adj), _p(passed),
+                                           _p(worst), _p(counts), reason, 512), "validate")
+        return dict(passed=bool(passed[0]), worst=float(worst[0]),
+                    counts=(int(counts[0]), int(counts[1])), reason=reason.value.decode())
+
+
+class LabelGrid:
+    """A label grid (labels[i, j, k], i along x from origin[0] in steps cell[0], k along
+    z upward from origin[2]) for point classification."""
+
+    def __init__(self, labels, origin, cell):
+        lab = np.asarray(labels, np.int32)
+        self.labels = lab
+        dims = np.array(lab.shape, np.int32)
+        flat = np.ascontiguousarray(lab.ravel(order="F"))
+        o, c = _a(origin), _a(cell)
+        self._h = ctypes.c_void_p()
+        _chk(load_host().li_label_grid_create(_p(dims), _p(o), _p(c), _p(flat),
+                                              ctypes.byref(self._h)), "label grid")
+
+    def close(self):
+        if self._h:
+            load_host().li_label_grid_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def durbin_watson(xyz, residuals):
+    """computeDurbinWatson: (DW along x, DW along y) of residuals at stations xyz."""
+    p = _a(xyz).reshape(-1, 3)
+    r = _a(residuals)
+    out = np.zeros(2)
+    _chk(load_host().li_durbin_watson(len(r), _p(p), _p(r), _p(out)), "durbin_watson")
+    return float(out[0]), float(out[1])
+
 
 @dataclasses.dataclass
 class Result:
@@ -336,6 +421,47 @@ class Problem:
         return dict(eig_gravity=out[0], eig_magnetic=out[1], ratio=out[2], alpha_mag=out[3],
                     iterations_gravity=int(out[4]), iterations_magnetic=int(out[5]),
                     fallback=bool(out[6]))
+
+    def run(self, max_iterations=500, property_interval=50, geometry=True, properties=False,
+            eigenvalue_scaling=True, validate=True, grid=None, gradient_smoothing=0.0,
+            bounds_margin=200.0, depth_bounds=False, depth_margin=500.0, lbfgs_history=10,
+            tolerance=1e-6, armijo_c1=1e-4, line_search_max_iter=50,
+            density_bounds=(-1e300, 1e300), density_damping=0.0,
+            susceptibility_bounds=(0.0, 1.0), susceptibility_damping=0.0,
+            property_max_iter=50, property_tolerance=1e-6, history_cap=None):
+        """InversionRunner::run from the model's current parameters: returns the history
+        (rows: iteration, gravity misfit, magnetic misfit, regularization, total, gravity
+        RMS, DW gravity x, y, DW magnetic x, y), the final parameters, densities and
+        susceptibilities, and the loop's summary and log."""
+        o = _RunnerOptions(
+            max_iterations=max_iterations, property_interval=property_interval,
+            geometry=int(geometry), properties=int(properties),
+            eigenvalue_scaling=int(eigenvalue_scaling), validate=int(validate),
+            depth_bounds=int(depth_bounds), lbfgs_history=lbfgs_history,
+            line_search_max_iter=line_search_max_iter, property_max_iter=property_max_iter,
+            depth_margin=depth_margin, bounds_margin=bounds_margin,
+            gradient_smoothing=gradient_smoothing, tolerance=tolerance, armijo_c1=armijo_c1,
+            density_lower=density_bounds[0], density_upper=density_bounds[1],
+            density_damping=density_damping, susceptibility_lower=susceptibility_bounds[0],
+            susceptibility_upper=susceptibility_bounds[1],
+            susceptibility_damping=susceptibility_damping, property_tolerance=property_tolerance)
+        cap = history_cap or 3 * max_iterations + 64
+        hist = np.zeros((cap, 10))
+        nh = ctypes.c_int64()
+        params = np.zeros(self.model.n_dof)
+        summary = np.zeros(8)
+        log = ctypes.create_string_buffer(1 << 16)
+        _chk(load_host().li_problem_run(self._h, ctypes.byref(o),
+                                        None if grid is None else grid._h, _p(params),
+                                        _p(hist), cap, ctypes.byref(nh), _p(summary), log,
+                                        1 << 16), "run")
+        dens, sus = self.model.properties()
+        return dict(history=hist[:nh.value].copy(), params=params, converged=bool(summary[0]),
+                    iterations=int(summary[1]), final_misfit=float(summary[2]),
+                    final_rms=float(summary[3]), alpha_mag=float(summary[4]),
+                    validation_retries=int(summary[5]), validation_rejections=int(summary[6]),
+                    total_entries=int(summary[7]), log=log.value.decode(),
+                    densities=dens, susceptibilities=sus)
 
     def set_fused(self, on: bool):
         """True (default with magnetic data): one GPU context for gz and TMI, one adjoint
