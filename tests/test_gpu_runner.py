@@ -72,11 +72,12 @@ def test_runner_geometry_with_calibration_and_gate():
     rms = np.sqrt(np.mean((gz - g_pred) ** 2))
     assert r["final_rms"] == pytest.approx(rms, rel=1e-12)
     # the last entry is the last evaluation at new parameters; recompute its statistics
-    # when it is the final point
+    # when it is the final point (the run reaches residuals of ~1e-9 mGal, where the
+    # fused forward and the separate one differ in the sixth digit)
     if np.isclose(h[-1, 4], cf[3], rtol=1e-13):
         dwg = inversion.durbin_watson(st, gz - g_pred)
         dwm = inversion.durbin_watson(st, tm - t_pred)
-        np.testing.assert_allclose(h[-1, 5:10], [rms, *dwg, *dwm], rtol=1e-10)
+        np.testing.assert_allclose(h[-1, 5:10], [rms, *dwg, *dwm], rtol=1e-6)
     assert r["total_entries"] == len(h)
     # the objective at the start with the calibrated weight (entry 0 precedes calibration)
     start = c0[0] + r["alpha_mag"] * c0[1] + c0[2]
