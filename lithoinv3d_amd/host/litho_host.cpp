@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <array>
+#include <thread>
 #include <tuple>
 #include <chrono>
 #include <cmath>
@@ -98,6 +99,115 @@ bool axisFree(VertexFreedom f, int a)
 }
 }  // namespace
 
+namespace {
+// body(i) for i in [0, n) on up to 16 threads
+template <class F>
+void parallel_range(int64_t n, F body)
+{
+    const int64_t nt = std::max<int64_t>(1, std::min<int64_t>(
+        {(int64_t)std::max(1u, std::thread::hardware_concurrency()), 16, n / 2048 + 1}));
+    if (nt == 1) {
+        for (int64_t i = 0; i < n; ++i) body(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int64_t t = 0; t < nt; ++t)
+        th.emplace_back([&, t] {
+            for (int64_t i = t; i < n; i += nt) body(i);
+        });
+    for (auto& x : th) x.join();
+}
+
+// Control points of one unit in a dense grid of cubic buckets, for K-nearest queries.
+struct CpBuckets {
+    double lo[3] = {0, 0, 0}, h = 1.0;
+    int64_t n[3] = {0, 0, 0};
+    std::vector<int64_t> off;
+    std::vector<int32_t> cps;
+
+    void build(const std::vector<int32_t>& cpList, const std::vector<double>& V, double bucket)
+    {
+        h = bucket;
+        double hi[3] = {-1e300, -1e300, -1e300};
+        for (int a = 0; a < 3; ++a) lo[a] = 1e300;
+        for (int32_t c : cpList)
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::min(lo[a], V[3 * (size_t)c + a]);
+                hi[a] = std::max(hi[a], V[3 * (size_t)c + a]);
+            }
+        // coarsen the buckets if the grid would be large against the CP count
+        for (;;) {
+            int64_t total = 1;
+            for (int a = 0; a < 3; ++a) {
+                n[a] = (int64_t)std::floor((hi[a] - lo[a]) / h) + 1;
+                total *= n[a];
+            }
+            if (total <= 8 * (int64_t)cpList.size() + 64) break;
+            h *= 2.0;
+        }
+        off.assign((size_t)(n[0] * n[1] * n[2]) + 1, 0);
+        auto idx = [&](int32_t c) {
+            int64_t b[3];
+            for (int a = 0; a < 3; ++a)
+                b[a] = std::clamp((int64_t)std::floor((V[3 * (size_t)c + a] - lo[a]) / h),
+                                  (int64_t)0, n[a] - 1);
+            return (b[0] * n[1] + b[1]) * n[2] + b[2];
+        };
+        for (int32_t c : cpList) ++off[(size_t)idx(c) + 1];
+        for (size_t i = 1; i < off.size(); ++i) off[i] += off[i - 1];
+        cps.resize(cpList.size());
+        std::vector<int64_t> fill(off.begin(), off.end() - 1);
+        for (int32_t c : cpList) cps[(size_t)fill[(size_t)idx(c)]++] = c;
+    }
+
+    // the K nearest control points to p (fewer if the unit has fewer), nearest first
+    void nearest(const double* p, const std::vector<double>& V, int K,
+                 std::vector<std::pair<double, int32_t>>& out) const
+    {
+        out.clear();
+        if (cps.empty()) return;
+        int64_t b[3];
+        for (int a = 0; a < 3; ++a) b[a] = (int64_t)std::floor((p[a] - lo[a]) / h);
+        // distance from p to the grid box; shells beyond it are searched until the K-th
+        // distance is inside the searched cube
+        const int64_t rmax = std::max({n[0], n[1], n[2]}) + 2 +
+                             std::max({std::abs(b[0]), std::abs(b[1]), std::abs(b[2]),
+                                       std::abs(b[0] - n[0]), std::abs(b[1] - n[1]),
+                                       std::abs(b[2] - n[2])});
+        const size_t want = std::min<size_t>((size_t)K, cps.size());
+        for (int64_t r = 0; r <= rmax; ++r) {
+            // add the shell at Chebyshev distance r
+            for (int64_t dx = -r; dx <= r; ++dx)
+                for (int64_t dy = -r; dy <= r; ++dy)
+                    for (int64_t dz = -r; dz <= r; ++dz) {
+                        if (std::max({std::abs(dx), std::abs(dy), std::abs(dz)}) != r) continue;
+                        const int64_t x = b[0] + dx, y = b[1] + dy, z = b[2] + dz;
+                        if (x < 0 || y < 0 || z < 0 || x >= n[0] || y >= n[1] || z >= n[2]) continue;
+                        const size_t cell = (size_t)((x * n[1] + y) * n[2] + z);
+                        for (int64_t k = off[cell]; k < off[cell + 1]; ++k) {
+                            const int32_t c = cps[(size_t)k];
+                            const double* q = &V[3 * (size_t)c];
+                            const double d2 = (q[0] - p[0]) * (q[0] - p[0]) +
+                                              (q[1] - p[1]) * (q[1] - p[1]) +
+                                              (q[2] - p[2]) * (q[2] - p[2]);
+                            out.push_back({d2, c});
+                        }
+                    }
+            if (out.size() >= want) {
+                std::nth_element(out.begin(), out.begin() + (want - 1), out.end());
+                // every point within r*h of p lies in the searched cube
+                const double rh = (double)r * h;
+                if (out.size() == cps.size() || out[want - 1].first <= rh * rh) break;
+            }
+        }
+        const size_t keep = std::min(want, out.size());
+        std::partial_sort(out.begin(), out.begin() + keep, out.end());
+        out.resize(keep);
+        for (auto& e : out) e.first = std::sqrt(e.first);
+    }
+};
+}  // namespace
+
 uint32_t LithologyModel::setControlPointStride(int stride, double cellSize)
 {
     const size_t nv = vertexCount();
@@ -118,21 +228,22 @@ uint32_t LithologyModel::setControlPointStride(int stride, double cellSize)
         q[v] = {(int64_t)std::llround(m_vertices[3 * v] * 1e6),
                 (int64_t)std::llround(m_vertices[3 * v + 1] * 1e6),
                 (int64_t)std::llround(m_vertices[3 * v + 2] * 1e6)};
-    std::vector<int32_t> order(nv);
-    for (size_t v = 0; v < nv; ++v) order[v] = (int32_t)v;
-    std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+    auto lexLess = [&](int32_t a, int32_t b) {
         if (q[a].x != q[b].x) return q[a].x < q[b].x;
         if (q[a].y != q[b].y) return q[a].y < q[b].y;
         return q[a].z < q[b].z;
-    });
+    };
     if (cellSize <= 0.0) {
-        // lithology_model.cpp:916-934
-        double mn[3] = {1e12, 1e12, 1e12};
+        // lithology_model.cpp:916-934: the 300 lexicographically first positions
+        std::vector<int32_t> first(nv);
+        for (size_t v = 0; v < nv; ++v) first[v] = (int32_t)v;
         const size_t ns = std::min<size_t>(nv, 300);
+        std::partial_sort(first.begin(), first.begin() + ns, first.end(), lexLess);
+        double mn[3] = {1e12, 1e12, 1e12};
         for (size_t i = 0; i < ns; ++i)
             for (size_t j = i + 1; j < ns; ++j) {
-                const Q& a = q[order[i]];
-                const Q& b = q[order[j]];
+                const Q& a = q[first[i]];
+                const Q& b = q[first[j]];
                 const double d[3] = {std::abs((double)(a.x - b.x)) * 1e-6,
                                      std::abs((double)(a.y - b.y)) * 1e-6,
                                      std::abs((double)(a.z - b.z)) * 1e-6};
@@ -141,81 +252,124 @@ uint32_t LithologyModel::setControlPointStride(int stride, double cellSize)
             }
         cellSize = std::max({mn[0], mn[1], mn[2], 1.0});
     }
+    // lithology_model.cpp:952-989: one control point per populated cell of a stride*cell
+    // grid, the position closest to the cell centre (ties: the lexicographically first)
     const int64_t strideQ = std::max<int64_t>(1, (int64_t)stride * (int64_t)(cellSize * 1e6));
-    auto cell = [strideQ](int64_t v) { return v >= 0 ? v / strideQ : (v - strideQ + 1) / strideQ; };
-    struct Best { int32_t v; int64_t d2; };
-    std::map<std::tuple<int64_t, int64_t, int64_t>, Best> best;
-    for (int32_t v : order) {
-        const int64_t cx = cell(q[v].x), cy = cell(q[v].y), cz = cell(q[v].z);
+    auto cellOf = [strideQ](int64_t v) { return v >= 0 ? v / strideQ : (v - strideQ + 1) / strideQ; };
+    struct Cand { int64_t cx, cy, cz, d2; int32_t v; };
+    std::vector<Cand> cand(nv);
+    parallel_range((int64_t)nv, [&](int64_t i) {
+        const size_t v = (size_t)i;
+        const int64_t cx = cellOf(q[v].x), cy = cellOf(q[v].y), cz = cellOf(q[v].z);
         const int64_t ox = cx * strideQ + strideQ / 2, oy = cy * strideQ + strideQ / 2,
                       oz = cz * strideQ + strideQ / 2;
-        const double dx = (double)(q[v].x - ox), dy = (double)(q[v].y - oy), dz = (double)(q[v].z - oz);
-        const int64_t d2 = (int64_t)(dx * dx + dy * dy + dz * dz);
-        auto key = std::make_tuple(cx, cy, cz);
-        auto it = best.find(key);
-        if (it == best.end() || d2 < it->second.d2) best[key] = {v, d2};
-    }
-    std::vector<int32_t> cps;
+        const int64_t d2 = (q[v].x - ox) * (q[v].x - ox) + (q[v].y - oy) * (q[v].y - oy) +
+                           (q[v].z - oz) * (q[v].z - oz);
+        cand[v] = {cx, cy, cz, d2, (int32_t)v};
+    });
+    std::sort(cand.begin(), cand.end(), [&](const Cand& a, const Cand& b) {
+        if (a.cx != b.cx) return a.cx < b.cx;
+        if (a.cy != b.cy) return a.cy < b.cy;
+        if (a.cz != b.cz) return a.cz < b.cz;
+        if (a.d2 != b.d2) return a.d2 < b.d2;
+        return lexLess(a.v, b.v);
+    });
     m_isCP.assign(nv, 0);
-    for (auto& kv : best) {
-        cps.push_back(kv.second.v);
-        m_isCP[kv.second.v] = 1;
-    }
-    // K = 4 nearest control points per non-control vertex, inverse-distance weights
-    // (surface_mesh.cpp:566-598); buckets of one grid cell keep the search local.
-    const double bucket = std::max(1.0, stride * cellSize);
-    std::map<std::array<int64_t, 3>, std::vector<int32_t>> buckets;
-    for (size_t i = 0; i < cps.size(); ++i) {
-        const double* p = &m_vertices[3 * (size_t)cps[i]];
-        std::array<int64_t, 3> c = {(int64_t)std::floor(p[0] / bucket),
-                                    (int64_t)std::floor(p[1] / bucket),
-                                    (int64_t)std::floor(p[2] / bucket)};
-        buckets[c].push_back(cps[i]);
-    }
-    m_wOff.assign(nv + 1, 0);
-    const int K = 4;
-    std::vector<std::pair<double, int32_t>> cand;
-    for (size_t v = 0; v < nv; ++v) {
-        if (m_isCP[v]) {
-            m_wOff[v + 1] = m_wOff[v];
-            continue;
-        }
-        const double* p = &m_vertices[3 * v];
-        const std::array<int64_t, 3> c = {(int64_t)std::floor(p[0] / bucket),
-                                          (int64_t)std::floor(p[1] / bucket),
-                                          (int64_t)std::floor(p[2] / bucket)};
-        cand.clear();
-        for (int r = 1; r < 64; ++r) {
-            cand.clear();
-            for (int64_t dx = -r; dx <= r; ++dx)
-                for (int64_t dy = -r; dy <= r; ++dy)
-                    for (int64_t dz = -r; dz <= r; ++dz) {
-                        auto it = buckets.find({c[0] + dx, c[1] + dy, c[2] + dz});
-                        if (it == buckets.end()) continue;
-                        for (int32_t cv : it->second) {
-                            const double* pc = &m_vertices[3 * (size_t)cv];
-                            const double d = std::sqrt((pc[0] - p[0]) * (pc[0] - p[0]) +
-                                                       (pc[1] - p[1]) * (pc[1] - p[1]) +
-                                                       (pc[2] - p[2]) * (pc[2] - p[2]));
-                            cand.push_back({d, cv});
-                        }
-                    }
-            // the K nearest are exact once the K-th distance is within the searched shell
-            if ((int)cand.size() >= K) {
-                std::partial_sort(cand.begin(), cand.begin() + K, cand.end());
-                if (cand[K - 1].first <= r * bucket) break;
+    for (size_t i = 0; i < nv; ++i)
+        if (i == 0 || cand[i].cx != cand[i - 1].cx || cand[i].cy != cand[i - 1].cy ||
+            cand[i].cz != cand[i - 1].cz)
+            m_isCP[(size_t)cand[i].v] = 1;
+    std::vector<Cand>().swap(cand);
+
+    // SurfaceMesh::setControlPointsExplicit (surface_mesh.cpp:544-601) per unit surface: a
+    // vertex follows the K = 4 nearest control points on that surface with inverse-distance
+    // weights; a vertex on several surfaces takes the mean of their interpolations, as the
+    // reference's shared-vertex averaging does (lithology_model.cpp:230-247).  A surface
+    // with fewer than two control points has none (the reference then gives each of its
+    // vertices a parameter): its vertices become control points here.
+    const int nu = groupCount();
+    std::vector<std::vector<int32_t>> unitVerts(nu);
+    {
+        std::vector<uint8_t> on((size_t)nu * nv, 0);  // on[u * nv + v]: v on u's surface
+        for (size_t f = 0; f < faceCount(); ++f)
+            for (int side = 0; side < 2; ++side) {
+                const int u = side == 0 ? m_front[f] : m_back[f];
+                if (u < 0) continue;
+                for (int k = 0; k < 3; ++k) on[(size_t)u * nv + m_faces[3 * f + k]] = 1;
             }
-        }
-        const size_t keep = std::min<size_t>(K, cand.size());
-        std::partial_sort(cand.begin(), cand.begin() + keep, cand.end());
-        double tot = 0.0;
-        for (size_t j = 0; j < keep; ++j) tot += 1.0 / std::max(cand[j].first, 1e-12);
-        for (size_t j = 0; j < keep; ++j) {
-            m_wCp.push_back(cand[j].second);
-            m_w.push_back((1.0 / std::max(cand[j].first, 1e-12)) / tot);
-        }
-        m_wOff[v + 1] = m_wOff[v] + (int64_t)keep;
+        for (int u = 0; u < nu; ++u)
+            for (size_t v = 0; v < nv; ++v)
+                if (on[(size_t)u * nv + v]) unitVerts[u].push_back((int32_t)v);
     }
+    std::vector<std::vector<int32_t>> unitCps(nu);
+    for (int u = 0; u < nu; ++u) {
+        for (int32_t v : unitVerts[u])
+            if (m_isCP[(size_t)v]) unitCps[u].push_back(v);
+        if (unitCps[u].size() < 2)
+            for (int32_t v : unitVerts[u]) m_isCP[(size_t)v] = 1;
+    }
+    for (int u = 0; u < nu; ++u) {
+        unitCps[u].clear();
+        for (int32_t v : unitVerts[u])
+            if (m_isCP[(size_t)v]) unitCps[u].push_back(v);
+    }
+    const double bucket = std::max(1.0, stride * cellSize);
+    std::vector<CpBuckets> grids(nu);
+    for (int u = 0; u < nu; ++u) grids[u].build(unitCps[u], m_vertices, bucket);
+    // units of each vertex (CSR)
+    std::vector<int64_t> vuOff(nv + 1, 0);
+    for (int u = 0; u < nu; ++u)
+        for (int32_t v : unitVerts[u]) ++vuOff[(size_t)v + 1];
+    for (size_t v = 0; v < nv; ++v) vuOff[v + 1] += vuOff[v];
+    std::vector<int32_t> vuList(vuOff[nv]);
+    {
+        std::vector<int64_t> fill(vuOff.begin(), vuOff.end() - 1);
+        for (int u = 0; u < nu; ++u)
+            for (int32_t v : unitVerts[u]) vuList[(size_t)fill[(size_t)v]++] = u;
+    }
+    const int K = 4;
+    std::vector<int64_t> slot(nv + 1, 0);
+    for (size_t v = 0; v < nv; ++v) slot[v + 1] = slot[v] + (m_isCP[v] ? 0 : K * (vuOff[v + 1] - vuOff[v]));
+    std::vector<int32_t> wc(slot[nv], -1);
+    std::vector<double> ww(slot[nv], 0.0);
+    std::vector<int32_t> cnt(nv, 0);
+    parallel_range((int64_t)nv, [&](int64_t i) {
+        const size_t v = (size_t)i;
+        if (m_isCP[v]) return;
+        const int64_t nUnits = vuOff[v + 1] - vuOff[v];
+        if (nUnits == 0) return;
+        thread_local std::vector<std::pair<double, int32_t>> nb;
+        thread_local std::vector<std::pair<int32_t, double>> acc;
+        acc.clear();
+        for (int64_t k = vuOff[v]; k < vuOff[v + 1]; ++k) {
+            grids[vuList[(size_t)k]].nearest(&m_vertices[3 * v], m_vertices, K, nb);
+            double tot = 0.0;
+            for (auto& e : nb) tot += 1.0 / std::max(e.first, 1e-12);
+            for (auto& e : nb)
+                acc.push_back({e.second, (1.0 / std::max(e.first, 1e-12)) / tot / (double)nUnits});
+        }
+        std::sort(acc.begin(), acc.end());
+        int32_t m = 0;
+        for (size_t j = 0; j < acc.size(); ++j) {
+            if (m > 0 && wc[(size_t)slot[v] + m - 1] == acc[j].first) {
+                ww[(size_t)slot[v] + m - 1] += acc[j].second;
+                continue;
+            }
+            wc[(size_t)slot[v] + m] = acc[j].first;
+            ww[(size_t)slot[v] + m] = acc[j].second;
+            ++m;
+        }
+        cnt[v] = m;
+    });
+    m_wOff.assign(nv + 1, 0);
+    for (size_t v = 0; v < nv; ++v) m_wOff[v + 1] = m_wOff[v] + cnt[v];
+    m_wCp.resize((size_t)m_wOff[nv]);
+    m_w.resize((size_t)m_wOff[nv]);
+    for (size_t v = 0; v < nv; ++v)
+        for (int32_t j = 0; j < cnt[v]; ++j) {
+            m_wCp[(size_t)m_wOff[v] + j] = wc[(size_t)slot[v] + j];
+            m_w[(size_t)m_wOff[v] + j] = ww[(size_t)slot[v] + j];
+        }
     rebuildDofs();
     return totalDofCount();
 }

@@ -176,3 +176,75 @@ def test_surface_smoothness(stride):
         # gradient sums to zero
         assert M.smoothness(p + 3.0, w)[0] == pytest.approx(val, rel=1e-9)
         assert abs(g.sum()) < 1e-9 * np.abs(g).sum()
+
+
+def reference_cp_weights(m, stride, cell, K=4):
+    """Restatement of the reference's control points on the per-unit layout:
+    LithologyModel::setControlPointStride (lithology_model.cpp:864-1040) picks, per cell of
+    a stride*cell grid over 1 um-quantized positions, the position nearest the cell centre
+    (first in lexicographic order on ties); SurfaceMesh::setControlPointsExplicit
+    (surface_mesh.cpp:544-601) makes each unit's copy of a non-control vertex follow the K
+    nearest control points of that unit's surface with inverse-distance weights
+    (all of a surface's vertices are parameters when it has fewer than two control
+    points); applyParameterVector (lithology_model.cpp:230-247) averages the copies of a
+    shared vertex.  Returns the control-point mask and the dense weight matrix W
+    (displacement of vertex v = sum_c W[v, c] * displacement of control point c)."""
+    V = m.vertices
+    nv = len(V)
+    q = np.round(V * 1e6).astype(np.int64)
+    sq = stride * int(cell * 1e6)
+    c = q // sq
+    ctr = c * sq + sq // 2
+    d2 = ((q - ctr) ** 2).sum(1)
+    best = {}
+    for v in sorted(range(nv), key=lambda i: tuple(q[i])):
+        k = tuple(c[v])
+        if k not in best or d2[v] < d2[best[k]]:
+            best[k] = v
+    cp = np.zeros(nv, bool)
+    cp[list(best.values())] = True
+    units = [np.unique(m.faces[(m.front == u) | (m.back == u)]) for u in range(m.n_units)]
+    for vs in units:
+        if cp[vs].sum() < 2:
+            cp[vs] = True
+    W = np.zeros((nv, nv))
+    nunits = np.zeros(nv)
+    for vs in units:
+        nunits[vs] += 1
+    for vs in units:
+        cps = vs[cp[vs]]
+        for v in vs:
+            if cp[v]:
+                continue
+            d = np.linalg.norm(V[cps] - V[v], axis=1)
+            near = np.argsort(d, kind="stable")[:K]
+            w = 1.0 / np.maximum(d[near], 1e-12)
+            W[v, cps[near]] += w / w.sum() / nunits[v]
+    W[cp, cp] = 1.0
+    return cp, W
+
+
+@pytest.mark.parametrize("stride,cell", [(2, 25.0), (3, 25.0), (2, 0.0)])
+def test_control_points_against_reference_restatement(stride, cell):
+    """Control-point set and interpolation weights equal the restatement of the
+    reference's rule (above); cell 0 estimates the cell size from the positions like the
+    reference (25 m here)."""
+    m = small_model()
+    M = inversion.Model(m)
+    M.fix_exterior(inversion.Z_ONLY)
+    M.set_axis_scale(10.0)
+    M.set_cp_stride(stride, cell)
+    cp_ref, W = reference_cp_weights(m, stride, cell or 25.0)
+    _, _, cp = M.dofs()
+    np.testing.assert_array_equal(cp, cp_ref)
+    # displacements of a random parameter step against W applied to the control points'
+    # displacements, for every vertex whose z is free
+    p0 = M.params()
+    dp = np.random.default_rng(stride).normal(size=len(p0))
+    dz = (M.apply(p0 + dp) - M.apply(p0))[:, 2]
+    M.apply(p0)
+    free = np.ones(m.n_vertices, bool)
+    free[m.faces[m.back < 0].ravel()] = False  # FIXED exterior
+    expect = W @ np.where(cp, dz, 0.0)
+    np.testing.assert_allclose(dz[free], expect[free], rtol=0, atol=1e-9)
+    assert np.abs(dz[~free]).max() == 0.0
