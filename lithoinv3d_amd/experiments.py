@@ -14,7 +14,8 @@
                           run's final misfit, and reports iterations and seconds to it.
 
   * ``large_inversion``   a few FP32C iterations of the joint XYZ_FREE inversion at the
-                          large workload, resident on one GPU.
+                          large workload with 50 boreholes' depth-to-contact picks and the
+                          learned H0, resident on one GPU.
 
 CLI:  python -m lithoinv3d_amd.experiments learned-h0 --train 200 --test 20 --out FILE
       python -m lithoinv3d_amd.experiments large --max-iter 3
@@ -222,17 +223,87 @@ def compare_solvers(prob, net=None, norm=None, max_iter=40, exact_diag=False,
     return out
 
 
+def borehole_picks(m, M, ptrue, n=50, seed=5, sigma=5.0, step=5.0, margin=100.0):
+    """Depth-to-contact picks of n seeded vertical boreholes at the true geometry: along
+    each vertical line the unit is sampled every `step` metres (point classification);
+    wherever it changes between two units, the elevation of their contact on that line
+    (the depth-to-contact term's own intersection at ptrue) becomes a pick with the given
+    standard deviation.  Returns litho_io.ContactPicks."""
+    from .litho_io import ContactPicks
+
+    rng = np.random.default_rng(seed)
+    V = m.vertices
+    lo, hi = V.min(0), V.max(0)
+    xy = rng.uniform(lo[:2] + margin, hi[:2] - margin, (n, 2))
+    z = np.arange(hi[2] - step / 2, lo[2], -step)
+    M.apply(ptrue)
+    pts = np.concatenate([np.column_stack([np.full(len(z), x), np.full(len(z), y), z])
+                          for x, y in xy])
+    lab = M.classify(pts).reshape(n, len(z))
+    cand, above, below = [], [], []
+    for b in range(n):
+        for k in range(len(z) - 1):
+            u, w = lab[b, k], lab[b, k + 1]
+            if u >= 0 and w >= 0 and u != w:
+                cand.append((xy[b, 0], xy[b, 1], 0.5 * (z[k] + z[k + 1])))
+                above.append(u)
+                below.append(w)
+    picks = ContactPicks(np.array(cand, np.float64).reshape(-1, 3), np.array(above, np.int32),
+                         np.array(below, np.int32), np.full(len(cand), float(sigma)))
+    # the contact's elevation on the line at ptrue, from the term itself
+    _, _, zc = M.boreholes(picks, ptrue, weight=1.0, radius=2 * step)
+    ok = np.isfinite(zc)
+    picks = ContactPicks(np.column_stack([picks.xyz[ok, :2], zc[ok]]), picks.above[ok],
+                         picks.below[ok], picks.sigma[ok])
+    return picks
+
+
+def learned_log_diag_xyz(m, M, stations, net, norm, field_dir, field_nT):
+    """Learned log Gauss-Newton diagonal for every parameter of M (any axes): per axis,
+    the subset estimate for the parameters moving that axis plus the network's
+    correction from features along that axis."""
+    v, ax, _ = M.dofs()
+    off, vert, w = M.dof_map()
+    out = np.zeros(len(v))
+    va = mlp.vertex_areas_normals(m, field_dir)
+    for a in range(3):
+        sel = np.nonzero(ax == a)[0]
+        if len(sel) == 0:
+            continue
+        cnt = off[sel + 1] - off[sel]
+        so = np.zeros(len(sel) + 1, np.int64)
+        so[1:] = np.cumsum(cnt)
+        idx = np.concatenate([np.arange(off[j], off[j + 1]) for j in sel])
+        dm = (so, vert[idx], w[idx])
+        sub = mlp.subset_gn_diagonal(m, stations, dm, field_dir, field_nT, SIGMA_GZ,
+                                     SIGMA_TMI, axis=a)
+        X = mlp.dof_features(m, dm, v[sel], stations, AXIS_SCALE, sub, field_dir, axis=a, va=va)
+        out[sel] = sub.log_diag() + norm.t_inv(net.forward(norm.x(X))).astype(np.float64)
+    return out
+
+
 def large_inversion(name="contact256x256x96_st512", stride=8, max_iter=3, ref_weight=1e-2,
-                    seed=5):
+                    seed=5, n_boreholes=50, h0="learned", train_models=40):
     """Joint gz + TMI inversion at the large workload, everything resident on one GPU:
     XYZ_FREE control points (every ``stride``-th cell), FP32C forward and adjoint, the
-    reference-model term, synthetic data from a perturbed interface.  The first step is
-    scaled so the largest parameter moves one unit (the unscaled -g step would cost a
-    long series of halvings, each a full forward).  Reports setup, per-iteration times,
-    misfit and device memory in use."""
+    reference-model term, n_boreholes seeded vertical boreholes with depth-to-contact
+    picks (5 m), synthetic data from a perturbed interface.  h0 = "learned": the
+    diagonal H0 from the network trained on `train_models` small contact models (the
+    training time is reported); "scaled": the identity.  The first step is scaled so the
+    largest parameter moves one unit (an unscaled step would cost a long series of
+    halvings, each a full forward); the diagonal's shape is kept.  Reports setup,
+    per-iteration times, misfit and device memory in use."""
     import torch
 
-    out = {"workload": name}
+    out = {"workload": name, "h0": h0}
+    net = norm = None
+    if h0 == "learned":
+        t0 = time.perf_counter()
+        X, t = training_set(range(1000, 1000 + train_models), (32, 32, 16), 64)
+        net, norm, losses = train_mlp(X, t, steps=2000)
+        out["h0_training_s"] = time.perf_counter() - t0
+        out["h0_training_samples"] = int(len(t))
+        log(f"large: H0 network trained on {len(t)} parameters in {out['h0_training_s']:.1f}s")
     t0 = time.perf_counter()
     cfg = workload.build(name)
     m, st = cfg["model"], cfg["stations"]
@@ -256,6 +327,16 @@ def large_inversion(name="contact256x256x96_st512", stride=8, max_iter=3, ref_we
     out["setup_s"] = time.perf_counter() - t0
     log(f"large: {m.n_vertices} vertices, {m.n_faces} facets, {len(st)} stations, {n} "
         f"parameters; setup {out['setup_s']:.1f}s")
+    picks = None
+    if n_boreholes > 0:
+        t0 = time.perf_counter()
+        picks = borehole_picks(m, M, ptrue, n_boreholes, seed)
+        M.apply(p0)
+        out["boreholes"] = n_boreholes
+        out["borehole_picks"] = int(len(picks.xyz))
+        out["borehole_setup_s"] = time.perf_counter() - t0
+        log(f"large: {len(picks.xyz)} contact picks from {n_boreholes} boreholes "
+            f"({out['borehole_setup_s']:.1f}s)")
     inc, dec, fld = cfg["inc"], cfg["dec"], cfg["field"]
     P = inversion.Problem(M, st, np.zeros(len(st)), tmi_obs=np.zeros(len(st)), inc=inc,
                           dec=dec, field=fld, precision="fp32c")
@@ -263,26 +344,38 @@ def large_inversion(name="contact256x256x96_st512", stride=8, max_iter=3, ref_we
     gz, tm = P.forward(ptrue)
     out["forward_s"] = time.perf_counter() - t1
     P.close()
+    M.apply(p0)
     rng = np.random.default_rng(seed + 1)
     gz = gz + rng.normal(0, SIGMA_GZ, len(gz))
     tm = tm + rng.normal(0, SIGMA_TMI, len(tm))
     log(f"large: synthetic data in {out['forward_s']:.1f}s")
     P = inversion.Problem(M, st, gz, np.full(len(st), SIGMA_GZ), tm, np.full(len(st), SIGMA_TMI),
                           inc=inc, dec=dec, field=fld, alpha_mag=1.0, precision="fp32c",
-                          ref_weight=ref_weight)
+                          ref_weight=ref_weight, boreholes=picks, borehole_weight=1.0)
     t2 = time.perf_counter()
     g0 = P.gradient(p0)
     out["first_gradient_s"] = time.perf_counter() - t2
     free, total = torch.cuda.mem_get_info(0)
     out["device_memory_in_use_gb"] = (total - free) / 1e9
-    h0 = np.full(n, 1.0 / max(float(np.abs(g0).max()), 1e-300))
+    D = np.ones(n)
+    if net is not None:
+        t3 = time.perf_counter()
+        fh = synth.field_unit_vector(inc, dec)
+        D = np.exp(-H0_POWER * learned_log_diag_xyz(m, M, st, net, norm, fh, fld))
+        out["h0_seconds"] = time.perf_counter() - t3
+        net.close()
+        log(f"large: learned H0 in {out['h0_seconds']:.1f}s")
+    h0 = D / max(float(np.abs(D * g0).max()), 1e-300)
     log(f"large: first gradient {out['first_gradient_s']:.1f}s; "
         f"{out['device_memory_in_use_gb']:.1f} GB in use")
     r = P.minimize(p0, lo, hi, max_iter=max_iter, h0=h0)
+    comp = P.evaluate(r.x)
     P.close()
     h = r.history
     out["history"] = [dict(iteration=int(row[0]), f=float(row[1]), seconds=float(row[4]),
                            n_obj=int(row[5]), n_grad=int(row[6])) for row in h]
+    out["final_components"] = dict(gravity=float(comp[0]), magnetic=float(comp[1]),
+                                   regularization=float(comp[2]), total=float(comp[3]))
     out["message"] = r.message
     return out
 
@@ -310,10 +403,15 @@ def main(argv=None):
     c.add_argument("--workload", default="contact256x256x96_st512")
     c.add_argument("--stride", type=int, default=8)
     c.add_argument("--max-iter", type=int, default=3)
+    c.add_argument("--boreholes", type=int, default=50)
+    c.add_argument("--h0", choices=("learned", "scaled"), default="learned")
+    c.add_argument("--train-models", type=int, default=40)
     c.add_argument("--out", default=None)
     args = ap.parse_args(argv)
     if args.cmd == "large":
-        res = large_inversion(args.workload, args.stride, args.max_iter)
+        res = large_inversion(args.workload, args.stride, args.max_iter,
+                              n_boreholes=args.boreholes, h0=args.h0,
+                              train_models=args.train_models)
         if args.out:
             json.dump(res, open(args.out, "w"), indent=1)
         print(json.dumps(res))

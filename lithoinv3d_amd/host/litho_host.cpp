@@ -10,6 +10,7 @@
 #include <limits>
 #include <map>
 #include <random>
+#include <set>
 #include <stdexcept>
 #include <unordered_map>
 
@@ -855,24 +856,70 @@ BoreholeContactTerm::BoreholeContactTerm(std::shared_ptr<LithologyModel> model,
     const auto& fr = m_model->front();
     const auto& bk = m_model->back();
     m_cand.resize(m_picks.size());
+    for (const ContactPick& p : m_picks)
+        if (!(p.sigma > 0.0)) throw std::invalid_argument("BoreholeContactTerm: sigma must be > 0");
+    const size_t nf = fr.size();
+    if (nf == 0 || m_picks.empty()) return;
+    // the facets of any picked unit pair, binned in (x, y) by their bounding boxes; a
+    // pick's candidates are the facets of its pair whose box grown by the search radius
+    // holds the pick (the bins only narrow the search)
+    std::set<std::pair<int, int>> pairs;
+    for (const ContactPick& p : m_picks) pairs.insert({std::min(p.above, p.below), std::max(p.above, p.below)});
+    std::vector<int64_t> sel;
+    std::vector<double> box;  // lo x, lo y, hi x, hi y per selected facet
+    double ext = 0.0, x0 = 1e300, y0 = 1e300, x1 = -1e300, y1 = -1e300;
+    for (size_t f = 0; f < nf; ++f) {
+        if (!pairs.count({std::min(fr[f], bk[f]), std::max(fr[f], bk[f])})) continue;
+        double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300};
+        for (int c = 0; c < 3; ++c)
+            for (int a = 0; a < 2; ++a) {
+                const double v = X[3 * (size_t)F[3 * f + c] + a];
+                lo[a] = std::min(lo[a], v);
+                hi[a] = std::max(hi[a], v);
+            }
+        sel.push_back((int64_t)f);
+        box.insert(box.end(), {lo[0], lo[1], hi[0], hi[1]});
+        ext += (hi[0] - lo[0]) + (hi[1] - lo[1]);
+        x0 = std::min(x0, lo[0]);
+        y0 = std::min(y0, lo[1]);
+        x1 = std::max(x1, hi[0]);
+        y1 = std::max(y1, hi[1]);
+    }
+    if (sel.empty()) return;
+    const double h = std::max({ext / (double)sel.size(), (x1 - x0) / 2048.0, (y1 - y0) / 2048.0, 1e-6});
+    const int64_t nx = (int64_t)std::floor((x1 - x0) / h) + 1, ny = (int64_t)std::floor((y1 - y0) / h) + 1;
+    auto bx = [&](double x) { return std::clamp((int64_t)std::floor((x - x0) / h), (int64_t)0, nx - 1); };
+    auto by = [&](double y) { return std::clamp((int64_t)std::floor((y - y0) / h), (int64_t)0, ny - 1); };
+    std::vector<int64_t> off((size_t)(nx * ny) + 1, 0);
+    for (size_t i = 0; i < sel.size(); ++i)
+        for (int64_t ix = bx(box[4 * i]); ix <= bx(box[4 * i + 2]); ++ix)
+            for (int64_t iy = by(box[4 * i + 1]); iy <= by(box[4 * i + 3]); ++iy) ++off[(size_t)(ix * ny + iy) + 1];
+    for (size_t i = 1; i < off.size(); ++i) off[i] += off[i - 1];
+    std::vector<int32_t> ent((size_t)off.back());
+    {
+        std::vector<int64_t> fill(off.begin(), off.end() - 1);
+        for (size_t i = 0; i < sel.size(); ++i)
+            for (int64_t ix = bx(box[4 * i]); ix <= bx(box[4 * i + 2]); ++ix)
+                for (int64_t iy = by(box[4 * i + 1]); iy <= by(box[4 * i + 3]); ++iy)
+                    ent[(size_t)fill[(size_t)(ix * ny + iy)]++] = (int32_t)i;
+    }
     for (size_t k = 0; k < m_picks.size(); ++k) {
         const ContactPick& p = m_picks[k];
-        if (!(p.sigma > 0.0)) throw std::invalid_argument("BoreholeContactTerm: sigma must be > 0");
-        for (size_t f = 0; f < fr.size(); ++f) {
-            const bool pair = (fr[f] == p.above && bk[f] == p.below) ||
-                              (fr[f] == p.below && bk[f] == p.above);
-            if (!pair) continue;
-            double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300};
-            for (int c = 0; c < 3; ++c)
-                for (int a = 0; a < 2; ++a) {
-                    const double v = X[3 * (size_t)F[3 * f + c] + a];
-                    lo[a] = std::min(lo[a], v);
-                    hi[a] = std::max(hi[a], v);
+        const int ua = std::min(p.above, p.below), ub = std::max(p.above, p.below);
+        auto& c = m_cand[k];
+        for (int64_t ix = bx(p.x - searchRadius); ix <= bx(p.x + searchRadius); ++ix)
+            for (int64_t iy = by(p.y - searchRadius); iy <= by(p.y + searchRadius); ++iy)
+                for (int64_t e = off[(size_t)(ix * ny + iy)]; e < off[(size_t)(ix * ny + iy) + 1]; ++e) {
+                    const size_t i = (size_t)ent[(size_t)e];
+                    const int64_t f = sel[i];
+                    if (std::min(fr[f], bk[f]) != ua || std::max(fr[f], bk[f]) != ub) continue;
+                    if (p.x >= box[4 * i] - searchRadius && p.x <= box[4 * i + 2] + searchRadius &&
+                        p.y >= box[4 * i + 1] - searchRadius && p.y <= box[4 * i + 3] + searchRadius)
+                        c.push_back(f);
                 }
-            if (p.x >= lo[0] - searchRadius && p.x <= hi[0] + searchRadius &&
-                p.y >= lo[1] - searchRadius && p.y <= hi[1] + searchRadius)
-                m_cand[k].push_back((int64_t)f);
-        }
+        // in facet order, once each (a facet spans several bins)
+        std::sort(c.begin(), c.end());
+        c.erase(std::unique(c.begin(), c.end()), c.end());
     }
 }
 

@@ -86,13 +86,27 @@ PointClassifier::PointClassifier(const LithologyModel& model, const LabelGridDat
         z0 = std::min(z0, V[3 * v + 2]);
         z1 = std::max(z1, V[3 * v + 2]);
     }
-    // about four facets per bin on average
-    const int side = (int)std::clamp(std::sqrt((double)nf / 4.0), 1.0, 4096.0);
-    m_ny = m_nz = side;
+    // bins about twice the mean facet extent, so a facet covers a few bins (finer bins
+    // multiply the entries of large facets; coarser ones lengthen every ray's list)
+    double ey = 0.0, ez = 0.0;
+    for (size_t f = 0; f < nf; ++f) {
+        double a = 1e300, b = -1e300, c = 1e300, d = -1e300;
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t v = F[3 * f + k];
+            a = std::min(a, V[3 * v + 1]);
+            b = std::max(b, V[3 * v + 1]);
+            c = std::min(c, V[3 * v + 2]);
+            d = std::max(d, V[3 * v + 2]);
+        }
+        ey += b - a;
+        ez += d - c;
+    }
     m_y0 = y0;
     m_z0 = z0;
-    m_hy = std::max((y1 - y0) / side, 1e-9);
-    m_hz = std::max((z1 - z0) / side, 1e-9);
+    m_hy = std::max({2.0 * ey / (double)nf, (y1 - y0) / 4096.0, 1e-9});
+    m_hz = std::max({2.0 * ez / (double)nf, (z1 - z0) / 4096.0, 1e-9});
+    m_ny = (int)std::min<double>(4096.0, std::floor((y1 - y0) / m_hy) + 1.0);
+    m_nz = (int)std::min<double>(4096.0, std::floor((z1 - z0) / m_hz) + 1.0);
     auto cellRange = [&](size_t f, int& ya, int& yb, int& za, int& zb) {
         double lo_y = 1e300, hi_y = -1e300, lo_z = 1e300, hi_z = -1e300;
         for (int k = 0; k < 3; ++k) {

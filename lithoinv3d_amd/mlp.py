@@ -29,10 +29,11 @@ from . import lithofwd
 
 N_FEATURES = 16
 FEATURE_NAMES = (
-    "depth_km", "log_area", "log_chi_sum_area", "log_chi_sum_area_fpar", "n_z",
+    "depth_km", "log_area", "log_chi_sum_area", "log_chi_sum_area_fpar", "n_axis",
     "log_stations_within_2_depths", "abs_drho", "log_abs_dchi", "log_subset_colnorm_gz",
     "log_subset_colnorm_tmi", "log_subset_spacing_over_depth", "log_area_over_depth2",
-    "log_chi_sum_area_nz", "abs_drho_nz", "abs_dchi_nz_scaled", "edge_distance_over_depth",
+    "log_chi_sum_area_naxis", "abs_drho_naxis", "abs_dchi_naxis_scaled",
+    "edge_distance_over_depth",
 )
 SUBSET_STRIDE = 32
 
@@ -138,7 +139,8 @@ class MLP:
 def vertex_areas_normals(m, field_dir=(0.0, 0.0, 1.0)):
     """Per-vertex sums over the incident facets of one third of each facet's area A_f, of
     A_f n_f, A_f |drho_f| and A_f |dchi_f| (contrasts across the facet), and of
-    A_f (chi_front + chi_back) weighted by 1, by 1 - (F.n_f)^2 and by |n_f,z|.  The sum
+    A_f (chi_front + chi_back) weighted by 1, by 1 - (F.n_f)^2 and by |n_f,x|, |n_f,y|,
+    |n_f,z|.  The sum
     of susceptibilities follows the reference's TMI edge coefficient, which adds the
     magnetizations on both sides of a contact instead of differencing them."""
     V, F = m.vertices, m.faces.astype(np.int64)
@@ -155,14 +157,16 @@ def vertex_areas_normals(m, field_dir=(0.0, 0.0, 1.0)):
     fpar = 1.0 - (nrm @ np.asarray(field_dir, np.float64)) ** 2
     a3 = area / 3
     cols = np.stack([a3, nrm[:, 0] * a3, nrm[:, 1] * a3, nrm[:, 2] * a3, drho * a3,
-                     dchi * a3, csum * a3, csum * fpar * a3, csum * np.abs(nrm[:, 2]) * a3], 1)
-    nv = m.n_vertices
-    acc = np.zeros((nv, cols.shape[1]))
-    for k in range(3):
-        for j in range(cols.shape[1]):
-            acc[:, j] += np.bincount(F[:, k], weights=cols[:, j], minlength=nv)
+                     dchi * a3, csum * a3, csum * fpar * a3, csum * np.abs(nrm[:, 0]) * a3,
+                     csum * np.abs(nrm[:, 1]) * a3, csum * np.abs(nrm[:, 2]) * a3], 1)
+    from scipy.sparse import csr_matrix
+
+    nv, nf = m.n_vertices, len(F)
+    # vertex x facet incidence (three corners per facet) times the facet columns
+    inc = csr_matrix((np.ones(3 * nf), (F.T.ravel(), np.tile(np.arange(nf), 3))), shape=(nv, nf))
+    acc = np.asarray(inc @ cols)
     return dict(A=acc[:, 0], N=acc[:, 1:4], R=acc[:, 4], C=acc[:, 5], S=acc[:, 6],
-                S_par=acc[:, 7], S_z=acc[:, 8])
+                S_par=acc[:, 7], S_n=acc[:, 8:11])
 
 
 def station_subset(stations, stride=SUBSET_STRIDE):
@@ -216,14 +220,18 @@ def subset_gn_diagonal(m, stations, dof_map, field_dir, field_nT=50000.0, sigma_
 
 
 def dof_features(m, dof_map, dof_vertex, stations, axis_scale, sub: SubsetDiagonal,
-                 field_dir=(0.0, 0.0, 1.0)):
-    """(n_dof, 16) features per control-point parameter (FEATURE_NAMES), from the
-    parameter's vertices and their interpolation weights (dof_map), the stations, the
-    inducing-field direction and the subset Gauss-Newton diagonal."""
+                 field_dir=(0.0, 0.0, 1.0), axis=2, va=None):
+    """(n_dof, 16) features per control-point parameter (FEATURE_NAMES) moving its
+    vertices along `axis`, from the parameter's vertices and their interpolation weights
+    (dof_map), the stations, the inducing-field direction and the subset Gauss-Newton
+    diagonal.  The normal-component features take the component along the parameter's
+    axis (n_z for the depth parameters the network is trained on).  va: the model's
+    vertex_areas_normals, when already computed."""
     from scipy.spatial import cKDTree
 
     off, vert, w = dof_map
-    va = vertex_areas_normals(m, field_dir)
+    if va is None:
+        va = vertex_areas_normals(m, field_dir)
     n = len(off) - 1
     ww = w / axis_scale
     seg = np.repeat(np.arange(n), np.diff(off))
@@ -255,7 +263,7 @@ def dof_features(m, dof_map, dof_vertex, stations, axis_scale, sub: SubsetDiagon
         np.log1p(Aeff),
         np.log(np.abs(dofsum(va["S"])) + tiny),
         np.log(np.abs(dofsum(va["S_par"])) + tiny),
-        nn[:, 2],
+        nn[:, axis],
         np.log1p(cnt),
         Reff,
         np.log10(np.maximum(Ceff, 1e-6)),
@@ -263,9 +271,9 @@ def dof_features(m, dof_map, dof_vertex, stations, axis_scale, sub: SubsetDiagon
         np.log(np.maximum(sub.tmi, 1e-300)),
         np.full(n, np.log(sub.spacing)) - np.log(depth),
         np.log(np.maximum(Aeff, 1e-6) / depth ** 2),
-        np.log(np.abs(dofsum(va["S_z"])) + tiny),
-        Reff * np.abs(nn[:, 2]),
-        Ceff * np.abs(nn[:, 2]) * 100.0,
+        np.log(np.abs(dofsum(va["S_n"][:, axis])) + tiny),
+        Reff * np.abs(nn[:, axis]),
+        Ceff * np.abs(nn[:, axis]) * 100.0,
         edge / depth,
     ], axis=1)
     return X
