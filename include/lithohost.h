@@ -35,6 +35,14 @@ int li_model_properties(void* h, double* density, double* susceptibility);
  * the rest `interior_freedom` (VertexFreedom of surface_mesh.h: 0 FIXED, 1 Z_ONLY,
  * 2 X_ONLY, 3 Y_ONLY, 8 XYZ_FREE). */
 int li_model_fix_exterior(void* h, int interior_freedom);
+/* finalizeModelSetup's freedom reset then fixExteriorFaces (apps/forrestania/model_setup.h:9-41,
+ * lithology_model.cpp:369-579): every vertex takes `freedom`, then vertices on
+ * model-bounding facets are classed by those facets' dominant normal axes (corners FIXED,
+ * edges one axis, faces two; contact vertices off the top and bottom get z back);
+ * codes 5 XY_FREE, 6 XZ_FREE, 7 YZ_FREE besides those above */
+int li_model_fix_exterior_faces(void* h, int freedom);
+/* the current freedom of each vertex (n_v values) */
+int li_model_freedom(void* h, uint8_t* freedom);
 /* setVertexFreedom per vertex (n_v values, same codes) */
 int li_model_set_freedom(void* h, const uint8_t* freedom);
 /* axis scale of the parameter vector (runner.cpp:241-248: params in units of s metres) */
@@ -55,6 +63,9 @@ int li_model_pullback(void* h, const double* vgrad, double* out);
 int li_model_dof_map(void* h, int64_t* off, int32_t* vert, double* w, int64_t* n_entries);
 /* geometry bounds params -/+ margin/axisScale (runner.cpp:241-248) */
 int li_model_bounds(void* h, const double* params, double margin_m, double* lo, double* hi);
+/* applyModelBounds + getBounds (lithology_model.cpp:249-282, 802-846): the model box of
+ * the current geometry grown by margin_multiplier percent of its largest side */
+int li_model_model_bounds(void* h, double margin_multiplier, double* lo, double* hi);
 /* SmoothnessRegularization (smoothness.cpp:8-388): weight * penalty at params and its
  * parameter gradient (grad may be NULL) */
 int li_model_smoothness(void* h, const double* params, double weight, double* value,
@@ -127,6 +138,10 @@ typedef struct {
     int32_t lbfgs_history;
     int32_t line_search_max_iter;
     int32_t property_max_iter;
+    int32_t model_bounds;        /* 1: bounds from the model box grown by
+                                    model_bounds_multiplier % (applyModelBounds);
+                                    0: start -/+ bounds_margin */
+    double model_bounds_multiplier;
     double depth_margin;         /* parameter units */
     double bounds_margin;        /* metres (li_model_bounds) */
     double gradient_smoothing;   /* smoothGradient weight (runner.cpp:643-706), 0 = off */

@@ -88,11 +88,14 @@ __device__ __forceinline__ T edge_ln_grad(T s1, T ell, T d1, T d2, T rho2, T& gs
     const T X2 = fabs(s2) + d2;
     const bool pos1 = s1 >= T(0);
     const bool neg2 = s2 < T(0);
-    const bool mid = !(pos1 || neg2);
+    // a station on the edge's line inside the segment (rho = 0) has no finite
+    // ln(X1 X2 / rho^2); it takes the end form, finite there (the integral itself diverges;
+    // the reference's edge term is 0 at such a point, geometry.cpp:48-50)
+    const bool mid = !(pos1 || neg2) && rho2 > T(0);
     const T dsum = d1 + d2;
     const T X12 = X1 * X2;
     const T r = fast_rcp(X12 * dsum);
-    const T irho = fast_rcp(rho2);
+    const T irho = fast_rcp(rho2);  // used by the middle case only
     const T iX1 = X2 * dsum * r;
     const T iX2 = X1 * dsum * r;
     const T y = mid ? (X12 - rho2) * irho : ell * (X1 + X2) * (pos1 ? X2 : X1) * r;

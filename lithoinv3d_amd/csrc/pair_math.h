@@ -160,6 +160,9 @@ __device__ __forceinline__ float edge_ln_f64(double ox, double oy, double oz, do
     if (s1 >= 0.0) { num = X2; den = X1; }
     else if (s2 < 0.0) { num = X1; den = X2; }
     else { num = X1 * X2; den = rho2; }
+    // a station on the segment itself (rho = 0): the integral diverges; the reference's
+    // edge term is 0 there (geometry.cpp:48-50)
+    if (!(den > 0.0)) return 0.0f;
     return (float)log(num / den);
 }
 

@@ -113,6 +113,19 @@ int li_model_fix_exterior(void* h, int interior_freedom)
     return guard([&] { ((ModelHandle*)h)->m->fixExteriorVertices((VertexFreedom)interior_freedom); });
 }
 
+int li_model_fix_exterior_faces(void* h, int freedom)
+{
+    return guard([&] { ((ModelHandle*)h)->m->fixExteriorFaces((VertexFreedom)freedom); });
+}
+
+int li_model_freedom(void* h, uint8_t* freedom)
+{
+    return guard([&] {
+        const auto& m = *((ModelHandle*)h)->m;
+        for (size_t v = 0; v < m.vertexCount(); ++v) freedom[v] = (uint8_t)m.vertexFreedom(v);
+    });
+}
+
 int li_model_set_freedom(void* h, const uint8_t* freedom)
 {
     return guard([&] {
@@ -201,6 +214,16 @@ int li_model_bounds(void* h, const double* params, double margin_m, double* lo, 
 
 // Joint gravity + magnetic problem on one station set.  *_std <= 0 means unweighted.
 // SurfaceSmoothness of the model at params: weight * penalty and its parameter gradient.
+int li_model_model_bounds(void* h, double margin_multiplier, double* lo, double* hi)
+{
+    return guard([&] {
+        Vec l, u;
+        ((ModelHandle*)h)->m->getModelBounds(margin_multiplier, l, u);
+        std::memcpy(lo, l.data(), sizeof(double) * l.size());
+        std::memcpy(hi, u.data(), sizeof(double) * u.size());
+    });
+}
+
 int li_model_smoothness(void* h, const double* params, double weight, double* value,
                         double* grad)
 {
@@ -401,6 +424,8 @@ int li_problem_run(void* h, const LiRunnerOptions* o, void* label_grid, double* 
         ro.depthBounds = o->depth_bounds != 0;
         ro.depthMargin = o->depth_margin;
         ro.boundsMargin = o->bounds_margin;
+        ro.modelBounds = o->model_bounds != 0;
+        ro.modelBoundsMultiplier = o->model_bounds_multiplier;
         ro.gradientSmoothing = o->gradient_smoothing;
         ro.lbfgsHistory = o->lbfgs_history;
         ro.tolerance = o->tolerance;

@@ -79,6 +79,55 @@ void LithologyModel::fixExteriorVertices(VertexFreedom interior)
     rebuildDofs();
 }
 
+void LithologyModel::fixExteriorFaces(VertexFreedom freedom)
+{
+    const size_t nv = vertexCount();
+    m_freedom.assign(nv, freedom);
+    // per vertex: bounding facets by dominant normal class, and contact facets
+    std::vector<uint8_t> tb(nv, 0), ns(nv, 0), ew(nv, 0), contact(nv, 0);
+    for (size_t f = 0; f < m_front.size(); ++f) {
+        const uint32_t* t = &m_faces[3 * f];
+        if (m_back[f] >= 0) {
+            for (int k = 0; k < 3; ++k) contact[t[k]] = 1;
+            continue;
+        }
+        const double* a = &m_vertices[3 * (size_t)t[0]];
+        const double* b = &m_vertices[3 * (size_t)t[1]];
+        const double* c = &m_vertices[3 * (size_t)t[2]];
+        const double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        const double e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        const double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                             e1[0] * e2[1] - e1[1] * e2[0]};
+        if (std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]) < 1e-30) continue;
+        const double ax = std::abs(n[0]), ay = std::abs(n[1]), az = std::abs(n[2]);
+        std::vector<uint8_t>& cls = (az >= ax && az >= ay) ? tb : (ay >= ax ? ns : ew);
+        for (int k = 0; k < 3; ++k) cls[t[k]] = 1;
+    }
+    for (size_t v = 0; v < nv; ++v) {
+        const int nClass = tb[v] + ns[v] + ew[v];
+        if (nClass == 0) continue;
+        VertexFreedom f;
+        if (nClass >= 3)
+            f = VertexFreedom::FIXED;
+        else if (nClass == 2)
+            f = (tb[v] && ns[v]) ? VertexFreedom::X_ONLY
+                                 : (tb[v] && ew[v]) ? VertexFreedom::Y_ONLY : VertexFreedom::Z_ONLY;
+        else
+            f = tb[v] ? VertexFreedom::XY_FREE : (ns[v] ? VertexFreedom::X_ONLY : VertexFreedom::Y_ONLY);
+        if (contact[v] && !tb[v]) {
+            switch (f) {
+                case VertexFreedom::FIXED: f = VertexFreedom::XYZ_FREE; break;
+                case VertexFreedom::X_ONLY: f = VertexFreedom::XZ_FREE; break;
+                case VertexFreedom::Y_ONLY: f = VertexFreedom::YZ_FREE; break;
+                case VertexFreedom::XY_FREE: f = VertexFreedom::XYZ_FREE; break;
+                default: break;
+            }
+        }
+        m_freedom[v] = f;
+    }
+    rebuildDofs();
+}
+
 void LithologyModel::setAxisScale(int axis, double s)
 {
     if (axis >= 0 && axis < 3 && s > 0) m_axisScale[axis] = s;
@@ -95,6 +144,9 @@ bool axisFree(VertexFreedom f, int a)
         case VertexFreedom::X_ONLY: return a == 0;
         case VertexFreedom::Y_ONLY: return a == 1;
         case VertexFreedom::Z_ONLY: return a == 2;
+        case VertexFreedom::XY_FREE: return a != 2;
+        case VertexFreedom::XZ_FREE: return a != 1;
+        case VertexFreedom::YZ_FREE: return a != 0;
         default: return false;
     }
 }
@@ -381,18 +433,12 @@ void LithologyModel::rebuildDofs()
     m_dofAxis.clear();
     for (size_t v = 0; v < vertexCount(); ++v) {
         if (m_cpStride > 0 && !m_isCP[v]) continue;
-        switch (m_freedom[v]) {
-            case VertexFreedom::FIXED: break;
-            case VertexFreedom::Z_ONLY: m_dofVertex.push_back((int)v); m_dofAxis.push_back(2); break;
-            case VertexFreedom::X_ONLY: m_dofVertex.push_back((int)v); m_dofAxis.push_back(0); break;
-            case VertexFreedom::Y_ONLY: m_dofVertex.push_back((int)v); m_dofAxis.push_back(1); break;
-            case VertexFreedom::XYZ_FREE:
-                for (int a = 0; a < 3; ++a) {
-                    m_dofVertex.push_back((int)v);
-                    m_dofAxis.push_back(a);
-                }
-                break;
-        }
+        // one parameter per free axis, x before y before z (surface_mesh.cpp:91-172)
+        for (int a = 0; a < 3; ++a)
+            if (axisFree(m_freedom[v], a)) {
+                m_dofVertex.push_back((int)v);
+                m_dofAxis.push_back(a);
+            }
     }
 }
 
@@ -485,6 +531,25 @@ void LithologyModel::getBounds(const Vec& params, double margin_m, Vec& lower, V
         const double m = margin_m / m_axisScale[m_dofAxis[d]];
         lower[d] = params[d] - m;
         upper[d] = params[d] + m;
+    }
+}
+
+void LithologyModel::getModelBounds(double marginMultiplier, Vec& lower, Vec& upper) const
+{
+    double lo[3] = {1e30, 1e30, 1e30}, hi[3] = {-1e30, -1e30, -1e30};
+    for (size_t v = 0; v < vertexCount(); ++v)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], m_vertices[3 * v + a]);
+            hi[a] = std::max(hi[a], m_vertices[3 * v + a]);
+        }
+    const double margin = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1.0}) * 0.01 *
+                          marginMultiplier;
+    lower.resize(m_dofVertex.size());
+    upper.resize(m_dofVertex.size());
+    for (size_t d = 0; d < m_dofVertex.size(); ++d) {
+        const int a = m_dofAxis[d];
+        lower[d] = (lo[a] - margin) / m_axisScale[a];
+        upper[d] = (hi[a] + margin) / m_axisScale[a];
     }
 }
 

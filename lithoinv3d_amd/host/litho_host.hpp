@@ -58,7 +58,11 @@ using GravityData = std::vector<GravityPoint>;
 using MagneticData = std::vector<MagneticPoint>;
 
 // modules/litho-surface/include/litho_invert/surface/surface_mesh.h (VertexFreedom)
-enum class VertexFreedom : uint8_t { FIXED = 0, Z_ONLY = 1, X_ONLY = 2, Y_ONLY = 3, XYZ_FREE = 8 };
+// (ALONG_VECTOR = 4 of the reference is not supported)
+enum class VertexFreedom : uint8_t {
+    FIXED = 0, Z_ONLY = 1, X_ONLY = 2, Y_ONLY = 3, XY_FREE = 5, XZ_FREE = 6, YZ_FREE = 7,
+    XYZ_FREE = 8
+};
 
 // modules/litho-model/include/litho_invert/litho/litho_group.h:7-34
 enum class RemanentMagnetizationMode { EffectiveSusceptibility = 0, FixedVectorPerGroup = 1, VectorPerGroup = 2 };
@@ -96,6 +100,16 @@ public:
     // rest takes `interior` (the role of LithologyModel::fixExteriorFaces,
     // lithology_model.cpp:369-579, for closed voxel models).
     void fixExteriorVertices(VertexFreedom interior);
+    // LithologyModel::fixExteriorFaces after finalizeModelSetup's freedom reset
+    // (lithology_model.cpp:369-579, apps/forrestania/model_setup.h:9-41): every vertex takes
+    // `freedom`, then each vertex on model-bounding facets (back == -1) is classed by the
+    // dominant normal axes of those facets (z; +-y; +-x): three classes FIXED; two: X_ONLY
+    // (z and y), Y_ONLY (z and x), Z_ONLY (x and y); one: XY_FREE (z), X_ONLY (y), Y_ONLY (x);
+    // a vertex that is also on a contact facet and on no z-class facet gets z back
+    // (FIXED -> XYZ_FREE, X_ONLY -> XZ_FREE, Y_ONLY -> YZ_FREE, XY_FREE -> XYZ_FREE).
+    // The reference classes each unit mesh's copy of a shared vertex by that mesh's own
+    // facets; here one vertex is classed by all its bounding facets.
+    void fixExteriorFaces(VertexFreedom freedom);
     void setAxisScale(int axis, double s);
     double axisScale(int axis) const { return m_axisScale[axis]; }
 
@@ -123,6 +137,10 @@ public:
                       std::vector<double>& w) const;
     // params -/+ margin/axisScale (runner.cpp:241-248)
     void getBounds(const Vec& params, double margin_m, Vec& lower, Vec& upper) const;
+    // applyModelBounds + getBounds (lithology_model.cpp:249-282, 802-846): every
+    // parameter inside the model's bounding box of the current geometry grown by
+    // marginMultiplier percent of its largest side (at least 1 m), per axis
+    void getModelBounds(double marginMultiplier, Vec& lower, Vec& upper) const;
 
 private:
     void rebuildDofs();

@@ -381,7 +381,10 @@ RunnerResult runInversion(std::shared_ptr<LithologyModel> model, RunnerProblem& 
 
     Vec params = model->assembleParameterVector();
     Vec lower, upper;
-    model->getBounds(params, opt.boundsMargin, lower, upper);
+    if (opt.modelBounds)
+        model->getModelBounds(opt.modelBoundsMultiplier, lower, upper);
+    else
+        model->getBounds(params, opt.boundsMargin, lower, upper);
     if (opt.depthBounds)
         for (size_t i = 0; i < params.size(); ++i) {
             lower[i] = std::max(lower[i], params[i] - opt.depthMargin);

@@ -88,6 +88,8 @@ struct RunnerOptions {
     bool eigenvalueScaling = true;   // calibrate alpha_mag at the start (magnetics only)
     double gradientSmoothing = 0.0;  // smoothGradient weight, 0 = off
     double boundsMargin = 200.0;     // getBounds margin (m)
+    bool modelBounds = false;        // bounds from the model box (applyModelBounds) instead
+    double modelBoundsMultiplier = 1.0;
     bool depthBounds = false;        // also clamp to the start -/+ depthMargin (param units)
     double depthMargin = 500.0;
     bool validate = true;            // the validation gate after each geometry phase

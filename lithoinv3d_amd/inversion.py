@@ -17,7 +17,7 @@ from . import lithofwd
 _HERE = os.path.dirname(os.path.abspath(__file__))
 HOST_PATH = os.path.join(_HERE, "liblithohost.so")
 
-FIXED, Z_ONLY, X_ONLY, Y_ONLY, XYZ_FREE = 0, 1, 2, 3, 8
+FIXED, Z_ONLY, X_ONLY, Y_ONLY, XY_FREE, XZ_FREE, YZ_FREE, XYZ_FREE = 0, 1, 2, 3, 5, 6, 7, 8
 
 _P = ctypes.c_void_p
 _D = ctypes.POINTER(ctypes.c_double)
@@ -35,8 +35,8 @@ class _RunnerOptions(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in (
         "max_iterations", "property_interval", "geometry", "properties", "eigenvalue_scaling",
         "validate", "depth_bounds", "lbfgs_history", "line_search_max_iter",
-        "property_max_iter")] + [(n, ctypes.c_double) for n in (
-            "depth_margin", "bounds_margin", "gradient_smoothing", "tolerance", "armijo_c1",
+        "property_max_iter", "model_bounds")] + [(n, ctypes.c_double) for n in (
+            "model_bounds_multiplier", "depth_margin", "bounds_margin", "gradient_smoothing", "tolerance", "armijo_c1",
             "density_lower", "density_upper", "density_damping", "susceptibility_lower",
             "susceptibility_upper", "susceptibility_damping", "property_tolerance")]
 
@@ -81,6 +81,9 @@ def load_host():
                                     ctypes.c_char_p, i64]
     L.li_durbin_watson.argtypes = [i64, _P, _P, _P]
     L.li_model_properties.argtypes = [_P, _P, _P]
+    L.li_model_fix_exterior_faces.argtypes = [_P, ctypes.c_int]
+    L.li_model_freedom.argtypes = [_P, _P]
+    L.li_model_model_bounds.argtypes = [_P, ctypes.c_double, _P, _P]
     L.li_label_grid_create.argtypes = [_P, _P, _P, _P, ctypes.POINTER(_P)]
     L.li_label_grid_destroy.argtypes = [_P]
     L.li_label_grid_destroy.restype = None
@@ -144,6 +147,17 @@ class Model:
                 load_host().li_model_destroy(self._h)
         except Exception:
             pass
+
+    def fix_exterior_faces(self, freedom=XYZ_FREE):
+        """The reference's model setup: every vertex takes `freedom`, then the vertices of
+        model-bounding facets get the freedoms fixExteriorFaces gives them (corners
+        FIXED, edges one axis, faces two, contact vertices off the top and bottom keep z)."""
+        _chk(load_host().li_model_fix_exterior_faces(self._h, int(freedom)), "fix_exterior_faces")
+
+    def freedoms(self):
+        out = np.zeros(self.n_v, np.uint8)
+        _chk(load_host().li_model_freedom(self._h, _p(out)), "freedom")
+        return out
 
     def fix_exterior(self, interior=Z_ONLY):
         _chk(load_host().li_model_fix_exterior(self._h, interior), "fix_exterior")
@@ -226,6 +240,14 @@ class Model:
                                             radius, _p(p), weight, ctypes.byref(v), _p(g), _p(z)),
              "boreholes")
         return v.value, g, z
+
+    def model_bounds(self, margin_multiplier=1.0):
+        """applyModelBounds + getBounds: the model's box grown by margin_multiplier percent
+        of its largest side, in parameter units."""
+        lo, hi = np.zeros(self.n_dof), np.zeros(self.n_dof)
+        _chk(load_host().li_model_model_bounds(self._h, margin_multiplier, _p(lo), _p(hi)),
+             "model_bounds")
+        return lo, hi
 
     def bounds(self, params, margin_m):
         p = _a(params)
@@ -424,12 +446,15 @@ class Problem:
 
     def run(self, max_iterations=500, property_interval=50, geometry=True, properties=False,
             eigenvalue_scaling=True, validate=True, grid=None, gradient_smoothing=0.0,
-            bounds_margin=200.0, depth_bounds=False, depth_margin=500.0, lbfgs_history=10,
+            bounds_margin=200.0, model_bounds=None, depth_bounds=False, depth_margin=500.0,
+            lbfgs_history=10,
             tolerance=1e-6, armijo_c1=1e-4, line_search_max_iter=50,
             density_bounds=(-1e300, 1e300), density_damping=0.0,
             susceptibility_bounds=(0.0, 1.0), susceptibility_damping=0.0,
             property_max_iter=50, property_tolerance=1e-6, history_cap=None):
-        """InversionRunner::run from the model's current parameters: returns the history
+        """InversionRunner::run from the model's current parameters (bounds: the start
+        -/+ bounds_margin metres, or with model_bounds = m the model's box grown by m
+        percent of its largest side, as applyModelBounds(m)): returns the history
         (rows: iteration, gravity misfit, magnetic misfit, regularization, total, gravity
         RMS, DW gravity x, y, DW magnetic x, y), the final parameters, densities and
         susceptibilities, and the loop's summary and log."""
@@ -439,6 +464,8 @@ class Problem:
             eigenvalue_scaling=int(eigenvalue_scaling), validate=int(validate),
             depth_bounds=int(depth_bounds), lbfgs_history=lbfgs_history,
             line_search_max_iter=line_search_max_iter, property_max_iter=property_max_iter,
+            model_bounds=int(model_bounds is not None),
+            model_bounds_multiplier=1.0 if model_bounds is None else float(model_bounds),
             depth_margin=depth_margin, bounds_margin=bounds_margin,
             gradient_smoothing=gradient_smoothing, tolerance=tolerance, armijo_c1=armijo_c1,
             density_lower=density_bounds[0], density_upper=density_bounds[1],

@@ -214,3 +214,36 @@ def test_adjoint_at_scale_vs_central_fd():
           f"fp32c vs fp64 over all vertices {e3264:.2e}")
     assert e64 < 1e-5
     assert e32 < 2e-3 and e3264 < 2e-3
+
+
+def test_adjoint_finite_for_station_on_an_edge_line():
+    """A station on the line of a facet edge, inside the segment (the diagonal of a
+    voxel face, in the face's plane), makes the middle-case edge log ln(X1 X2 / rho^2)
+    singular; the adjoint takes the end form there and stays finite in both precisions,
+    as the forward does (the singular pair's facet gets a finite, meaningless term: the
+    integral diverges there)."""
+    labels = np.zeros((2, 2, 1), np.int32)
+    labels[1, :, :] = 1
+    m = synth.label_grid_model(labels, (0.0, 0.0, 0.0), (10.0, 10.0, 10.0), [0.3, -0.2],
+                               [0.01, 0.02])
+    F = m.faces.astype(np.int64)
+    V = m.vertices
+    # a top-face diagonal: two vertices of one top triangle that are not axis-aligned
+    top = np.nonzero(np.all(np.isclose(V[F][:, :, 2], 0.0), axis=1))[0]
+    tri = V[F[top[0]]]
+    pairs = [(0, 1), (1, 2), (2, 0)]
+    a, b = next((tri[i], tri[j]) for i, j in pairs if np.all(np.abs(tri[i, :2] - tri[j, :2]) > 1e-9))
+    on_edge = 0.5 * (a + b)
+    st = np.array([on_edge, [3.0, 4.0, 25.0], [15.0, 12.0, 40.0]])
+    fh = synth.field_unit_vector(-60.0, 0.0)
+    wg = np.array([1.0, 0.5, -0.3])
+    wt = np.array([1e-3, 2e-3, -1e-3])
+    for prec in ("fp64", "fp32c"):
+        say(f"adjoint with a station on an edge line, {prec}")
+        with ForwardContext(m, st, fh, 50000.0, precision=prec) as ctx:
+            gz, tm = ctx.forward()
+            g = ctx.adjoint(wg, wt)
+            g_others = ctx.adjoint(np.array([0.0, 0.5, -0.3]), np.array([0.0, 2e-3, -1e-3]))
+        print(f"{prec}: gz {gz}, |grad| max {np.abs(g).max():.3e}")
+        assert np.all(np.isfinite(gz)) and np.all(np.isfinite(tm))
+        assert np.all(np.isfinite(g)) and np.all(np.isfinite(g_others))

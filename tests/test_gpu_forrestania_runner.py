@@ -4,13 +4,16 @@ test_forrestania_integration.cpp:588-729) restated over the GPU inversion loop.
 Setup as there (:53-82, :264-291, :588-627, :691-722): units 0-3 are the reference's
 volume_group_1..4 (tests/golden/forrestania) with densities 2.67, 2.80, 3.00, 3.50 g/cc
 and susceptibilities 0, 0.005, 0.02, 0.08 SI; 8 x 8 stations over the model's (x, y)
-extent at z = 0 with 0.05 mGal standard deviation, inset by 1 m from the extent (the
-reference's grid puts the edge stations exactly on the model's side faces, where the
-solid angle is the sign of a rounded zero); gz and TMI (inclination -66.2, declination
+extent at z = 0.5 m with 0.05 mGal standard deviation, inset by 1 m from the extent (the
+reference's grid at z = 0 puts stations exactly on the model's side faces and on its
+horizontal contact facets at z = 0, inside the model, where the solid angle is the
+sign of a rounded zero); gz and TMI (inclination -66.2, declination
 -0.1, 58 874 nT) of the true model as data; the starting model with every vertex that
-can move in z (the interior after the exterior is fixed) raised by 10 m; XYZ_FREE, axis
-scale 10, smoothness weight 1 (the runner's lambda), magnetic weight 1, eigenvalue
-calibration, L-BFGS history 20, tolerance 1e-6, one geometry iteration per run.
+can move in z raised by 10 m after finalizeModelSetup (XYZ_FREE, then the exterior
+classes of fixExteriorFaces) and bounds from the model box grown by 1 %
+(applyModelBounds); axis scale 10, smoothness weight 1 (the runner's lambda), magnetic
+weight 1, eigenvalue calibration, L-BFGS history 20, tolerance 1e-6, one geometry
+iteration per run.
 
 Checks after every run, as runAllChecks (:480-577): largest displacement from the
 start below 500 m; the top and bottom (vertices within 1 m of the extreme z) flat to
@@ -45,12 +48,12 @@ def cluster_model():
     return IO.litho_model(meshes, [2.67, 2.80, 3.00, 3.50], [0.0, 0.005, 0.02, 0.08])
 
 
-def stations(m, n=8, inset=1.0):
+def stations(m, n=8, inset=1.0, z=0.5):
     lo, hi = m.vertices.min(0), m.vertices.max(0)
     x = np.linspace(lo[0] + inset, hi[0] - inset, n)
     y = np.linspace(lo[1] + inset, hi[1] - inset, n)
     X, Y = np.meshgrid(x, y)
-    return np.column_stack([X.ravel(), Y.ravel(), np.zeros(n * n)])
+    return np.column_stack([X.ravel(), Y.ravel(), np.full(n * n, z)])
 
 
 def setup():
@@ -62,7 +65,7 @@ def setup():
     gz, tm = P.forward(M0.params())
     P.close()
     M = inversion.Model(m)
-    M.fix_exterior(inversion.XYZ_FREE)
+    M.fix_exterior_faces(inversion.XYZ_FREE)  # finalizeModelSetup (model_setup.h:9-41)
     M.set_axis_scale(10.0)
     v, ax, _ = M.dofs()
     p = M.params()
@@ -78,8 +81,7 @@ def checks(m, M, V0, V, base, label):
     top = np.abs(V0[:, 2] - zmax) <= 1.0
     bot = np.abs(V0[:, 2] - zmin) <= 1.0
     assert np.abs(V[top, 2] - zmax).max() <= 0.1 and np.abs(V[bot, 2] - zmin).max() <= 0.1, label
-    fixed = np.zeros(m.n_vertices, bool)
-    fixed[m.faces[m.back < 0].ravel()] = True
+    fixed = M.freedoms() == inversion.FIXED
     moved = disp > 1e-6
     assert (~fixed).sum() > 0 and moved.sum() > 0.5 * (~fixed).sum(), (label, moved.sum())
     lo, hi = V0.min(0), V0.max(0)
@@ -112,7 +114,7 @@ def test_forrestania_runner_comprehensive(n_runs):
         P = problem(M, st, gz, tm)
         c = P.evaluate(M.params())  # gravity, magnetic (weight 1), regularization, total
         r = P.run(max_iterations=1, lbfgs_history=20, tolerance=1e-6, eigenvalue_scaling=True,
-                  validate=True, bounds_margin=500.0)
+                  validate=True, model_bounds=1.0)
         P.close()
         print(r["log"].strip())
         V = M.apply(r["params"])

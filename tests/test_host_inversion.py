@@ -248,3 +248,95 @@ def test_control_points_against_reference_restatement(stride, cell):
     expect = W @ np.where(cp, dz, 0.0)
     np.testing.assert_allclose(dz[free], expect[free], rtol=0, atol=1e-9)
     assert np.abs(dz[~free]).max() == 0.0
+
+
+def ref_box(x0, y0, z0, x1, y1, z1):
+    """The reference tests' makeBoxMesh (vertices 0-3 at z0, 4-7 at z1, its triangles)."""
+    V = np.array([[x0, y0, z0], [x1, y0, z0], [x1, y1, z0], [x0, y1, z0],
+                  [x0, y0, z1], [x1, y0, z1], [x1, y1, z1], [x0, y1, z1]], float)
+    T = np.array([[0, 1, 2], [0, 2, 3], [4, 7, 6], [4, 6, 5], [3, 2, 6], [3, 6, 7],
+                  [0, 4, 5], [0, 5, 1], [1, 5, 6], [1, 6, 2], [0, 3, 7], [0, 7, 4]], np.uint32)
+    return V, T
+
+
+def test_fix_exterior_faces_reference_cases():
+    """modules/litho-model/tests/test_fix_exterior.cpp restated: a single box's eight
+    vertices are corners (FIXED); two stacked boxes keep the upper box's top corners
+    FIXED and give the shared face's corners (two side classes, on a contact) Z_ONLY;
+    the classification is idempotent.  Plus the edge and face classes on a 2 x 2 x 2
+    voxel block of two units (edges move along themselves, a side-face centre along one
+    horizontal axis, a top-face centre in x and y, a side-face centre on the contact gets
+    z back, vertices inside keep the given freedom)."""
+    from lithoinv3d_amd import litho_io as IO
+
+    a = IO.assemble([ref_box(0.0, 0.0, 0.0, 10.0, 10.0, -20.0)])
+    M = inversion.Model(synth.LithoModel(a.vertices, a.faces, a.front, a.back, [2.67], [0.0]))
+    M.fix_exterior_faces(inversion.XYZ_FREE)
+    np.testing.assert_array_equal(M.freedoms(), np.full(8, inversion.FIXED))
+    assert M.n_dof == 0
+    M.fix_exterior_faces(inversion.XYZ_FREE)
+    np.testing.assert_array_equal(M.freedoms(), np.full(8, inversion.FIXED))
+
+    a = IO.assemble([ref_box(0.0, 0.0, 0.0, 10.0, 10.0, -20.0),
+                     ref_box(0.0, 0.0, -20.0, 10.0, 10.0, -50.0)])
+    m2 = synth.LithoModel(a.vertices, a.faces, a.front, a.back, [2.67, 3.0], [0.0, 0.0])
+    M = inversion.Model(m2)
+    M.fix_exterior_faces(inversion.XYZ_FREE)
+    fr = M.freedoms()
+    z = m2.vertices[:, 2]
+    assert np.all(fr[np.isclose(z, 0.0)] == inversion.FIXED)
+    assert np.all(fr[np.isclose(z, -50.0)] == inversion.FIXED)
+    assert np.all(fr[np.isclose(z, -20.0)] == inversion.Z_ONLY)
+    first = fr.copy()
+    M.fix_exterior_faces(inversion.XYZ_FREE)
+    np.testing.assert_array_equal(M.freedoms(), first)
+
+    # 2 x 2 x 2 voxels, two units split at x = 25 (unit 0 for i = 0, unit 1 for i = 1)
+    labels = np.zeros((2, 2, 2), np.int32)
+    labels[1] = 1
+    m = synth.label_grid_model(labels, (0.0, 0.0, 0.0), (25.0, 25.0, 25.0), [0.1, 0.2],
+                               [0.0, 0.0])
+    M = inversion.Model(m)
+    M.fix_exterior_faces(inversion.XYZ_FREE)
+    fr = M.freedoms()
+    V = m.vertices
+
+    def at(x, y, z):
+        k = np.nonzero(np.all(np.isclose(V, [x, y, z]), axis=1))[0]
+        assert len(k) == 1
+        return fr[k[0]]
+
+    assert at(0.0, 0.0, 0.0) == inversion.FIXED        # corner: z, y and x classes
+    assert at(0.0, 25.0, 0.0) == inversion.Y_ONLY      # top-west edge (z and x): along y
+    assert at(0.0, 0.0, -25.0) == inversion.Z_ONLY     # south-west edge (y and x): along z
+    assert at(0.0, 25.0, -25.0) == inversion.Y_ONLY    # west face centre (x class only)
+    assert at(25.0, 25.0, 0.0) == inversion.XY_FREE    # top face centre, on the contact
+    assert at(25.0, 0.0, -25.0) == inversion.XZ_FREE   # south face centre on the contact: z back
+    assert at(25.0, 25.0, -25.0) == inversion.XYZ_FREE  # inside, on the contact
+
+
+def test_global_dofs_reference_cases():
+    """modules/litho-model/tests/test_global_dofs.cpp restated on the shared-vertex
+    model: two units holding the same four positions (Z_ONLY) share 4 parameters, not 8;
+    4 XYZ_FREE plus 4 Z_ONLY vertices at different positions give 16; writing -200 to
+    every z parameter and reading them back gives -200 (axis scale 1)."""
+    from lithoinv3d_amd import litho_io as IO
+
+    sq = lambda z: (np.array([[0, 0, z], [1, 0, z], [1, 1, z], [0, 1, z]], float),
+                    np.array([[0, 1, 2], [0, 2, 3]], np.uint32))
+    a = IO.assemble([sq(-100.0), sq(-100.0)])
+    m = synth.LithoModel(a.vertices, a.faces, a.front, a.back, [2.67, 3.0], [0.0, 0.0])
+    M = inversion.Model(m)
+    M.set_freedom(np.full(m.n_vertices, inversion.Z_ONLY, np.uint8))
+    assert m.n_vertices == 4 and M.n_dof == 4
+    p = np.full(M.n_dof, -200.0)
+    M.apply(p)
+    np.testing.assert_allclose(M.params(), -200.0)
+
+    b = IO.assemble([sq(-50.0), sq(-150.0)])
+    m2 = synth.LithoModel(b.vertices, b.faces, b.front, b.back, [2.67, 3.0], [0.0, 0.0])
+    M2 = inversion.Model(m2)
+    fr = np.where(np.isclose(m2.vertices[:, 2], -50.0), inversion.XYZ_FREE,
+                  inversion.Z_ONLY).astype(np.uint8)
+    M2.set_freedom(fr)
+    assert M2.n_dof == 16

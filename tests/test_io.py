@@ -192,3 +192,30 @@ def test_forrestania_files(oracle):
     st = s.xyz[::40]
     np.testing.assert_allclose(om.gravity(st), ref.gravity(st), rtol=0, atol=1e-12)
     np.testing.assert_allclose(om.magnetic(fh, st), ref.magnetic(fh, st), rtol=1e-12, atol=1e-8)
+
+
+def test_tsurf_writer_reference_cases(tmp_path):
+    """modules/litho-io/tests/test_ts_content.cpp restated for the TSurf writer: 8
+    digits after the point in scientific notation (123.456789012345 -> 1.23456789e+02),
+    1-based triangle indices, every vertex written once, an empty mesh still has the
+    header, TFACE and END with no VRTX or TRGL line, printable ASCII only, and the same
+    mesh written twice gives identical files."""
+    V = np.array([[123.456789012345, 0.0, -50.0], [0.0, 0.0, 0.0], [1.0, 1.0, 1.0]])
+    T = np.array([[0, 1, 2]], np.uint32)
+    p = tmp_path / "precision.ts"
+    IO.write_ts(p, V, T, name="precision")
+    lines = p.read_text().splitlines()
+    v1 = next(l for l in lines if l.startswith("VRTX 1 "))
+    assert v1.split()[2] == "1.23456789e+02"
+    assert [l for l in lines if l.startswith("TRGL")] == ["TRGL 1 2 3"]
+    assert [l.split()[1] for l in lines if l.startswith("VRTX")] == ["1", "2", "3"]
+    raw = p.read_bytes()
+    assert all(b in (10, 13) or 0x20 <= b <= 0x7E for b in raw)
+    q = tmp_path / "again.ts"
+    IO.write_ts(q, V, T, name="precision")
+    assert q.read_bytes() == raw
+    e = tmp_path / "empty.ts"
+    IO.write_ts(e, np.zeros((0, 3)), np.zeros((0, 3), np.uint32), name="empty")
+    text = e.read_text()
+    assert "GOCAD TSurf 1" in text and "TFACE" in text and text.rstrip().endswith("END")
+    assert not any(l.startswith(("VRTX", "TRGL")) for l in text.splitlines())
