@@ -10,6 +10,10 @@ message (this build runs on one GPU).
 
 value = station x facet pairs per second for forward + adjoint
       = n_stations * n_facets / (seconds per step)
+
+After the timed region (not part of value): the accuracy of the FP32C forward against
+the FP64 path, the CPU baseline, and the second half of the headline metric (seconds of
+the learned-H0 FP32C L-BFGS-B to the plain FP64 run's final misfit, "inversion").
 """
 from __future__ import annotations
 
@@ -96,6 +100,47 @@ def cpu_baseline(model, st, fh, field, seconds):
     }
 
 
+def inversion_check(n_models, n_train=24):
+    """The second half of the headline metric on the config-1-style contact models
+    (64x64x32 cells, 128x128 stations, about 1 660 control-point parameters): the H0
+    network trained on n_train seeded models, then per held-out model the plain FP64
+    L-BFGS-B run of 40 iterations and the FP32C runs to its final misfit with and without
+    the learned H0 (seconds include the learned H0's subset column norms, features and
+    network).  Medians over the held-out models."""
+    from lithoinv3d_amd import experiments as E
+
+    t0 = time.perf_counter()
+    X, t = E.training_set(range(1000, 1000 + n_train), (64, 64, 32), 128)
+    net, norm, _ = E.train_mlp(X, t, steps=3000)
+    t_train = time.perf_counter() - t0
+    rows = []
+    for s in range(2000, 2000 + n_models):
+        r = E.compare_solvers(E.contact_problem(s, (64, 64, 32), 128), net, norm, max_iter=40)
+        rows.append(r)
+        log(f"inversion seed {s}: {json.dumps(r)}")
+    net.close()
+
+    def med(f):
+        v = [f(r) for r in rows]
+        v = [x for x in v if x is not None]
+        return float(np.median(v)) if v else None
+
+    hit = lambda r, k, i: r[k]["to_target"][i] if r[k]["to_target"] else None
+    return {
+        "models": "contact64x64x32_st128, seeds 2000..%d held out, %d trained" % (
+            1999 + n_models, n_train),
+        "train_seconds": t_train,
+        "fp64_seconds": med(lambda r: r["fp64"]["seconds"]),
+        "fp64_iterations": med(lambda r: r["fp64"]["iterations"]),
+        "fp32c_seconds_to_fp64_misfit": med(lambda r: hit(r, "fp32c", 1)),
+        "learned_fp32c_seconds_to_fp64_misfit": med(lambda r: hit(r, "learned_fp32c", 1)),
+        "learned_fp32c_iterations_to_fp64_misfit": med(lambda r: hit(r, "learned_fp32c", 0)),
+        "learned_time_ratio_vs_fp64": med(
+            lambda r: hit(r, "learned_fp32c", 1) / r["fp64"]["seconds"]
+            if r["learned_fp32c"]["to_target"] else None),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +151,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--inversion", type=int, default=2,
+                    help="held-out models for the L-BFGS-B seconds-to-target check (0: skip)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one plain device copy of this many MiB (a kernel of known "
                          "traffic for scaling the HBM byte counters of a profiler run)")
@@ -246,6 +293,15 @@ def main():
         cpu = cpu_baseline(m, st, fh, field, args.cpu_seconds)
         log(f"cpu baseline: {cpu}")
 
+    inv = None
+    if args.inversion > 0:
+        log("L-BFGS-B seconds to the plain FP64 misfit, learned H0")
+        try:
+            inv = inversion_check(args.inversion)
+        except Exception as e:  # the throughput line is still reported
+            inv = {"error": repr(e)}
+        log(f"inversion: {inv}")
+
     out = {
         "metric": "station x facet evals/s (fwd+grad)",
         "value": pairs / sec,
@@ -282,6 +338,7 @@ def main():
         "kernels": per_kernel,
         "accuracy": acc,
         "cpu_baseline": cpu,
+        "inversion": inv,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
