@@ -14,7 +14,7 @@
 //   Gp    = sum w dF/dp                                  (3)
 //   A_jk  = sum w (dF/dp_j p_k - dF/dp_k p_j), j<k     (3)  -- the frame's rotation
 //   tbar  = sum w dF/dtheta                              (16)
-// 22 numbers per facet, lane = facet, stations wave-uniform (scalar loads).  A finalize
+// 20 numbers per facet, lane = facet, stations wave-uniform (scalar loads).  A finalize
 // kernel then pushes them through R(a,b,c) and theta(a,b,c) once per facet with
 // forward-mode duals (9 directions):
 //   dPhi/dx_k = sum_{j<k'} (dR_j/dx_k . R_k') A_jk' + tbar . dtheta/dx_k - [x_k in a] (R^T Gp)_k
@@ -33,10 +33,12 @@
 
 namespace lfk {
 
-constexpr int kSeeds = 22;
+// S_PHI1 / S_PHI2: the in-plane directions (e1u, e1v), (e2u, e2v) of edges 1 and 2 are
+// unit vectors, so only their angle has a seed: (se_u, se_v) . (-e_v, e_u).
+constexpr int kSeeds = 20;
 enum SeedIdx {
     S_GPX = 0, S_GPY, S_GPZ, S_AXY, S_AXZ, S_AYZ,
-    S_L0, S_CU, S_CV, S_E1U, S_E1V, S_L1, S_E2U, S_E2V, S_L2, S_TWOA,
+    S_L0, S_CU, S_CV, S_PHI1, S_L1, S_PHI2, S_L2, S_TWOA,
     S_K1, S_K2, S_W0, S_W1, S_W2, S_W3
 };
 
@@ -108,7 +110,7 @@ __device__ __forceinline__ T edge_ln_grad(T s1, T ell, T d1, T d2, T rho2, T& gs
 }
 
 // One (station, facet) pair: frame-form forward and its reverse sweep for the weights
-// (lam on gz, mu on TMI), adding the pair's 22 facet seeds into acc (see the layout in
+// (lam on gz, mu on TMI), adding the pair's 20 facet seeds into acc (see the layout in
 // SeedIdx).  Linear in (lam, mu).
 template <class T>
 __device__ __forceinline__ void pair_seeds(const FaceFrame<T>& F, double ox, double oy,
@@ -185,26 +187,22 @@ __device__ __forceinline__ void pair_seeds(const FaceFrame<T>& F, double ox, dou
     sdp1 += T(2) * dp1 * srho1;
     sdp2 += T(2) * dp2 * srho2;
     T spz2 = srho1 + srho2;
-    // s1b = t e1u - py e1v
+    // s1b = t e1u - py e1v, dp1 = t e1v + py e1u
     st_ += ss1b * F.e1u;
     spy -= ss1b * F.e1v;
-    T se1u = ss1b * t;
-    T se1v = -ss1b * py;
-    // s1c = dcx e2u + dcy e2v
-    T sdcx = ss1c * F.e2u;
-    T sdcy = ss1c * F.e2v;
-    T se2u = ss1c * dcx;
-    T se2v = ss1c * dcy;
-    // dp1 = t e1v + py e1u
     st_ += sdp1 * F.e1v;
     spy += sdp1 * F.e1u;
-    se1v += sdp1 * t;
-    se1u += sdp1 * py;
-    // dp2 = dcx e2v - dcy e2u
+    // s1c = dcx e2u + dcy e2v, dp2 = dcx e2v - dcy e2u
+    T sdcx = ss1c * F.e2u;
+    T sdcy = ss1c * F.e2v;
     sdcx += sdp2 * F.e2v;
     sdcy -= sdp2 * F.e2u;
-    se2v += sdp2 * dcx;
-    se2u -= sdp2 * dcy;
+    // edge angles: (s_e1u, s_e1v) = ss1b (t, -py) + sdp1 (py, t) projected on (-e1v, e1u)
+    // is sdp1 s1b - ss1b dp1; likewise sdp2 s1c - ss1c dp2 for edge 2
+    acc[S_PHI1] = fma(sdp1, s1b, acc[S_PHI1]);
+    acc[S_PHI1] = fma(-ss1b, dp1, acc[S_PHI1]);
+    acc[S_PHI2] = fma(sdp2, s1c, acc[S_PHI2]);
+    acc[S_PHI2] = fma(-ss1c, dp2, acc[S_PHI2]);
     // th = atan2(N, D)
     const T q = fast_rcp(N * N + D * D);
     const T sN = sth * D * q;
@@ -260,17 +258,13 @@ __device__ __forceinline__ void pair_seeds(const FaceFrame<T>& F, double ox, dou
     acc[S_GPX] += spx;
     acc[S_GPY] += spy;
     acc[S_GPZ] += spz;
-    acc[S_AXY] += spx * py - spy * px;
-    acc[S_AXZ] += spx * pz - spz * px;
-    acc[S_AYZ] += spy * pz - spz * py;
+    acc[S_AXY] = fma(-spy, px, fma(spx, py, acc[S_AXY]));
+    acc[S_AXZ] = fma(-spz, px, fma(spx, pz, acc[S_AXZ]));
+    acc[S_AYZ] = fma(-spz, py, fma(spy, pz, acc[S_AYZ]));
     acc[S_L0] += sl0c;
     acc[S_CU] += scu;
     acc[S_CV] += scv;
-    acc[S_E1U] += se1u;
-    acc[S_E1V] += se1v;
     acc[S_L1] += sL1;
-    acc[S_E2U] += se2u;
-    acc[S_E2V] += se2v;
     acc[S_L2] += sL2;
     acc[S_TWOA] += stwoA;
 }
@@ -298,12 +292,17 @@ __global__ __launch_bounds__(256) LF_ADJOINT_OCC void k_adjoint_pairs(const Face
     for (int k = 0; k < kSeeds; ++k) acc[k] = T(0);
     const int64_t i0 = (int64_t)chunk * st_per_chunk;
     const int64_t i1 = min(n_s, i0 + st_per_chunk);
+    // a 32-bit count keeps the loop test on the scalar unit (it has no 64-bit less-than)
+    const StationW* __restrict__ sp = st + (i0 < i1 ? i0 : 0);
+    const int cnt = i0 < i1 ? (int)(i1 - i0) : 0;
     // the next station record is requested (scalar loads) while this one is used
-    StationW Sn = st[i0 < i1 ? i0 : 0];
-    for (int64_t i = i0; i < i1; ++i) {
+    StationW Sn = sp[0];
+    for (int i = 0; i < cnt; ++i) {
         const StationW S = Sn;
-        Sn = st[i + 1 < i1 ? i + 1 : i];
+        Sn = sp[i + 1 < cnt ? i + 1 : i];
         if (S.skip) continue;
+        // T(w) from the scalar FP64 weight lands in a vector register, where the ~12 products
+        // that use it issue faster than with a scalar-register operand
         pair_seeds(F, S.x, S.y, S.z, T(S.wg), T(S.wt), acc);
     }
     if (active) {
@@ -313,7 +312,7 @@ __global__ __launch_bounds__(256) LF_ADJOINT_OCC void k_adjoint_pairs(const Face
     }
 }
 
-// Per-facet derivative matrix T (22 x 9): the derivative of the facet's contribution
+// Per-facet derivative matrix T (20 x 9): the derivative of the facet's contribution
 // with respect to vertex coordinate k (a.x, a.y, a.z, b.x, ..., c.z) is
 // sum_m seed_m * T[m][k] for the per-pair (or accumulated) seeds of SeedIdx.  Built
 // from the facet constants with forward-mode duals along the nine directions.
@@ -351,11 +350,10 @@ __device__ __forceinline__ bool facet_T(const double P[3][3], double drho, const
         Tm[S_L0][k] = fd.l0.d;
         Tm[S_CU][k] = fd.cu.d;
         Tm[S_CV][k] = fd.cv.d;
-        Tm[S_E1U][k] = fd.e1u.d;
-        Tm[S_E1V][k] = fd.e1v.d;
+        // d(angle) of the unit edge directions: (e_u, e_v) x d(e_u, e_v)
+        Tm[S_PHI1][k] = fv.e1u * fd.e1v.d - fv.e1v * fd.e1u.d;
         Tm[S_L1][k] = fd.l1.d;
-        Tm[S_E2U][k] = fd.e2u.d;
-        Tm[S_E2V][k] = fd.e2v.d;
+        Tm[S_PHI2][k] = fv.e2u * fd.e2v.d - fv.e2v * fd.e2u.d;
         Tm[S_L2][k] = fd.l2.d;
         Tm[S_TWOA][k] = fd.twoA.d;
         Tm[S_K1][k] = fd.K1.d;

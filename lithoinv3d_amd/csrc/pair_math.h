@@ -104,13 +104,15 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float x)
 
 // ln(1 + y) for y >= 0.  u = fl(1 + y); u - 1 is exact for 1 <= u < 2^24 and so is
 // c = y - (u - 1), the rounding error of u; ln(1 + y) = ln u + ln(1 + c/u) ~ ln u + c/u.
-// For u < 2 replacing c/u by c costs at most |c| y (relative 6e-8); for u >= 2 the term is
-// below 1e-7 of ln u and dropped.  ln u from the hardware base-2 log (1 ulp).
+// c/u is replaced by c * sat(2 - u) (sat clamps to [0, 1], the instruction's clamp bit):
+// for 1 <= u < 2, |1/u - (2 - u)| = (u - 1)^2 / u <= 1/2, so the correction is off by at
+// most |c|/2 (relative 3e-8); for u >= 2 the term is below 1e-7 of ln u and dropped.
+// ln u from the hardware base-2 log (1 ulp).
 __device__ __forceinline__ float log1p_t(float y)
 {
     const float u = 1.0f + y;
     const float c = y - (u - 1.0f);
-    return fmaf(0.693147180559945f, fast_log2(u), u < 2.0f ? c : 0.0f);
+    return fmaf(0.693147180559945f, fast_log2(u), c * __builtin_amdgcn_fmed3f(2.0f - u, 0.0f, 1.0f));
 }
 __device__ __forceinline__ double log1p_t(double y) { return log1p(y); }
 
