@@ -20,14 +20,15 @@ TRANSCENDENTAL_FLOP_EQ = 8
 OPS = {
     # forward, gz + TMI fused, FP32C: FP64 station offset (3 adds), frame projection,
     # squared distances, solid-angle N and D, atan2 polynomial, edge (s, dperp, rho^2),
-    # three cancellation-free edge logs (ln(1+y) with the rounding correction of 1+y),
-    # gz and TMI combine with Kahan sums; 3 sqrt + 4 rcp + 3 log2
-    "forward_f32c": dict(fma=50, addmul=58, special=10),
+    # three cancellation-free edge logs (ln(1+y) with the rounding correction of 1+y,
+    # weighted by sat(2 - u)), gz and TMI combine with Kahan sums; 3 sqrt + 4 rcp + 3 log2
+    "forward_f32c": dict(fma=50, addmul=64, special=10),
     # adjoint, gz + TMI seeds: the forward part, coefficient seeds, reverse sweep through
-    # the edge logs, solid angle and distances, accumulation of the 22 facet seeds;
+    # the edge logs, solid angle and distances, accumulation of the 20 facet seeds (one
+    # angle seed per in-plane edge direction, torques by two FMAs each);
     # 3 rsqrt (distances and their reciprocals) + 8 rcp (atan2, two per edge log with its
     # derivative, the atan2 derivative) + 3 log2
-    "adjoint_pairs_f32": dict(fma=103, addmul=150, special=14),
+    "adjoint_pairs_f32": dict(fma=106, addmul=142, special=14),
 }
 
 
