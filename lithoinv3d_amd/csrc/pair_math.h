@@ -136,7 +136,8 @@ __device__ __forceinline__ T edge_ln(T s1, T ell, T d1, T d2, T rho2, T& X1, T& 
     neg2 = s2 < T(0);
     const bool mid = !(pos1 || neg2);
     const T num = mid ? X1 * X2 - rho2 : ell * (X1 + X2);
-    const T den = mid ? rho2 : (pos1 ? X1 : X2) * (d1 + d2);
+    // outside the edge the nearer end has the smaller X (X1 when s1 >= 0, X2 when s2 < 0)
+    const T den = mid ? rho2 : fmin(X1, X2) * (d1 + d2);
     return log1p_t(num * fast_rcp(den));
 }
 
