@@ -120,3 +120,32 @@ def test_empty_and_ragged(oracle):
         gz, tm = ctx.forward()
         g = ctx.adjoint(np.ones(len(st)), np.ones(len(st)))
     assert np.all(gz == 0) and np.all(tm == 0) and np.all(g == 0)
+
+
+def test_degenerate_facets_contribute_nothing(oracle):
+    """A zero-area facet (collinear vertices) and one with a zero-length edge (a repeated
+    vertex) contribute nothing to gz or TMI, as the reference's guards make them
+    (gravity_forward.cpp:19, :33; magnetic_forward.cpp:320-330), and give no gradient to
+    the vertices only they use; the rest of the model matches the oracle as usual."""
+    m, st = small_contact(seed=12, shape=(3, 3, 2))
+    fh = synth.field_unit_vector(-60.0, 0.0)
+    nv = len(m.vertices)
+    extra = np.array([[10.0, 10.0, -5.0], [20.0, 20.0, -5.0], [30.0, 30.0, -5.0]])
+    faces = np.vstack([m.faces, [[nv, nv + 1, nv + 2]], [[nv, nv, nv + 1]]]).astype(m.faces.dtype)
+    m2 = synth.LithoModel(np.vstack([m.vertices, extra]), faces,
+                          np.concatenate([m.front, [0, 0]]).astype(m.front.dtype),
+                          np.concatenate([m.back, [-1, -1]]).astype(m.back.dtype),
+                          m.density, m.susceptibility)
+    gz_o, tm_o = oracle_forward(oracle, m, st, fh)
+    w = np.random.default_rng(3).normal(size=len(st))
+    for prec, tg, tt in (("fp64", 1e-4, 1e-3), ("fp32c", 1e-3, 1e-2)):
+        say(f"degenerate facets {prec}")
+        with ForwardContext(m2, st, fh, 50000.0, precision=prec) as ctx:
+            gz, tm = ctx.forward()
+            g = ctx.adjoint(w, w)
+        with ForwardContext(m, st, fh, 50000.0, precision=prec) as ctx:
+            g0 = ctx.adjoint(w, w)
+        print(f"{prec}: max|dgz| {np.abs(gz - gz_o).max():.2e}, max|dTMI| {np.abs(tm - tm_o).max():.2e}")
+        assert np.abs(gz - gz_o).max() <= tg and np.abs(tm - tm_o).max() <= tt
+        assert np.all(np.isfinite(g)) and np.all(g[nv:] == 0.0)
+        np.testing.assert_allclose(g[:nv], g0, rtol=0, atol=1e-12 * np.abs(g0).max())
