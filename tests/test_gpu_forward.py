@@ -104,7 +104,8 @@ def test_set_vertices_and_properties(c0, oracle):
 
 
 def test_empty_and_ragged(oracle):
-    """No faces -> zeros; a station count that is not a multiple of the block size."""
+    """No faces -> zeros; no stations -> empty data and a zero gradient; a station count
+    that is not a multiple of the block size."""
     m, st = small_contact(seed=11, shape=(3, 3, 2))
     fh = synth.field_unit_vector(-60.0, 0.0)
     st = st[:37]
@@ -120,6 +121,12 @@ def test_empty_and_ragged(oracle):
         gz, tm = ctx.forward()
         g = ctx.adjoint(np.ones(len(st)), np.ones(len(st)))
     assert np.all(gz == 0) and np.all(tm == 0) and np.all(g == 0)
+    say("no stations")
+    with ForwardContext(m, st[:0], fh, 50000.0, precision="fp32c") as ctx:
+        gz, tm = ctx.forward()
+        g = ctx.adjoint(np.zeros(0), np.zeros(0))
+    assert gz.shape == (0,) and tm.shape == (0,)
+    assert g.shape == (len(m.vertices), 3) and np.all(g == 0)
 
 
 def test_degenerate_facets_contribute_nothing(oracle):
