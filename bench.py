@@ -226,6 +226,17 @@ def main():
     log(f"{args.steps} steps in {t2 - t1:.3f}s; kernels: " +
         ", ".join(f"{k} {v[0] / max(v[1], 1):.2f} ms x{v[1]}" for k, v in tim.items() if v[1]))
     fb = ctx.fp64_edges()
+    # the host-buffer entry points (vertices in, data and gradient out through host memory,
+    # as the reference's call sites would use them), after the timed region: PCIe included
+    wg_h, wt_h = wg.cpu().numpy(), wt.cpu().numpy()
+    ctx.sync()
+    t3 = time.perf_counter()
+    for _ in range(2):
+        ctx.set_vertices(m.vertices)
+        ctx.forward()
+        ctx.adjoint(wg_h, wt_h)
+    host_sec = (time.perf_counter() - t3) / 2
+    log(f"host-buffer step {host_sec * 1e3:.1f} ms")
 
     fwd_ms = tim["forward"][0] / max(tim["forward"][1], 1)
     adj_ms = tim["adjoint_pairs"][0] / max(tim["adjoint_pairs"][1], 1)
@@ -325,6 +336,9 @@ def main():
             "step": "facet setup + forward gz,TMI + adjoint dPhi/dxyz",
             "reference_layout_pairs_per_s": n_s * m.reference_face_count() / sec,
         },
+        "host_buffers": {"ms_per_step": host_sec * 1e3, "pairs_per_s": pairs / host_sec,
+                         "note": "lf_set_vertices + lf_forward + lf_adjoint with host arrays "
+                                 "(PCIe copies included), 2 steps after the timed region"},
         "roofline": {
             "bound": "valu",
             "kernel": dom,
