@@ -596,6 +596,14 @@ int li_problem_forward(void* h, const double* params, double* gz, double* tmi)
     return guard([&] {
         auto* P = (Problem*)h;
         const Vec p(params, params + P->model->totalDofCount());
+        if (gz && tmi && P->mag && P->fused && P->useFused) {
+            // both fields from one pass over the station x facet pairs
+            Vec g, t;
+            P->fused->forward(p, g, t);
+            std::memcpy(gz, g.data(), sizeof(double) * g.size());
+            std::memcpy(tmi, t.data(), sizeof(double) * t.size());
+            return;
+        }
         if (gz) {
             const Vec g = P->grav->compute(p);
             std::memcpy(gz, g.data(), sizeof(double) * g.size());
