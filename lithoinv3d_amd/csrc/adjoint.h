@@ -34,12 +34,13 @@
 namespace lfk {
 
 // S_PHI1 / S_PHI2: the in-plane directions (e1u, e1v), (e2u, e2v) of edges 1 and 2 are
-// unit vectors, so only their angle has a seed: (se_u, se_v) . (-e_v, e_u).
-constexpr int kSeeds = 20;
+// unit vectors, so only their angle has a seed: (se_u, se_v) . (-e_v, e_u).  S_K1 is the
+// seed of the gravity coefficient K1 (gz = K1 (sum_e dperp_e L_e + 2 pz th)).
+constexpr int kSeeds = 19;
 enum SeedIdx {
     S_GPX = 0, S_GPY, S_GPZ, S_AXY, S_AXZ, S_AYZ,
     S_L0, S_CU, S_CV, S_PHI1, S_L1, S_PHI2, S_L2, S_TWOA,
-    S_K1, S_K2, S_W0, S_W1, S_W2, S_W3
+    S_K1, S_W0, S_W1, S_W2, S_W3
 };
 
 struct StationW {
@@ -75,11 +76,12 @@ struct AdjointLaunch {
     double* grad;
 };
 
-// ln(Q) of edge_ln and its derivatives with respect to (s1, s2, d1, d2, rho^2) in the
+// ln(Q) of an edge and its derivatives with respect to (s1, s2, d1, d2, rho^2) in the
 // form of each case: Q = X2/X1, X1/X2 or X1 X2/rho^2 with X = |s| + d.  The forms agree
 // where d^2 = s^2 + rho^2, which the frame parametrization keeps, so the chain rule
-// through it is exact whichever form is differentiated.  The same y = Q - 1 as edge_ln,
-// with 1/X1, 1/X2 and the ratio's reciprocal taken from one reciprocal of
+// through it is exact whichever form is differentiated.  Used for edges near the station
+// (z > kZSeries), where 2 atanh(z) would lose accuracy to the 1 - z cancellation next to
+// the edge's line; 1/X1, 1/X2 and the ratio's reciprocal come from one reciprocal of
 // X1 X2 (d1 + d2), and 1/rho^2 from a second.
 template <class T>
 __device__ __forceinline__ T edge_ln_grad(T s1, T ell, T d1, T d2, T rho2, T& gs1, T& gs2,
@@ -109,17 +111,41 @@ __device__ __forceinline__ T edge_ln_grad(T s1, T ell, T d1, T d2, T rho2, T& gs
     return log1p_t(y);
 }
 
+// L = 2 atanh(z) and dL/dz = 2 / (1 - z^2) for an edge with z = ell / (d1 + d2) <= kZSeries:
+// in FP32 by their series (truncation below 2e-9 and 2e-8 relative at z = 0.1: the
+// gradient needs no more), in FP64 from the library functions.
+__device__ __forceinline__ float atanh2_dz(float z, float& g)
+{
+    const float s = z * z;
+    float p = fmaf(s, 1.0f / 7.0f, 1.0f / 5.0f);
+    p = fmaf(s, p, 1.0f / 3.0f);
+    p = fmaf(s, p, 1.0f);
+    float q = fmaf(s, s, s);  // s + s^2
+    q = fmaf(s, q, s);        // s + s^2 + s^3
+    g = fmaf(q, 2.0f, 2.0f);
+    return (z + z) * p;
+}
+__device__ __forceinline__ double atanh2_dz(double z, double& g)
+{
+    g = 2.0 / (1.0 - z * z);
+    return 2.0 * atanh(z);
+}
+
 // One (station, facet) pair: frame-form forward and its reverse sweep for the weights
-// (lam on gz, mu on TMI), adding the pair's 20 facet seeds into acc (see the layout in
-// SeedIdx).  Linear in (lam, mu).
+// (lam on gz, mu on TMI), adding the pair's 19 facet seeds into acc (see the layout in
+// SeedIdx).  Linear in (lam, mu).  Edge logs are L = 2 atanh(z), z = ell / (d1 + d2),
+// which depends on the edge length and the two end distances only; an edge with
+// z > kZSeries (station within about five edge lengths) takes the case form of
+// edge_ln_grad, differentiated through (s1, s2, d1, d2, rho^2), in a branch that is rare
+// for a wave.
 template <class T>
 __device__ __forceinline__ void pair_seeds(const FaceFrame<T>& F, double ox, double oy,
                                            double oz, T lam, T mu, T* acc)
 {
     const T Px = T(ox - F.ax), Py = T(oy - F.ay), Pz = T(oz - F.az);
-    const T px = Px * F.ux + Py * F.uy + Pz * F.uz;
-    const T py = Px * F.vx + Py * F.vy + Pz * F.vz;
-    const T pz = Px * F.nx + Py * F.ny + Pz * F.nz;
+    const T px = Px * F.ux + Py * F.uy + Pz * F.uz + F.du;
+    const T py = Px * F.vx + Py * F.vy + Pz * F.vz + F.dv;
+    const T pz = Px * F.nx + Py * F.ny + Pz * F.nz + F.dn;
     // ---- forward
     const T pz2 = pz * pz;
     const T rho0 = py * py + pz2;
@@ -135,68 +161,94 @@ __device__ __forceinline__ void pair_seeds(const FaceFrame<T>& F, double ox, dou
     const T rac = da2 - (F.cu * px + F.cv * py);
     const T rbc = t * dcx - py * dcy + pz2;
     const T D = da * db * dc + rab * dc + rac * db + rbc * da;
-    const T N = -F.twoA * pz;
+    const T twoA = F.l0 * F.cv;
+    const T N = -twoA * pz;
     const T th = atan2_t(N, D);
     const T dp1 = t * F.e1v + py * F.e1u;
     const T dp2 = dcx * F.e2v - dcy * F.e2u;
     const T s1b = t * F.e1u - py * F.e1v;
     const T s1c = dcx * F.e2u + dcy * F.e2v;
-    const T rho1 = dp1 * dp1 + pz2;
-    const T rho2 = dp2 * dp2 + pz2;
-    T g0s1, g0s2, g0d1, g0d2, g0r, g1s1, g1s2, g1d1, g1d2, g1r, g2s1, g2s2, g2d1, g2d2, g2r;
-    const T lam0 = edge_ln_grad(-px, F.l0, da, db, rho0, g0s1, g0s2, g0d1, g0d2, g0r);
-    const T lam1 = edge_ln_grad(s1b, F.l1, db, dc, rho1, g1s1, g1s2, g1d1, g1d2, g1r);
-    const T lam2 = edge_ln_grad(s1c, F.l2, dc, da, rho2, g2s1, g2s2, g2d1, g2d2, g2r);
+    // edges: z = ell q, q = 1 / (d1 + d2)
+    const T q0 = fast_rcp(da + db), q1 = fast_rcp(db + dc), q2 = fast_rcp(dc + da);
+    const T z0 = F.l0 * q0, z1 = F.l1 * q1, z2 = F.l2 * q2;
+    T g0, g1, g2;
+    T lam0 = atanh2_dz(z0, g0);
+    T lam1 = atanh2_dz(z1, g1);
+    T lam2 = atanh2_dz(z2, g2);
+    const bool n0 = z0 > T(kZSeries), n1 = z1 > T(kZSeries), n2 = z2 > T(kZSeries);
+    const bool any_near = n0 || n1 || n2;
+    if (any_near) {
+        T a1, a2, a3, a4, a5;
+        if (n0) lam0 = edge_ln_grad(-px, F.l0, da, db, rho0, a1, a2, a3, a4, a5);
+        if (n1) lam1 = edge_ln_grad(s1b, F.l1, db, dc, dp1 * dp1 + pz2, a1, a2, a3, a4, a5);
+        if (n2) lam2 = edge_ln_grad(s1c, F.l2, dc, da, dp2 * dp2 + pz2, a1, a2, a3, a4, a5);
+    }
     const T es = py * lam0 + dp1 * lam1 + dp2 * lam2;
+    const T pzth = pz * th;
     // ---- coefficient seeds
-    acc[S_K1] += lam * es;
-    acc[S_K2] += lam * (pz * th);
+    acc[S_K1] += lam * (es + pzth + pzth);
     acc[S_W0] += mu * th;
     acc[S_W1] += mu * lam0;
     acc[S_W2] += mu * lam1;
     acc[S_W3] += mu * lam2;
     // ---- reverse
     const T ses = lam * F.K1;
-    const T sth = lam * F.K2 * pz + mu * F.W0;
-    T spz = lam * F.K2 * th;
+    const T sth = T(2) * ses * pz + mu * F.W0;
+    T spz = T(2) * ses * th;
     const T sl0 = ses * py + mu * F.W1;
     const T sl1 = ses * dp1 + mu * F.W2;
     const T sl2 = ses * dp2 + mu * F.W3;
     T spy = ses * lam0;
     T sdp1 = ses * lam1;
     T sdp2 = ses * lam2;
-    // edge 0: s1 = -px, s2 = t, d1 = da, d2 = db
-    T spx = -sl0 * g0s1;
-    T st_ = sl0 * g0s2;
-    T sda = sl0 * g0d1;
-    T sdb = sl0 * g0d2;
-    T srho0 = sl0 * g0r;
-    // edge 1: s1 = s1b, s2 = s1b + l1, d1 = db, d2 = dc
-    const T ss1b = sl1 * (g1s1 + g1s2);
-    const T sL1 = sl1 * g1s2;
-    sdb += sl1 * g1d1;
-    T sdc = sl1 * g1d2;
-    const T srho1 = sl1 * g1r;
-    // edge 2: s1 = s1c, s2 = s1c + l2, d1 = dc, d2 = da
-    const T ss1c = sl2 * (g2s1 + g2s2);
-    const T sL2 = sl2 * g2s2;
-    sdc += sl2 * g2d1;
-    sda += sl2 * g2d2;
-    const T srho2 = sl2 * g2r;
-    // rho1, rho2
-    sdp1 += T(2) * dp1 * srho1;
-    sdp2 += T(2) * dp2 * srho2;
-    T spz2 = srho1 + srho2;
+    // far edges: L(ell, d1, d2); w = sL dL/dz q, dz/dell = q, dz/dd = -z q
+    const T w0 = n0 ? T(0) : sl0 * g0 * q0;
+    const T w1 = n1 ? T(0) : sl1 * g1 * q1;
+    const T w2 = n2 ? T(0) : sl2 * g2 * q2;
+    const T wz0 = w0 * z0, wz1 = w1 * z1, wz2 = w2 * z2;
+    T sda = -(wz0 + wz2);
+    T sdb = -(wz0 + wz1);
+    T sdc = -(wz1 + wz2);
+    T sL0 = w0, sL1 = w1, sL2 = w2;
+    T spx = T(0), st_ = T(0), srho0 = T(0), spz2 = T(0);
+    T ss1b = T(0), ss1c = T(0);
+    if (any_near) {
+        T gs1, gs2, gd1, gd2, gr;
+        if (n0) {
+            // s1 = -px, s2 = t, d1 = da, d2 = db
+            edge_ln_grad(-px, F.l0, da, db, rho0, gs1, gs2, gd1, gd2, gr);
+            spx -= sl0 * gs1;
+            st_ += sl0 * gs2;
+            sda += sl0 * gd1;
+            sdb += sl0 * gd2;
+            srho0 += sl0 * gr;
+        }
+        if (n1) {
+            // s1 = s1b, s2 = s1b + l1, d1 = db, d2 = dc, rho^2 = dp1^2 + pz^2
+            edge_ln_grad(s1b, F.l1, db, dc, dp1 * dp1 + pz2, gs1, gs2, gd1, gd2, gr);
+            ss1b += sl1 * (gs1 + gs2);
+            sL1 += sl1 * gs2;
+            sdb += sl1 * gd1;
+            sdc += sl1 * gd2;
+            sdp1 += T(2) * dp1 * (sl1 * gr);
+            spz2 += sl1 * gr;
+        }
+        if (n2) {
+            edge_ln_grad(s1c, F.l2, dc, da, dp2 * dp2 + pz2, gs1, gs2, gd1, gd2, gr);
+            ss1c += sl2 * (gs1 + gs2);
+            sL2 += sl2 * gs2;
+            sdc += sl2 * gd1;
+            sda += sl2 * gd2;
+            sdp2 += T(2) * dp2 * (sl2 * gr);
+            spz2 += sl2 * gr;
+        }
+    }
     // s1b = t e1u - py e1v, dp1 = t e1v + py e1u
-    st_ += ss1b * F.e1u;
-    spy -= ss1b * F.e1v;
-    st_ += sdp1 * F.e1v;
-    spy += sdp1 * F.e1u;
+    st_ += ss1b * F.e1u + sdp1 * F.e1v;
+    spy += sdp1 * F.e1u - ss1b * F.e1v;
     // s1c = dcx e2u + dcy e2v, dp2 = dcx e2v - dcy e2u
-    T sdcx = ss1c * F.e2u;
-    T sdcy = ss1c * F.e2v;
-    sdcx += sdp2 * F.e2v;
-    sdcy -= sdp2 * F.e2u;
+    T sdcx = ss1c * F.e2u + sdp2 * F.e2v;
+    T sdcy = ss1c * F.e2v - sdp2 * F.e2u;
     // edge angles: (s_e1u, s_e1v) = ss1b (t, -py) + sdp1 (py, t) projected on (-e1v, e1u)
     // is sdp1 s1b - ss1b dp1; likewise sdp2 s1c - ss1c dp2 for edge 2
     acc[S_PHI1] = fma(sdp1, s1b, acc[S_PHI1]);
@@ -208,7 +260,7 @@ __device__ __forceinline__ void pair_seeds(const FaceFrame<T>& F, double ox, dou
     const T sN = sth * D * q;
     const T sD = -sth * N * q;
     const T stwoA = -sN * pz;
-    spz -= sN * F.twoA;
+    spz -= sN * twoA;
     // D = da db dc + rab dc + rac db + rbc da
     sda += sD * (db * dc + rbc);
     sdb += sD * (da * dc + rac);
@@ -261,7 +313,7 @@ __device__ __forceinline__ void pair_seeds(const FaceFrame<T>& F, double ox, dou
     acc[S_AXY] = fma(-spy, px, fma(spx, py, acc[S_AXY]));
     acc[S_AXZ] = fma(-spz, px, fma(spx, pz, acc[S_AXZ]));
     acc[S_AYZ] = fma(-spz, py, fma(spy, pz, acc[S_AYZ]));
-    acc[S_L0] += sl0c;
+    acc[S_L0] += sl0c + sL0;
     acc[S_CU] += scu;
     acc[S_CV] += scv;
     acc[S_L1] += sL1;
@@ -269,67 +321,110 @@ __device__ __forceinline__ void pair_seeds(const FaceFrame<T>& F, double ox, dou
     acc[S_TWOA] += stwoA;
 }
 
-// Optional occupancy target for the pair kernel (waves per SIMD), a build-time tuning knob.
-#ifdef LF_ADJOINT_WAVES_PER_EU
-#define LF_ADJOINT_OCC __attribute__((amdgpu_waves_per_eu(LF_ADJOINT_WAVES_PER_EU)))
-#else
-#define LF_ADJOINT_OCC
+// Seed sums.  The per-pair seeds of one facet over the stations of a chunk have random
+// signs (station weights are residuals), so an FP32 running sum of n of them carries a
+// relative error growing like sqrt(n) eps, and the finalize cancels the seeds against
+// each other by about R/L (a far facet's edge and area terms of size L/R combine into a
+// vertex sensitivity of size (L/R)^2).  The FP32 sums therefore run over at most
+// kFlushStations stations and are then added into FP64 sums, one LDS column per thread
+// (19 x 256 doubles, 38 KiB per block), written to the chunk's seeds at the end.
+#ifndef LF_ADJOINT_FLUSH
+#define LF_ADJOINT_FLUSH 32
+#endif
+constexpr int kFlushStations = LF_ADJOINT_FLUSH;
+constexpr int kAdjointBlock = 256;
+
+// FP32 pair kernel occupancy (waves per SIMD): the compiler's own choice for this loop
+// (about 180 VGPRs) leaves 2; asking for more trades a few registers for issue slots.
+#ifndef LF_ADJOINT_WAVES_PER_EU
+#define LF_ADJOINT_WAVES_PER_EU 4
 #endif
 
 template <class T>
-__global__ __launch_bounds__(256) LF_ADJOINT_OCC void k_adjoint_pairs(const FaceFrame<T>* __restrict__ faces,
-                                                       int64_t n_f,
-                                                       const StationW* __restrict__ st,
-                                                       int64_t n_s, int64_t st_per_chunk,
-                                                       T* __restrict__ seeds)
+__device__ __forceinline__ void adjoint_pairs_body(const FaceFrame<T>* __restrict__ faces,
+                                                   int64_t n_f, const StationW* __restrict__ st,
+                                                   int64_t n_s, int64_t st_per_chunk,
+                                                   double* __restrict__ seeds,
+                                                   double (*sum64)[kAdjointBlock])
 {
     const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int chunk = blockIdx.y;
     const bool active = f < n_f;
     const FaceFrame<T> F = faces[active ? f : n_f - 1];
-    T acc[kSeeds];
-#pragma unroll
-    for (int k = 0; k < kSeeds; ++k) acc[k] = T(0);
     const int64_t i0 = (int64_t)chunk * st_per_chunk;
     const int64_t i1 = min(n_s, i0 + st_per_chunk);
-    // a 32-bit count keeps the loop test on the scalar unit (it has no 64-bit less-than)
+    // 32-bit counts keep the loop tests on the scalar unit (it has no 64-bit less-than)
     const StationW* __restrict__ sp = st + (i0 < i1 ? i0 : 0);
     const int cnt = i0 < i1 ? (int)(i1 - i0) : 0;
-    // the next station record is requested (scalar loads) while this one is used
-    StationW Sn = sp[0];
-    for (int i = 0; i < cnt; ++i) {
-        const StationW S = Sn;
-        Sn = sp[i + 1 < cnt ? i + 1 : i];
-        if (S.skip) continue;
-        // T(w) from the scalar FP64 weight lands in a vector register, where the ~12 products
-        // that use it issue faster than with a scalar-register operand
-        pair_seeds(F, S.x, S.y, S.z, T(S.wg), T(S.wt), acc);
+    // each thread reads and writes only its own column: no barrier needed
+#pragma unroll
+    for (int k = 0; k < kSeeds; ++k) sum64[k][threadIdx.x] = 0.0;
+    for (int b0 = 0; b0 < cnt; b0 += kFlushStations) {
+        const int b1 = min(cnt, b0 + kFlushStations);
+        T acc[kSeeds];
+#pragma unroll
+        for (int k = 0; k < kSeeds; ++k) acc[k] = T(0);
+        // the next station record is requested (scalar loads) while this one is used
+        StationW Sn = sp[b0];
+        for (int i = b0; i < b1; ++i) {
+            const StationW S = Sn;
+            Sn = sp[i + 1 < b1 ? i + 1 : i];
+            if (S.skip) continue;
+            // T(w) from the scalar FP64 weight lands in a vector register, where the
+            // products that use it issue faster than with a scalar-register operand
+            pair_seeds(F, S.x, S.y, S.z, T(S.wg), T(S.wt), acc);
+        }
+#pragma unroll
+        for (int k = 0; k < kSeeds; ++k) sum64[k][threadIdx.x] += (double)acc[k];
     }
     if (active) {
+        double* __restrict__ out = seeds + (int64_t)chunk * kSeeds * n_f + f;
 #pragma unroll
-        for (int k = 0; k < kSeeds; ++k)
-            seeds[((int64_t)chunk * kSeeds + k) * n_f + f] = acc[k];
+        for (int k = 0; k < kSeeds; ++k) out[(int64_t)k * n_f] = sum64[k][threadIdx.x];
     }
 }
 
-// Per-facet derivative matrix T (20 x 9): the derivative of the facet's contribution
-// with respect to vertex coordinate k (a.x, a.y, a.z, b.x, ..., c.z) is
-// sum_m seed_m * T[m][k] for the per-pair (or accumulated) seeds of SeedIdx.  Built
-// from the facet constants with forward-mode duals along the nine directions.
-// Returns false for a degenerate facet (T = 0).
-__device__ __forceinline__ bool facet_T(const double P[3][3], double drho, const double Md[3],
+__global__ __launch_bounds__(kAdjointBlock) __attribute__((amdgpu_waves_per_eu(LF_ADJOINT_WAVES_PER_EU)))
+void k_adjoint_pairs_f32(const FaceFrame<float>* __restrict__ faces, int64_t n_f,
+                         const StationW* __restrict__ st, int64_t n_s, int64_t st_per_chunk,
+                         double* __restrict__ seeds)
+{
+    __shared__ double sum64[kSeeds][kAdjointBlock];
+    adjoint_pairs_body<float>(faces, n_f, st, n_s, st_per_chunk, seeds, sum64);
+}
+
+__global__ __launch_bounds__(kAdjointBlock) void k_adjoint_pairs_f64(
+    const FaceFrame<double>* __restrict__ faces, int64_t n_f, const StationW* __restrict__ st,
+    int64_t n_s, int64_t st_per_chunk, double* __restrict__ seeds)
+{
+    __shared__ double sum64[kSeeds][kAdjointBlock];
+    adjoint_pairs_body<double>(faces, n_f, st, n_s, st_per_chunk, seeds, sum64);
+}
+
+// Per-facet derivative matrix T (19 x 9): the derivative of the facet's contribution
+// with respect to vertex coordinate k (a.x, a.y, a.z, b.x, ..., c.z of the facet's own
+// corner order) is sum_m seed_m * T[m][k] for the per-pair (or accumulated) seeds of
+// SeedIdx.  Built from the facet constants with forward-mode duals along the nine
+// directions, in the corner order of the facet's frame record (frame_rotation), with the
+// columns returned in the facet's own order.  Returns false for a degenerate facet (T = 0).
+__device__ __forceinline__ bool facet_T(const double P0[3][3], double drho, const double Md[3],
                                         const double Ms[3], const double Fd[3], bool textbook,
                                         double Tm[kSeeds][9])
 {
+    const int rot = frame_rotation(P0);
+    double P[3][3];
+    rotate_corners(P0, rot, P);
     FrameVals<double> fv;
     facet_frame(P, drho, Md, Ms, Fd, textbook, fv);
     for (int m = 0; m < kSeeds; ++m)
         for (int k = 0; k < 9; ++k) Tm[m][k] = 0.0;
     if (fv.bad) return false;
-    for (int k = 0; k < 9; ++k) {
+    for (int kr = 0; kr < 9; ++kr) {
+        // direction kr of the record's corners is column k of the facet's corners
+        const int k = 3 * ((kr / 3 + rot) % 3) + kr % 3;
         Dual Q[3][3];
         for (int c = 0; c < 3; ++c)
-            for (int j = 0; j < 3; ++j) Q[c][j] = Dual(P[c][j], (c * 3 + j == k) ? 1.0 : 0.0);
+            for (int j = 0; j < 3; ++j) Q[c][j] = Dual(P[c][j], (c * 3 + j == kr) ? 1.0 : 0.0);
         FrameVals<Dual> fd;
         facet_frame(Q, drho, Md, Ms, Fd, textbook, fd);
         double wxy = 0, wxz = 0, wyz = 0;
@@ -338,11 +433,11 @@ __device__ __forceinline__ bool facet_T(const double P[3][3], double drho, const
             wxz += fd.u[j].d * fv.n[j];
             wyz += fd.v[j].d * fv.n[j];
         }
-        if (k < 3) {
+        if (kr < 3) {
             // p = R (o - a): moving a moves p by -R da
-            Tm[S_GPX][k] = -fv.u[k];
-            Tm[S_GPY][k] = -fv.v[k];
-            Tm[S_GPZ][k] = -fv.n[k];
+            Tm[S_GPX][k] = -fv.u[kr];
+            Tm[S_GPY][k] = -fv.v[kr];
+            Tm[S_GPZ][k] = -fv.n[kr];
         }
         Tm[S_AXY][k] = wxy;
         Tm[S_AXZ][k] = wxz;
@@ -357,7 +452,6 @@ __device__ __forceinline__ bool facet_T(const double P[3][3], double drho, const
         Tm[S_L2][k] = fd.l2.d;
         Tm[S_TWOA][k] = fd.twoA.d;
         Tm[S_K1][k] = fd.K1.d;
-        Tm[S_K2][k] = fd.K2.d;
         Tm[S_W0][k] = fd.W0.d;
         Tm[S_W1][k] = fd.W1.d;
         Tm[S_W2][k] = fd.W2.d;
@@ -387,10 +481,9 @@ __device__ __forceinline__ void load_facet(int64_t f, const double* __restrict__
     }
 }
 
-// Per facet: sum the chunk seeds in FP64, then apply the facet's derivative matrix.
-template <class T>
+// Per facet: sum the chunks' FP64 seeds, then apply the facet's derivative matrix.
 __global__ __launch_bounds__(128) void k_adjoint_finalize(
-    const T* __restrict__ seeds, int n_chunks, int64_t n_f, const double* __restrict__ verts,
+    const double* __restrict__ seeds, int n_chunks, int64_t n_f, const double* __restrict__ verts,
     const uint32_t* __restrict__ tri, const int32_t* __restrict__ front,
     const int32_t* __restrict__ back, const double* __restrict__ density,
     const double* __restrict__ mag, double fx, double fy, double fz, int tmi_mode,
@@ -401,7 +494,7 @@ __global__ __launch_bounds__(128) void k_adjoint_finalize(
     double s[kSeeds];
     for (int k = 0; k < kSeeds; ++k) {
         double a = 0.0;
-        for (int c = 0; c < n_chunks; ++c) a += (double)seeds[((int64_t)c * kSeeds + k) * n_f + f];
+        for (int c = 0; c < n_chunks; ++c) a += seeds[((int64_t)c * kSeeds + k) * n_f + f];
         s[k] = a;
     }
     double P[3][3], drho, Md[3], Ms[3];
@@ -570,22 +663,19 @@ inline int adjoint_init(AdjointWork& w, int64_t n_v, int64_t n_f, int64_t n_s,
     w.n_f = n_f;
     w.n_s = n_s;
     w.precision = precision;
-    // station chunks: enough blocks to fill the GPU, and at most 2048 stations per
-    // chunk so the per-chunk FP32 sums stay short
+    // station chunks: enough blocks (facet blocks x chunks, 65536 or more) that the
+    // last partial round of resident blocks costs little, at least 16 stations per chunk,
+    // and at most 8 GiB of FP64 seeds (a seed buffer of one chunk always fits)
     const int64_t face_blocks = std::max<int64_t>(1, (n_f + 255) / 256);
-    int64_t chunks = std::max<int64_t>(1, (4096 + face_blocks - 1) / face_blocks);
-    chunks = std::max<int64_t>(chunks, (n_s + 2047) / 2048);
+    int64_t chunks = std::max<int64_t>(1, (65536 + face_blocks - 1) / face_blocks);
     chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, (n_s + 15) / 16));
     chunks = std::min<int64_t>(chunks, 65535);
-    // keep the seed buffer under 4 GiB: fewer, longer station chunks on large models
-    const size_t seed_bytes = (size_t)kSeeds * std::max<int64_t>(n_f, 1) *
-                              (precision == 1 ? sizeof(float) : sizeof(double));
-    chunks = std::max<int64_t>(1, std::min<int64_t>(chunks, (int64_t)((4ull << 30) / seed_bytes)));
+    const size_t seed_bytes = (size_t)kSeeds * std::max<int64_t>(n_f, 1) * sizeof(double);
+    chunks = std::max<int64_t>(1, std::min<int64_t>(chunks, (int64_t)((8ull << 30) / seed_bytes)));
     w.st_per_chunk = std::max<int64_t>(1, (n_s + chunks - 1) / chunks);
     w.n_chunks = (int)std::max<int64_t>(1, (n_s + w.st_per_chunk - 1) / w.st_per_chunk);
-    const size_t esz = precision == 1 ? sizeof(float) : sizeof(double);
     const size_t nseed = (size_t)w.n_chunks * kSeeds * std::max<int64_t>(n_f, 1);
-    if (hipMalloc(&w.d_seeds, nseed * esz) != hipSuccess) return -1;
+    if (hipMalloc(&w.d_seeds, nseed * sizeof(double)) != hipSuccess) return -1;
     if (hipMalloc((void**)&w.d_st, sizeof(StationW) * std::max<int64_t>(n_s, 1)) != hipSuccess) return -1;
     if (hipMalloc((void**)&w.d_face_grad, sizeof(double) * 9 * std::max<int64_t>(n_f, 1)) != hipSuccess) return -1;
     if (hipMalloc((void**)&w.d_csr_off, sizeof(int64_t) * (n_v + 1)) != hipSuccess) return -1;
@@ -624,12 +714,12 @@ inline hipError_t adjoint_pairs(AdjointWork& w, const AdjointLaunch& L)
     const double* wt = L.wt ? L.wt : w.d_zero;
     hipLaunchKernelGGL(k_pack_stations, dim3((unsigned)((L.n_s + 255) / 256)), dim3(256), 0,
                        L.stream, L.sx, L.sy, L.sz, wg, wt, L.n_s, w.d_st);
-    const dim3 grid((unsigned)((L.n_f + 255) / 256), (unsigned)w.n_chunks);
+    const dim3 grid((unsigned)((L.n_f + kAdjointBlock - 1) / kAdjointBlock), (unsigned)w.n_chunks);
     if (L.precision == 1)
-        hipLaunchKernelGGL(k_adjoint_pairs<float>, grid, dim3(256), 0, L.stream, L.f32, L.n_f,
-                           w.d_st, L.n_s, w.st_per_chunk, (float*)w.d_seeds);
+        hipLaunchKernelGGL(k_adjoint_pairs_f32, grid, dim3(kAdjointBlock), 0, L.stream, L.f32, L.n_f,
+                           w.d_st, L.n_s, w.st_per_chunk, (double*)w.d_seeds);
     else
-        hipLaunchKernelGGL(k_adjoint_pairs<double>, grid, dim3(256), 0, L.stream, L.f64, L.n_f,
+        hipLaunchKernelGGL(k_adjoint_pairs_f64, grid, dim3(kAdjointBlock), 0, L.stream, L.f64, L.n_f,
                            w.d_st, L.n_s, w.st_per_chunk, (double*)w.d_seeds);
     return hipGetLastError();
 }
@@ -642,14 +732,9 @@ inline hipError_t adjoint_finalize(AdjointWork& w, const AdjointLaunch& L, const
     if (L.n_f == 0) return hipSuccess;
     const dim3 grid((unsigned)((L.n_f + 127) / 128));
     const int nch = L.n_s == 0 ? 0 : w.n_chunks;
-    if (L.precision == 1)
-        hipLaunchKernelGGL(k_adjoint_finalize<float>, grid, dim3(128), 0, L.stream,
-                           (const float*)w.d_seeds, nch, L.n_f, verts, tri, front, back,
-                           density, mag, fdir[0], fdir[1], fdir[2], tmi_mode, w.d_face_grad);
-    else
-        hipLaunchKernelGGL(k_adjoint_finalize<double>, grid, dim3(128), 0, L.stream,
-                           (const double*)w.d_seeds, nch, L.n_f, verts, tri, front, back,
-                           density, mag, fdir[0], fdir[1], fdir[2], tmi_mode, w.d_face_grad);
+    hipLaunchKernelGGL(k_adjoint_finalize, grid, dim3(128), 0, L.stream,
+                       (const double*)w.d_seeds, nch, L.n_f, verts, tri, front, back, density,
+                       mag, fdir[0], fdir[1], fdir[2], tmi_mode, w.d_face_grad);
     return hipGetLastError();
 }
 

@@ -61,8 +61,17 @@ __global__ __launch_bounds__(128) void k_face_setup(
         Ms[j] = mf + mb;  // the reference edge term does not: t = n x e is reversal-invariant
     }
     const double Fd[3] = {fx, fy, fz};
+    const bool textbook = tmi_mode == LF_TMI_TEXTBOOK;
     FrameVals<double> fv;
-    facet_frame(P, drho, Md, Ms, Fd, tmi_mode == LF_TMI_TEXTBOOK, fv);
+    facet_frame(P, drho, Md, Ms, Fd, textbook, fv);
+    // frame-form records use the corner order that makes the frame exact in FP32 where
+    // the facet allows it (the FP64 reference-form record keeps the facet's own order)
+    const int rot = frame_rotation(P);
+    double Q[3][3];
+    rotate_corners(P, rot, Q);
+    FrameVals<double> fq;
+    if (rot) facet_frame(Q, drho, Md, Ms, Fd, textbook, fq);
+    const FrameVals<double>& fr = rot ? fq : fv;
     if (out64) {
         FaceF64 r;
         for (int j = 0; j < 3; ++j) {
@@ -83,19 +92,17 @@ __global__ __launch_bounds__(128) void k_face_setup(
     }
     if (outfr64) {
         FaceFrame<double> r;
-        store_frame(fv, P[0], r);
+        store_frame(fr, Q[0], r);
         outfr64[f] = r;
     }
     if (out32) {
         FaceF32 r;
-        store_frame(fv, P[0], r);
+        store_frame(fr, Q[0], r, (uint32_t)f, true);
         out32[f] = r;
     }
     if (geo64) {
-        for (int j = 0; j < 3; ++j) {
-            geo64[6 * f + j] = P[1][j];
-            geo64[6 * f + 3 + j] = P[2][j];
-        }
+        for (int c = 0; c < 3; ++c)
+            for (int j = 0; j < 3; ++j) geo64[9 * f + 3 * c + j] = Q[c][j];
     }
 }
 
@@ -130,13 +137,14 @@ __device__ __forceinline__ void forward_pair(const FaceF32& F, const double* __r
                                              float& gs, float& gc, float& ts, float& tc,
                                              unsigned& nfb)
 {
-    // station offset from vertex a: FP64 difference, one rounding
+    // station offset from the record's origin: FP64 difference, one rounding; in the
+    // frame, plus the origin's offset d, it is the offset from vertex a
     const float Px = (float)(ox - F.ax);
     const float Py = (float)(oy - F.ay);
     const float Pz = (float)(oz - F.az);
-    const float px = Px * F.ux + Py * F.uy + Pz * F.uz;
-    const float py = Px * F.vx + Py * F.vy + Pz * F.vz;
-    const float pz = Px * F.nx + Py * F.ny + Pz * F.nz;  // h = -pz
+    const float px = fmaf(Px, F.ux, fmaf(Py, F.uy, fmaf(Pz, F.uz, F.du)));
+    const float py = fmaf(Px, F.vx, fmaf(Py, F.vy, fmaf(Pz, F.vz, F.dv)));
+    const float pz = fmaf(Px, F.nx, fmaf(Py, F.ny, fmaf(Pz, F.nz, F.dn)));  // h = -pz
     const float pz2 = pz * pz;
     const float rho0 = fmaf(py, py, pz2);
     const float da2 = fmaf(px, px, rho0);
@@ -144,44 +152,64 @@ __device__ __forceinline__ void forward_pair(const FaceF32& F, const double* __r
     const float db2 = fmaf(t, t, rho0);
     const float dcx = F.cu - px, dcy = F.cv - py;
     const float dc2 = fmaf(dcx, dcx, fmaf(dcy, dcy, pz2));
-    const float da = fast_sqrt(da2), db = fast_sqrt(db2), dc = fast_sqrt(dc2);
+    const float da = sqrt_nr(da2), db = sqrt_nr(db2), dc = sqrt_nr(dc2);
     // solid angle: N = r_a.(r_b x r_c) = 2A*h, D from the pairwise dots
     const float rab = fmaf(-F.l0, px, da2);
     const float rac = da2 - fmaf(F.cu, px, F.cv * py);
     const float rbc = fmaf(t, dcx, fmaf(-py, dcy, pz2));
     const float D = fmaf(da * db, dc, fmaf(rab, dc, fmaf(rac, db, rbc * da)));
-    const float N = -F.twoA * pz;
+    const float N = -(F.l0 * F.cv) * pz;
     const float th = atan2_t(N, D);  // Omega = 2*th
-    // edges: (s1, s2, d1, d2, dperp, rho^2)
+    // edge logs 2 atanh(ell / (d1 + d2)): the series for every edge more than about 2.5
+    // edge lengths away; the few nearer ones from (s1, rho^2) of the edge
+    const float z0 = F.l0 * rcp_nr(da + db);
+    const float z1 = F.l1 * rcp_nr(db + dc);
+    const float z2 = F.l2 * rcp_nr(dc + da);
+    float lam0 = atanh2_series(z0);
+    float lam1 = atanh2_series(z1);
+    float lam2 = atanh2_series(z2);
+    // in-plane distances of the station from the lines of edges 1 and 2 (gz weights)
     const float dp1 = fmaf(t, F.e1v, py * F.e1u);
     const float dp2 = fmaf(dcx, F.e2v, -dcy * F.e2u);
-    const float s1b = fmaf(t, F.e1u, -py * F.e1v);
-    const float s1c = fmaf(dcx, F.e2u, dcy * F.e2v);
-    const float rho1 = fmaf(dp1, dp1, pz2);
-    const float rho2 = fmaf(dp2, dp2, pz2);
-    float X1, X2;
-    bool p1, n2;
-    float lam0 = edge_ln<float>(-px, F.l0, da, db, rho0, X1, X2, p1, n2);
-    float lam1 = edge_ln<float>(s1b, F.l1, db, dc, rho1, X1, X2, p1, n2);
-    float lam2 = edge_ln<float>(s1c, F.l2, dc, da, rho2, X1, X2, p1, n2);
-    // FP64 for edges whose line passes close to the station while the station
-    // projects inside the edge (rho < sqrt(d1 d2)/32): there the FP32 offsets'
-    // absolute error would dominate the log.  Only possible near the facet plane.
-    if (fabsf(pz) < F.zfb) {
-        const bool fb0 = (-px < 0.f) && (t >= 0.f) && (rho0 * 1024.f < da * db);
-        const bool fb1 = (s1b < 0.f) && (s1b + F.l1 >= 0.f) && (rho1 * 1024.f < db * dc);
-        const bool fb2 = (s1c < 0.f) && (s1c + F.l2 >= 0.f) && (rho2 * 1024.f < dc * da);
-        if (fb0 | fb1 | fb2) {
-            const double* g = geo64 + 6 * f;
-            if (fb0) lam0 = edge_ln_f64(ox, oy, oz, F.ax, F.ay, F.az, g[0], g[1], g[2]);
-            if (fb1) lam1 = edge_ln_f64(ox, oy, oz, g[0], g[1], g[2], g[3], g[4], g[5]);
-            if (fb2) lam2 = edge_ln_f64(ox, oy, oz, g[3], g[4], g[5], F.ax, F.ay, F.az);
-            nfb += (unsigned)fb0 + (unsigned)fb1 + (unsigned)fb2;
+    if (fmaxf(z0, fmaxf(z1, z2)) > kZSeries) {
+        const float s1b = fmaf(t, F.e1u, -py * F.e1v);
+        const float s1c = fmaf(dcx, F.e2u, dcy * F.e2v);
+        const float rho1 = fmaf(dp1, dp1, pz2);
+        const float rho2 = fmaf(dp2, dp2, pz2);
+        float num, den;
+        if (z0 > kZSeries) {
+            edge_ratio(-px, F.l0, da, db, rho0, num, den);
+            lam0 = ln_ratio_near(num, den);
+        }
+        if (z1 > kZSeries) {
+            edge_ratio(s1b, F.l1, db, dc, rho1, num, den);
+            lam1 = ln_ratio_near(num, den);
+        }
+        if (z2 > kZSeries) {
+            edge_ratio(s1c, F.l2, dc, da, rho2, num, den);
+            lam2 = ln_ratio_near(num, den);
+        }
+        // FP64 for edges whose line passes close to the station while the station
+        // projects inside the edge (rho < sqrt(d1 d2)/32): there the FP32 offsets'
+        // absolute error would dominate the log.  Only possible near the facet plane
+        // (a straddling edge needs rho^2 * 1024 < d1 d2 <= ((len + 2 rho)/2)^2, i.e.
+        // rho < len/62, and rho >= |pz|), where the edge's z is above kZSeries.
+        if (fabsf(pz) * 60.f < fmaxf(F.l0, fmaxf(F.l1, F.l2))) {
+            const bool fb0 = (-px < 0.f) && (t >= 0.f) && (rho0 * 1024.f < da * db);
+            const bool fb1 = (s1b < 0.f) && (s1b + F.l1 >= 0.f) && (rho1 * 1024.f < db * dc);
+            const bool fb2 = (s1c < 0.f) && (s1c + F.l2 >= 0.f) && (rho2 * 1024.f < dc * da);
+            if (fb0 | fb1 | fb2) {
+                const double* g = geo64 + 9 * f;  // the record's corners a, b, c in FP64
+                if (fb0) lam0 = edge_ln_f64(ox, oy, oz, g[0], g[1], g[2], g[3], g[4], g[5]);
+                if (fb1) lam1 = edge_ln_f64(ox, oy, oz, g[3], g[4], g[5], g[6], g[7], g[8]);
+                if (fb2) lam2 = edge_ln_f64(ox, oy, oz, g[6], g[7], g[8], g[0], g[1], g[2]);
+                nfb += (unsigned)fb0 + (unsigned)fb1 + (unsigned)fb2;
+            }
         }
     }
     if (GZ) {
         const float es = fmaf(py, lam0, fmaf(dp1, lam1, dp2 * lam2));
-        kahan_add(gs, gc, fmaf(F.K1, es, F.K2 * (pz * th)));
+        kahan_add(gs, gc, F.K1 * fmaf(pz + pz, th, es));
     }
     if (TMI) {
         kahan_add(ts, tc, fmaf(F.W0, th, fmaf(F.W1, lam0, fmaf(F.W2, lam1, F.W3 * lam2))));
@@ -654,7 +682,7 @@ int lf_create(lf_ctx** out, const lf_desc* d)
     CR(dalloc(&c->d_sz, c->n_s));
     if (c->precision == LF_FP32C) {
         CR(dalloc(&c->d_f32, c->n_f));
-        CR(dalloc(&c->d_geo64, 6 * c->n_f));
+        CR(dalloc(&c->d_geo64, 9 * c->n_f));
     } else {
         CR(dalloc(&c->d_f64, c->n_f));
         CR(dalloc(&c->d_fr64, c->n_f));

@@ -27,18 +27,19 @@ namespace lfk {
 
 // ---------------------------------------------------------------------------- records
 
-// Frame-form facet record.  FaceFrame<float> is 128 bytes and is read with wave-uniform
-// (scalar) loads by the FP32C forward.
+// Frame-form facet record.  FaceFrame<float> is 128 bytes; the FP32C forward streams it
+// through LDS.  The station offset in the frame is p = R (o - a) = R (o - a') + d: the
+// record's origin a' is vertex a moved by a small per-facet offset (FP32C records; a' = a
+// for FP64 ones) and d = R (a' - a) puts it back, so p is the offset from a itself.
 template <class T>
 struct __attribute__((aligned(16))) FaceFrame {
-    double ax, ay, az;                         // vertex a, FP64
+    double ax, ay, az;                         // origin a', FP64
     T ux, uy, uz, vx, vy, vz, nx, ny, nz;      // facet frame
-    T l0, cu, cv;                              // |b-a|; (c-a) in the frame
+    T l0, cu, cv;                              // |b-a|; (c-a) in the frame (2A = l0 cv)
     T e1u, e1v, l1, e2u, e2v, l2;              // unit edges b->c, c->a in the frame; lengths
-    T K1, K2;                                  // gz  = K1*sum_e dperp_e*lnQ_e + K2*pz*atan2(N,D)
+    T K1;                                      // gz  = K1*(sum_e dperp_e*lnQ_e + 2 pz atan2(N,D))
     T W0, W1, W2, W3;                          // tmi = W0*atan2(N,D) + sum_e W_e*lnQ_e
-    T twoA;                                    // 2*area = l0*cv
-    T zfb;  // max edge length / 60: no edge can need the FP64 branch when |pz| >= zfb
+    T du, dv, dn;                              // R (a' - a)
 };
 using FaceF32 = FaceFrame<float>;
 static_assert(sizeof(FaceF32) == 128, "FaceF32 must be 128 bytes");
@@ -64,6 +65,26 @@ __device__ __forceinline__ double fast_rcp(double x) { return 1.0 / x; }
 __device__ __forceinline__ double fast_sqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ double fast_log2(double x) { return log2(x); }
 __device__ __forceinline__ double fast_rsqrt(double x) { return 1.0 / sqrt(x); }
+
+// The hardware FP32 reciprocal, square root and log2 are faithful (within 1 ulp) but not
+// unbiased: measured over every input of a binade on gfx950 their mean errors are about
+// -0.07 ulp (rcp), -0.15 ulp (sqrt), -0.09 ulp (rsq) and -0.2 to -0.4 ulp (log2).  A sum of
+// millions of same-signed terms (the reference TMI formula's edge terms all share one sign)
+// turns such a bias into an error of the total at that relative size, so the FP32C forward
+// uses the forms below: one Newton step after the hardware estimate leaves only the
+// rounding of the final fma, which is unbiased.
+__device__ __forceinline__ float sqrt_nr(float x)
+{
+    // x >= 0; x = 0 gives 0 (the clamp keeps rsq finite)
+    const float rs = fast_rsqrt(fmaxf(x, 1e-30f));
+    const float d = x * rs;
+    return fmaf(fmaf(-d, d, x), 0.5f * rs, d);
+}
+__device__ __forceinline__ float rcp_nr(float x)
+{
+    const float r = fast_rcp(x);
+    return fmaf(r, fmaf(-x, r, 1.0f), r);
+}
 
 // atan2 in FP32: argument reduction to a = min(|x|,|y|)/max(|x|,|y|) in [0, 1], odd
 // minimax polynomial atan(a) ~= a*P(a^2) of degree 7 in a^2 (max abs error 3.7e-8 rad;
@@ -123,22 +144,49 @@ __device__ __forceinline__ double log1p_t(double y) { return log1p(y); }
 //   s2 <  0      : Q = X1/X2 (identity (s+d)(d-s) = rho^2 at both ends),
 //                  Q - 1 = (ell - dd)/X2 = ell (X1 + X2) / (X2 (d1 + d2))
 //   s1 < 0 <= s2 : Q = X1 X2 / rho^2 (station projects inside the edge)
-// so every case is ln(1 + y) with y >= 0 formed from positive products and one reciprocal.
-// Also returns X1, X2 and the case flags for the derivative.
-template <class T>
-__device__ __forceinline__ T edge_ln(T s1, T ell, T d1, T d2, T rho2, T& X1, T& X2, bool& pos1,
-                                     bool& neg2)
+// so every case is ln(1 + y) with y >= 0 formed from positive products.
+
+// Numerator and denominator of Q - 1 = num / den for the forward's edge logarithm, with
+// the middle case in a form without cancellation: from (|s| + d)(d - |s|) = rho^2 and
+// d2 - d1 = ell (s1 + s2) / (d1 + d2),
+//   X1 X2 - rho^2 = ell (X1 X2 + rho^2) / (d1 + d2),
+// so every case is num = ell * A, den = (d1 + d2) * B with A, B sums or products of
+// positive numbers:
+//   outside the edge : A = X1 + X2,           B = min(X1, X2)
+//   inside           : A = X1 X2 + rho^2,     B = rho^2
+__device__ __forceinline__ void edge_ratio(float s1, float ell, float d1, float d2, float rho2,
+                                           float& num, float& den)
 {
-    const T s2 = s1 + ell;
-    X1 = fabs(s1) + d1;
-    X2 = fabs(s2) + d2;
-    pos1 = s1 >= T(0);
-    neg2 = s2 < T(0);
-    const bool mid = !(pos1 || neg2);
-    const T num = mid ? X1 * X2 - rho2 : ell * (X1 + X2);
-    // outside the edge the nearer end has the smaller X (X1 when s1 >= 0, X2 when s2 < 0)
-    const T den = mid ? rho2 : fmin(X1, X2) * (d1 + d2);
-    return log1p_t(num * fast_rcp(den));
+    const float s2 = s1 + ell;
+    const float X1 = fabsf(s1) + d1;
+    const float X2 = fabsf(s2) + d2;
+    const bool mid = (s1 < 0.f) && (s2 >= 0.f);
+    const float A = mid ? fmaf(X1, X2, rho2) : X1 + X2;
+    const float B = mid ? rho2 : fminf(X1, X2);
+    num = ell * A;
+    den = (d1 + d2) * B;
+}
+
+// Edge log from z = ell / (d1 + d2): in all three cases above Q = (d1 + d2 + ell) /
+// (d1 + d2 - ell), i.e. ln Q = 2 atanh(z) (from X1 X2 - rho^2 = ell (X1 X2 + rho^2) /
+// (d1 + d2) and the end-case identities).  For z <= kZSeries (the station more than about
+// five edge lengths from the edge) the series 2 z (1 + z^2/3 + ... + z^8/9) (truncation
+// below 1e-11 relative at z = 0.1) uses only fmas, so no biased hardware transcendental
+// reaches the result; nearer edges take ln_ratio_near, whose middle case has no 1 - z
+// cancellation for a station close to the edge's line.
+constexpr float kZSeries = 0.1f;
+__device__ __forceinline__ float atanh2_series(float z)
+{
+    const float s = z * z;
+    float p = fmaf(s, 1.0f / 9.0f, 1.0f / 7.0f);
+    p = fmaf(s, p, 1.0f / 5.0f);
+    p = fmaf(s, p, 1.0f / 3.0f);
+    p = fmaf(s, p, 1.0f);
+    return (z + z) * p;
+}
+__device__ __forceinline__ float ln_ratio_near(float num, float den)
+{
+    return log1p_t(num * rcp_nr(den));
 }
 
 // The same logarithm in FP64 from original coordinates (used by the FP32C forward for
@@ -203,7 +251,7 @@ template <class S>
 struct FrameVals {
     S u[3], v[3], n[3];
     S l0, cu, cv, e1u, e1v, l1, e2u, e2v, l2;
-    S K1, K2, W0, W1, W2, W3, twoA;
+    S K1, W0, W1, W2, W3, twoA;
     S kz, wom, we[3];
     S eh[3][3], perp[3][3], len[3];
     bool degen, edge_ok[3], bad;
@@ -268,7 +316,6 @@ __device__ __forceinline__ void facet_frame(const S P[3][3], double drho, const 
     o.e2v = o.eh[2][0] * o.v[0] + o.eh[2][1] * o.v[1] + o.eh[2][2] * o.v[2];
     o.l2 = o.len[2];
     o.K1 = o.kz;
-    o.K2 = S(2.0) * o.kz;
     o.W0 = S(2.0) * o.wom;
     o.W1 = o.we[0];
     o.W2 = o.we[1];
@@ -276,15 +323,126 @@ __device__ __forceinline__ void facet_frame(const S P[3][3], double drho, const 
     o.twoA = o.l0 * o.cv;
 }
 
+// ------------------------------------------------- corner order of the frame records
+
+// The facet integral does not depend on which corner is called a (a cyclic relabelling
+// keeps the winding), but the FP32 frame does: with u along an axis-parallel edge, the
+// frame of a voxel-face triangle (two axis-parallel legs and a diagonal) is exactly
+// (0, +-1) in FP32, while u along the diagonal rounds every projection by the same
+// relative amount -- an error common to all facets of that shape, which a sum over
+// millions of them does not average out.  frame_rotation returns the cyclic shift k
+// (corner c of the record = corner (c + k) % 3 of the facet) whose frame (u, v; n is
+// shared) has the smallest FP32 representation error; ties keep k = 0.  A function of
+// the FP64 vertices only, so the facet setup and the adjoint's derivative matrix pick
+// the same k.
+__device__ __forceinline__ double f32_rep_err(double x) { return fabs((double)(float)x - x); }
+
+__device__ __forceinline__ int frame_rotation(const double P[3][3])
+{
+    const double e1[3] = {P[1][0] - P[0][0], P[1][1] - P[0][1], P[1][2] - P[0][2]};
+    const double e2[3] = {P[2][0] - P[0][0], P[2][1] - P[0][1], P[2][2] - P[0][2]};
+    double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                   e1[0] * e2[1] - e1[1] * e2[0]};
+    const double an = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (!(an >= 1e-15)) return 0;
+    for (int j = 0; j < 3; ++j) n[j] /= an;
+    int best = 0;
+    double best_err = 1e300;
+    for (int k = 0; k < 3; ++k) {
+        const double* p1 = P[k];
+        const double* p2 = P[(k + 1) % 3];
+        const double e[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+        const double l = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+        if (!(l >= 1e-15)) continue;
+        const double u[3] = {e[0] / l, e[1] / l, e[2] / l};
+        const double v[3] = {n[1] * u[2] - n[2] * u[1], n[2] * u[0] - n[0] * u[2],
+                             n[0] * u[1] - n[1] * u[0]};
+        double err = 0.0;
+        for (int j = 0; j < 3; ++j) err += f32_rep_err(u[j]) + f32_rep_err(v[j]);
+        if (err < best_err) {
+            best_err = err;
+            best = k;
+        }
+    }
+    return best;
+}
+
+__device__ __forceinline__ void rotate_corners(const double P[3][3], int k, double Q[3][3])
+{
+    for (int c = 0; c < 3; ++c)
+        for (int j = 0; j < 3; ++j) Q[c][j] = P[(c + k) % 3][j];
+}
+
+// ------------------------------------------------------- rounding of record constants
+
+// FP32 rounding of a facet constant that is unbiased across facets: round to one of the
+// two neighbouring floats with the probability that makes the expected value exact, from
+// a fixed hash of (facet, field).  Round-to-nearest gives every facet of one shape the
+// same representation error (voxel models have a handful of distinct edge lengths,
+// directions and coefficients), which a sum over millions of facets accumulates; with
+// this rounding the errors of different facets are independent with zero mean, and the
+// records are still a deterministic function of the vertices and properties.
+__device__ __forceinline__ uint32_t mix32(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+__device__ __forceinline__ float round_unbiased(double x, uint32_t key)
+{
+    const float r = (float)x;
+    const double err = x - (double)r;
+    if (err == 0.0 || !(fabs(x) < 3.0e38)) return r;
+    const float nb = nextafterf(r, err > 0.0 ? 3.4e38f : -3.4e38f);
+    const double gap = fabs((double)nb - (double)r);
+    const double uni = (double)(mix32(key) >> 8) * (1.0 / 16777216.0);  // [0, 1)
+    return uni * gap < fabs(err) ? nb : r;
+}
+
+template <class T>
+struct RecRound {
+    uint32_t base;
+    __device__ T operator()(double x, int field) const { return T(x); }
+};
+template <>
+struct RecRound<float> {
+    uint32_t base;
+    __device__ float operator()(double x, int field) const
+    {
+        return round_unbiased(x, base * 32u + (uint32_t)field);
+    }
+};
+
+// Per-facet origin offset of the FP32C records: each component uniform in
+// [-2^-9, 2^-9) m from a fixed hash of the facet.  The FP32 rounding of o - a has the
+// same error for every facet whose vertex a shares a coordinate with another (a voxel
+// layer shares its z, a column its x), and a sum over millions of facets adds those
+// errors as if the station sat slightly off in each layer; from an offset origin the
+// roundings of different facets are independent.  2^-9 m is two units in the last
+// place of FP32 at 8-16 km, the largest station-facet distances of a survey; d = R (a'-a)
+// restores the exact offset, so the offset changes no result beyond rounding.
+constexpr double kOriginJitter = 1.0 / 512.0;
+
 // Writes the frame-form record; a facet with zero area or a zero-length edge gets a
-// harmless unit right triangle at a with all coefficients zero.
+// harmless unit right triangle at a with all coefficients zero.  fid seeds the unbiased
+// rounding and the origin offset of FP32 records (jitter = true).
 template <class T>
 __device__ __forceinline__ void store_frame(const FrameVals<double>& fv, const double a[3],
-                                            FaceFrame<T>& r)
+                                            FaceFrame<T>& r, uint32_t fid = 0,
+                                            bool jitter = false)
 {
-    r.ax = a[0];
-    r.ay = a[1];
-    r.az = a[2];
+    double dl[3] = {0.0, 0.0, 0.0};
+    if (jitter && !fv.bad)
+        for (int j = 0; j < 3; ++j)
+            dl[j] = ((double)(mix32(fid * 32u + 29u + (uint32_t)j) >> 8) * (2.0 / 16777216.0) -
+                     1.0) * kOriginJitter;
+    r.ax = a[0] + dl[0];
+    r.ay = a[1] + dl[1];
+    r.az = a[2] + dl[2];
     if (fv.bad) {
         r.ux = T(1); r.uy = T(0); r.uz = T(0);
         r.vx = T(0); r.vy = T(1); r.vz = T(0);
@@ -292,23 +450,21 @@ __device__ __forceinline__ void store_frame(const FrameVals<double>& fv, const d
         r.l0 = T(1); r.cu = T(0); r.cv = T(1);
         r.e1u = T(-0.70710678118654752); r.e1v = T(0.70710678118654752); r.l1 = T(1.41421356237309505);
         r.e2u = T(0); r.e2v = T(-1); r.l2 = T(1);
-        r.K1 = r.K2 = r.W0 = r.W1 = r.W2 = r.W3 = T(0);
-        r.twoA = T(1);
-        r.zfb = T(0);
+        r.K1 = r.W0 = r.W1 = r.W2 = r.W3 = T(0);
+        r.du = r.dv = r.dn = T(0);
     } else {
-        r.ux = T(fv.u[0]); r.uy = T(fv.u[1]); r.uz = T(fv.u[2]);
-        r.vx = T(fv.v[0]); r.vy = T(fv.v[1]); r.vz = T(fv.v[2]);
-        r.nx = T(fv.n[0]); r.ny = T(fv.n[1]); r.nz = T(fv.n[2]);
-        r.l0 = T(fv.l0); r.cu = T(fv.cu); r.cv = T(fv.cv);
-        r.e1u = T(fv.e1u); r.e1v = T(fv.e1v); r.l1 = T(fv.l1);
-        r.e2u = T(fv.e2u); r.e2v = T(fv.e2v); r.l2 = T(fv.l2);
-        r.K1 = T(fv.K1); r.K2 = T(fv.K2);
-        r.W0 = T(fv.W0); r.W1 = T(fv.W1); r.W2 = T(fv.W2); r.W3 = T(fv.W3);
-        r.twoA = T(fv.twoA);
-        // A straddling edge needs the FP64 branch only if rho^2 * 1024 < d1 d2; with
-        // d1 d2 <= ((len + 2 rho)/2)^2 that requires rho < len/62, and rho >= |pz|.
-        const double lmax = fmax(fv.len[0], fmax(fv.len[1], fv.len[2]));
-        r.zfb = T(lmax / 60.0);
+        const RecRound<T> q{fid};
+        r.ux = q(fv.u[0], 0); r.uy = q(fv.u[1], 1); r.uz = q(fv.u[2], 2);
+        r.vx = q(fv.v[0], 3); r.vy = q(fv.v[1], 4); r.vz = q(fv.v[2], 5);
+        r.nx = q(fv.n[0], 6); r.ny = q(fv.n[1], 7); r.nz = q(fv.n[2], 8);
+        r.l0 = q(fv.l0, 9); r.cu = q(fv.cu, 10); r.cv = q(fv.cv, 11);
+        r.e1u = q(fv.e1u, 12); r.e1v = q(fv.e1v, 13); r.l1 = q(fv.l1, 14);
+        r.e2u = q(fv.e2u, 15); r.e2v = q(fv.e2v, 16); r.l2 = q(fv.l2, 17);
+        r.K1 = q(fv.K1, 18);
+        r.W0 = q(fv.W0, 20); r.W1 = q(fv.W1, 21); r.W2 = q(fv.W2, 22); r.W3 = q(fv.W3, 23);
+        r.du = T(fv.u[0] * dl[0] + fv.u[1] * dl[1] + fv.u[2] * dl[2]);
+        r.dv = T(fv.v[0] * dl[0] + fv.v[1] * dl[1] + fv.v[2] * dl[2]);
+        r.dn = T(fv.n[0] * dl[0] + fv.n[1] * dl[1] + fv.n[2] * dl[2]);
     }
 }
 

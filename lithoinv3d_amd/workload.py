@@ -16,19 +16,21 @@ from . import synth
 
 TRANSCENDENTAL_FLOP_EQ = 8
 
-# per (station, facet) pair
+# per (station, facet) pair, counted over the common path of the compiled loops (the
+# branch for edges within about five edge lengths of the station, taken by few waves, is
+# not counted); the three FP64 subtractions of the station offset count as add/mul
 OPS = {
-    # forward, gz + TMI fused, FP32C: FP64 station offset (3 adds), frame projection,
-    # squared distances, solid-angle N and D, atan2 polynomial, edge (s, dperp, rho^2),
-    # three cancellation-free edge logs (ln(1+y) with the rounding correction of 1+y,
-    # weighted by sat(2 - u)), gz and TMI combine with Kahan sums; 3 sqrt + 4 rcp + 3 log2
-    "forward_f32c": dict(fma=50, addmul=64, special=10),
-    # adjoint, gz + TMI seeds: the forward part, coefficient seeds, reverse sweep through
-    # the edge logs, solid angle and distances, accumulation of the 20 facet seeds (one
-    # angle seed per in-plane edge direction, torques by two FMAs each);
-    # 3 rsqrt (distances and their reciprocals) + 8 rcp (atan2, two per edge log with its
-    # derivative, the atan2 derivative) + 3 log2
-    "adjoint_pairs_f32": dict(fma=106, addmul=142, special=14),
+    # forward, gz + TMI fused, FP32C: station offset from the record's origin and its frame
+    # projection, squared distances, three Newton-refined square roots (rsq + 2 fmas),
+    # solid-angle N and D, atan2 polynomial, z = ell / (d1 + d2) with a Newton-refined
+    # reciprocal per edge, the 2 atanh(z) series per edge, gz and TMI combine with Kahan
+    # sums; 3 rsq + 4 rcp
+    "forward_f32c": dict(fma=62, addmul=50, special=7),
+    # adjoint, gz + TMI seeds: the forward part (plain rsq distances and rcp edge ratios),
+    # the series of 2 atanh(z) and its derivative, coefficient seeds, reverse sweep through
+    # the edge ratios, solid angle and distances, accumulation of the 19 facet seeds;
+    # 3 rsq + 5 rcp (three edge ratios, atan2 and its derivative)
+    "adjoint_pairs_f32": dict(fma=110, addmul=102, special=8),
 }
 
 

@@ -9,9 +9,7 @@ left out: the reference's TMI there depends on the sign of a rounded zero.
 Unit properties: the cluster table's median density minus 2.67 g/cc and its median
 susceptibility, unit k taking row k-1 (the files do not say which cluster each volume
 holds; any values exercise the same arithmetic).  Tolerances: FP64 1e-4 mGal and
-1e-3 nT absolute; FP32C 1e-3 mGal, reference-formula TMI 1e-6 of the field's magnitude
-(about 0.08 nT here, where FP32 rounding of the terms alone is ~0.01 nT) and textbook TMI
-1e-2 nT; adjoint against the oracle's one-sided-FD J^T w 2e-4 (FP64) and 2e-3 (FP32C) of
+1e-3 nT absolute; FP32C 1e-3 mGal and 1e-2 nT in both TMI formulas; adjoint against the oracle's one-sided-FD J^T w 2e-4 (FP64) and 2e-3 (FP32C) of
 the largest component.
 """
 import os
@@ -84,7 +82,7 @@ def test_forward_vs_oracle(forrestania):
     print(f"{d['model'].n_faces} facets ({sum(len(m.triangles) for m in d['meshes'])} in the "
           f"per-unit files), {len(d['st'])} stations; gz {d['gz'].min():.4f} .. "
           f"{d['gz'].max():.4f} mGal, TMI {d['tmi'].min():.1f} .. {d['tmi'].max():.1f} nT")
-    for prec, tg, tt in (("fp64", 1e-4, 1e-3), ("fp32c", 1e-3, 1e-6 * np.abs(d["tmi"]).max())):
+    for prec, tg, tt in (("fp64", 1e-4, 1e-3), ("fp32c", 1e-3, 1e-2)):
         say(f"forrestania forward {prec}")
         with ForwardContext(d["model"], d["st"], d["fh"], FIELD, precision=prec) as ctx:
             gz, tm = ctx.forward()
