@@ -280,6 +280,7 @@ struct lf_mlp {
     unsigned short *W1b = nullptr, *W2b = nullptr;
     float *X = nullptr, *T = nullptr, *Y = nullptr;
     int64_t cap = 0;
+    int losses_cap = 0;  // floats in losses
     int n_parts = 512;
     int64_t step = 0;
     std::string err;
@@ -457,6 +458,13 @@ int lf_mlp_loss_grad(lf_mlp* h, int64_t n, const float* x, const float* t, float
     return rc;
 }
 
+// Training steps between stream synchronizations: each step enqueues four launches, so at
+// most 4 * kTrainSyncSteps launches are outstanding.  Without a bound a long training run
+// (3000 steps) queues 12 000 dispatches ahead of the GPU, each of which the HIP runtime,
+// and a profiler collecting per-dispatch counters, tracks until it completes; a run of
+// that kind under counter collection ended in a host crash inside the enqueue.
+constexpr int kTrainSyncSteps = 256;
+
 int lf_mlp_train(lf_mlp* h, int64_t n, const float* x, const float* t, int steps, float lr,
                  float* losses)
 {
@@ -465,9 +473,15 @@ int lf_mlp_train(lf_mlp* h, int64_t n, const float* x, const float* t, int steps
     MT(h, hipSetDevice(h->device));
     int rc = upload(h, n, x, t);
     if (rc) return rc;
-    if (h->losses) hipFree(h->losses);
-    h->losses = nullptr;
-    MT(h, hipMalloc((void**)&h->losses, sizeof(float) * std::max(steps, 1)));
+    if (steps > h->losses_cap) {
+        // the previous buffer may still be read by queued work of an earlier call
+        MT(h, hipStreamSynchronize(h->stream));
+        if (h->losses) MT(h, hipFree(h->losses));
+        h->losses = nullptr;
+        h->losses_cap = 0;
+        MT(h, hipMalloc((void**)&h->losses, sizeof(float) * steps));
+        h->losses_cap = steps;
+    }
     for (int k = 0; k < steps; ++k) {
         rc = grad_pass(h, n, h->losses + k);
         if (rc) return rc;
@@ -477,6 +491,7 @@ int lf_mlp_train(lf_mlp* h, int64_t n, const float* x, const float* t, int steps
         hipLaunchKernelGGL(k_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, h->stream, h->P, h->m,
                            h->v, h->grad, lr, b1c, b2c);
         MT(h, hipGetLastError());
+        if ((k + 1) % kTrainSyncSteps == 0) MT(h, hipStreamSynchronize(h->stream));
     }
     if (losses && steps)
         MT(h, hipMemcpyAsync(losses, h->losses, sizeof(float) * steps, hipMemcpyDeviceToHost, h->stream));
