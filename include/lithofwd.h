@@ -134,6 +134,27 @@ int lf_timing(lf_ctx* ctx, double* ms, int64_t* launches, int n_classes);
  * branch in FP32C (0 in FP64 mode). */
 int lf_stats(lf_ctx* ctx, int64_t* fp64_edges);
 
+/* ---- forward with no device allocation ---------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * forward of lf_create + lf_forward_device for a model given entirely in device memory,
+ * with no context.  desc's array pointers are DEVICE pointers on desc->device, which must
+ * be the current HIP device; d_gz / d_tmi are device buffers of n_stations doubles (either
+ * may be NULL); stream is a hipStream_t (NULL: the null stream).  The work is enqueued and
+ * the call returns without waiting.
+ * The facet records, station arrays and per-chunk partial sums live in a static device
+ * buffer of LF_NOALLOC_SCRATCH_BYTES per GPU, part of the library's code object (held from
+ * the library's first use on a GPU until the process ends).  A model needs
+ *   ~ n_faces * (200 B FP32C | 288 B FP64) + n_stations * 24 B
+ *     + 16 B * n_stations * n_chunks   (n_chunks ~ 4096 / ceil(n_stations / 256))
+ * and a larger one is refused with LF_ENOMEM (lf_last_error(NULL) gives the bytes).
+ * Every call on one GPU uses the same buffer: calls that overlap on a GPU race, so
+ * serialize them (one stream, or wait for the previous call).  Facet vertex and unit ids
+ * cannot be checked on the host here; a facet with an id out of range contributes
+ * nothing.  Errors: lf_last_error(NULL). */
+#define LF_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int lf_forward_noalloc(const lf_desc* desc, double* d_gz, double* d_tmi, void* stream);
+
 /* ---- learned diagonal initial inverse Hessian for L-BFGS-B --------------------------
  * No counterpart in the reference: it replaces H0 = gamma*I of
  * modules/litho-inversion/src/inversion/lbfgsb_optimizer.cpp:40-62 by gamma*D with D

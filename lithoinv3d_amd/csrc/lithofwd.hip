@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -40,22 +41,31 @@ __global__ __launch_bounds__(128) void k_face_setup(
     const double* __restrict__ verts, const uint32_t* __restrict__ tri,
     const int32_t* __restrict__ front, const int32_t* __restrict__ back,
     const double* __restrict__ density, const double* __restrict__ mag, double fx,
-    double fy, double fz, int tmi_mode, int64_t n_f, FaceF32* __restrict__ out32,
-    FaceF64* __restrict__ out64, FaceFrame<double>* __restrict__ outfr64,
-    double* __restrict__ geo64)
+    double fy, double fz, int tmi_mode, int64_t n_f, int64_t n_v, int n_u,
+    FaceF32* __restrict__ out32, FaceF64* __restrict__ out64,
+    FaceFrame<double>* __restrict__ outfr64, double* __restrict__ geo64)
 {
     const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= n_f) return;
+    // a facet with a vertex or unit id out of range (possible only for models passed in
+    // device memory, which the host cannot check) becomes a zero-area facet with no
+    // properties: it contributes nothing
+    const uint32_t v0 = tri[3 * f], v1 = tri[3 * f + 1], v2 = tri[3 * f + 2];
+    int uf = front[f], ub = back[f];
+    const bool valid = (int64_t)v0 < n_v && (int64_t)v1 < n_v && (int64_t)v2 < n_v &&
+                       uf >= 0 && uf < n_u && ub >= -1 && ub < n_u;
     double P[3][3];
-    for (int k = 0; k < 3; ++k) {
-        const uint32_t vi = tri[3 * f + k];
-        for (int j = 0; j < 3; ++j) P[k][j] = verts[3 * (int64_t)vi + j];
+    const uint32_t vid[3] = {v0, v1, v2};
+    for (int k = 0; k < 3; ++k)
+        for (int j = 0; j < 3; ++j) P[k][j] = valid ? verts[3 * (int64_t)vid[k] + j] : 0.0;
+    if (!valid) {
+        uf = 0;
+        ub = -1;
     }
-    const int uf = front[f], ub = back[f];
-    const double drho = density[uf] - (ub >= 0 ? density[ub] : 0.0);
+    const double drho = valid ? density[uf] - (ub >= 0 ? density[ub] : 0.0) : 0.0;
     double Md[3], Ms[3];
     for (int j = 0; j < 3; ++j) {
-        const double mf = mag[3 * uf + j];
+        const double mf = valid ? mag[3 * uf + j] : 0.0;
         const double mb = ub >= 0 ? mag[3 * ub + j] : 0.0;
         Md[j] = mf - mb;  // the Omega term flips with the winding
         Ms[j] = mf + mb;  // the reference edge term does not: t = n x e is reversal-invariant
@@ -493,25 +503,32 @@ struct TimedLaunch {
     }
 };
 
+// Station tiles x facet chunks of the forward grid: about blocks_per_cu (16, or
+// LITHOFWD_BLOCKS_PER_CU) blocks per CU, at least 32 facets per chunk.
+void chunk_plan(int64_t n_s, int64_t n_f, int precision, int dev_cus, int& n_tiles,
+                int64_t& faces_per_chunk, int& n_chunks)
+{
+    const int64_t tile_st = (int64_t)kBlock * (precision == LF_FP32C ? kSPL : 1);
+    n_tiles = (int)((n_s + tile_st - 1) / tile_st);
+    const char* env = getenv("LITHOFWD_BLOCKS_PER_CU");
+    const int per_cu = env ? std::max(1, atoi(env)) : 16;
+    const int64_t target = (int64_t)std::max(1, dev_cus) * per_cu;
+    const int64_t tiles = std::max(1, n_tiles);
+    int64_t chunks = (target + tiles - 1) / tiles;
+    const int64_t max_chunks = std::max<int64_t>(1, n_f / 32);
+    chunks = std::max<int64_t>(1, std::min(chunks, max_chunks));
+    faces_per_chunk = std::max<int64_t>(1, (n_f + chunks - 1) / chunks);
+    n_chunks = (int)std::max<int64_t>(1, (n_f + faces_per_chunk - 1) / faces_per_chunk);
+}
+
 int choose_chunks(lf_ctx* c)
 {
-    const int64_t tile_st = (int64_t)kBlock * (c->precision == LF_FP32C ? kSPL : 1);
-    c->n_tiles = (int)((c->n_s + tile_st - 1) / tile_st);
     int dev_cus = 256;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
         dev_cus = prop.multiProcessorCount;
-    const char* env = getenv("LITHOFWD_BLOCKS_PER_CU");
-    const int per_cu = env ? std::max(1, atoi(env)) : 16;
-    const int64_t target = (int64_t)dev_cus * per_cu;
-    const int64_t tiles = std::max(1, c->n_tiles);
-    int64_t chunks = (target + tiles - 1) / tiles;
-    const int64_t max_chunks = std::max<int64_t>(1, c->n_f / 32);
-    chunks = std::max<int64_t>(1, std::min(chunks, max_chunks));
-    c->faces_per_chunk = (c->n_f + chunks - 1) / chunks;
-    if (c->faces_per_chunk < 1) c->faces_per_chunk = 1;
-    c->n_chunks = (int)((c->n_f + c->faces_per_chunk - 1) / c->faces_per_chunk);
-    if (c->n_chunks < 1) c->n_chunks = 1;
+    chunk_plan(c->n_s, c->n_f, c->precision, dev_cus, c->n_tiles, c->faces_per_chunk,
+               c->n_chunks);
     return LF_OK;
 }
 
@@ -523,8 +540,8 @@ int ensure_faces(lf_ctx* c)
         const int64_t blocks = (c->n_f + 127) / 128;
         hipLaunchKernelGGL(k_face_setup, dim3((unsigned)blocks), dim3(128), 0, c->stream,
                            c->d_verts, c->d_tri, c->d_front, c->d_back, c->d_density, c->d_mag,
-                           c->fdir[0], c->fdir[1], c->fdir[2], c->tmi_mode, c->n_f,
-                           c->d_f32, c->d_f64, c->d_fr64, c->d_geo64);
+                           c->fdir[0], c->fdir[1], c->fdir[2], c->tmi_mode, c->n_f, c->n_v,
+                           (int)c->n_u, c->d_f32, c->d_f64, c->d_fr64, c->d_geo64);
         LF_TRY(c, hipGetLastError());
     }
     c->faces_dirty = false;
@@ -994,3 +1011,151 @@ int lf_stats(lf_ctx* c, int64_t* fp64_edges)
 }
 
 }  // extern "C"
+
+// ============================================================== forward, no allocation
+
+namespace {
+
+// Scratch of lf_forward_noalloc: one per GPU, part of the code object, so it exists from
+// the library's load on that GPU until the process ends.  Host code never uses the
+// symbol's name as an address (on the host it is a stand-in): it uses the per-device
+// address from hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[LF_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mutex;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+int g_scratch_cus[kMaxDevices] = {};
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+struct NoallocLayout {
+    size_t recs, geo, st, part, fb, total;
+};
+
+// Byte offsets in the scratch: facet records (FaceF32 + FP64 corners for FP32C, FaceF64
+// for FP64), station x/y/z arrays, per-chunk gz and TMI partials, the FP64-edge counter.
+NoallocLayout noalloc_layout(int64_t n_f, int64_t n_s, int precision, int n_chunks)
+{
+    NoallocLayout L;
+    L.recs = 0;
+    const size_t rec = precision == LF_FP32C ? sizeof(FaceF32) : sizeof(FaceF64);
+    L.geo = align256((size_t)n_f * rec);
+    const size_t geo = precision == LF_FP32C ? (size_t)n_f * 9 * sizeof(double) : 0;
+    L.st = align256(L.geo + geo);
+    L.part = align256(L.st + 3 * (size_t)n_s * sizeof(double));
+    L.fb = align256(L.part + 2 * (size_t)n_chunks * n_s * sizeof(double));
+    L.total = L.fb + sizeof(unsigned long long);
+    return L;
+}
+
+}  // namespace
+
+extern "C" int lf_forward_noalloc(const lf_desc* d, double* d_gz, double* d_tmi, void* stream)
+{
+    g_create_error.clear();
+    if (!d || d->n_vertices < 0 || d->n_faces < 0 || d->n_stations < 0 || d->n_units <= 0 ||
+        (d->n_faces > 0 && (!d->vertices || !d->faces || !d->front || !d->back)) ||
+        (d->n_stations > 0 && !d->stations) || !d->density || !d->magnetization ||
+        (d->precision != LF_FP64 && d->precision != LF_FP32C) ||
+        (d->tmi_mode != LF_TMI_REFERENCE && d->tmi_mode != LF_TMI_TEXTBOOK) ||
+        (d->want & ~3) != 0 || d->n_vertices > 0xffffffffLL) {
+        g_create_error = "lf_forward_noalloc: invalid descriptor";
+        return LF_EINVAL;
+    }
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev != d->device || dev < 0 || dev >= kMaxDevices) {
+        g_create_error = "lf_forward_noalloc: desc->device must be the current HIP device";
+        return LF_EINVAL;
+    }
+    unsigned char* base = nullptr;
+    int cus = 0;
+    {
+        std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+        if (!g_scratch_ptr[dev]) {
+            void* p = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p) {
+                g_create_error = std::string("lf_forward_noalloc: hipGetSymbolAddress: ") +
+                                 hipGetErrorString(e);
+                return LF_EHIP;
+            }
+            int n = 0;
+            if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+                n <= 0)
+                n = 256;
+            g_scratch_cus[dev] = n;
+            g_scratch_ptr[dev] = (unsigned char*)p;
+        }
+        base = g_scratch_ptr[dev];
+        cus = g_scratch_cus[dev];
+    }
+    const int64_t n_f = d->n_faces, n_s = d->n_stations;
+    const int want = d->want ? d->want : 3;
+    const bool gz = d_gz && (want & LF_WANT_GZ);
+    const bool tm = d_tmi && (want & LF_WANT_TMI);
+    hipStream_t st = (hipStream_t)stream;
+    if (n_s == 0 || (!gz && !tm)) return LF_OK;
+    const unsigned sblocks = (unsigned)((n_s + 255) / 256);
+    if (n_f == 0) {
+        if (gz) hipLaunchKernelGGL(k_fill_zero, dim3(sblocks), dim3(256), 0, st, d_gz, n_s);
+        if (tm) hipLaunchKernelGGL(k_fill_zero, dim3(sblocks), dim3(256), 0, st, d_tmi, n_s);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            g_create_error = std::string("lf_forward_noalloc: ") + hipGetErrorString(e);
+            return LF_EHIP;
+        }
+        return LF_OK;
+    }
+    int n_tiles = 1, n_chunks = 1;
+    int64_t fpc = 1;
+    chunk_plan(n_s, n_f, d->precision, cus, n_tiles, fpc, n_chunks);
+    const NoallocLayout L = noalloc_layout(n_f, n_s, d->precision, n_chunks);
+    if (L.total > (size_t)LF_NOALLOC_SCRATCH_BYTES) {
+        g_create_error = "lf_forward_noalloc: the model needs " + std::to_string(L.total) +
+                         " bytes of scratch, more than LF_NOALLOC_SCRATCH_BYTES (" +
+                         std::to_string((unsigned long long)LF_NOALLOC_SCRATCH_BYTES) + ")";
+        return LF_ENOMEM;
+    }
+    FaceF32* f32 = d->precision == LF_FP32C ? (FaceF32*)(base + L.recs) : nullptr;
+    FaceF64* f64 = d->precision == LF_FP64 ? (FaceF64*)(base + L.recs) : nullptr;
+    double* geo = d->precision == LF_FP32C ? (double*)(base + L.geo) : nullptr;
+    double* sx = (double*)(base + L.st);
+    double* sy = sx + n_s;
+    double* sz = sy + n_s;
+    double* pg = (double*)(base + L.part);
+    double* pt = pg + (size_t)n_chunks * n_s;
+    unsigned long long* fb = (unsigned long long*)(base + L.fb);
+
+    hipLaunchKernelGGL(k_face_setup, dim3((unsigned)((n_f + 127) / 128)), dim3(128), 0, st,
+                       d->vertices, d->faces, d->front, d->back, d->density, d->magnetization,
+                       d->field_dir[0], d->field_dir[1], d->field_dir[2], d->tmi_mode, n_f,
+                       d->n_vertices, (int)d->n_units, f32, f64,
+                       (FaceFrame<double>*)nullptr, geo);
+    hipLaunchKernelGGL(k_split_xyz, dim3(sblocks), dim3(256), 0, st, d->stations, n_s, sx, sy, sz);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, fb);
+    const dim3 grid((unsigned)((int64_t)n_tiles * n_chunks));
+#define LF_FWD_NOALLOC(G, T)                                                                  \
+    do {                                                                                     \
+        if (d->precision == LF_FP32C)                                                        \
+            hipLaunchKernelGGL((k_forward_f32c<G, T>), grid, dim3(kBlock), 0, st, f32, geo,  \
+                               n_f, fpc, sx, sy, sz, n_s, n_tiles, pg, pt, fb);             \
+        else                                                                                 \
+            hipLaunchKernelGGL((k_forward_f64<G, T>), grid, dim3(kBlock), 0, st, f64, n_f,   \
+                               fpc, sx, sy, sz, n_s, n_tiles, pg, pt);                       \
+    } while (0)
+    if (gz && tm) LF_FWD_NOALLOC(true, true);
+    else if (gz) LF_FWD_NOALLOC(true, false);
+    else LF_FWD_NOALLOC(false, true);
+#undef LF_FWD_NOALLOC
+    if (gz)
+        hipLaunchKernelGGL(k_reduce_chunks, dim3(sblocks), dim3(256), 0, st, pg, n_chunks, n_s, d_gz);
+    if (tm)
+        hipLaunchKernelGGL(k_reduce_chunks, dim3(sblocks), dim3(256), 0, st, pt, n_chunks, n_s, d_tmi);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        g_create_error = std::string("lf_forward_noalloc: ") + hipGetErrorString(e);
+        return LF_EHIP;
+    }
+    return LF_OK;
+}

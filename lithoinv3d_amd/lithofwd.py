@@ -24,7 +24,7 @@ TIMING_CLASSES = ("face_setup", "forward", "forward_reduce", "adjoint_pairs",
 EXPORTED = (
     "lf_create", "lf_destroy", "lf_last_error", "lf_set_vertices", "lf_set_vertices_device",
     "lf_set_properties", "lf_forward", "lf_forward_device", "lf_adjoint", "lf_adjoint_device",
-    "lf_sync", "lf_set_timing", "lf_timing", "lf_stats", "lf_colnorm2",
+    "lf_sync", "lf_set_timing", "lf_timing", "lf_stats", "lf_colnorm2", "lf_forward_noalloc",
 )
 # the learned-preconditioner MLP (declared in mlp.py)
 EXPORTED_MLP = (
@@ -109,6 +109,7 @@ def load_library(path: str = LIB_PATH):
     L.lf_timing.argtypes = [P, _D, ctypes.POINTER(i64), ctypes.c_int]
     L.lf_stats.argtypes = [P, ctypes.POINTER(i64)]
     L.lf_colnorm2.argtypes = [P, ctypes.c_int, i64, P, P, P, P, P]
+    L.lf_forward_noalloc.argtypes = [ctypes.POINTER(_Desc), P, P, P]
     for name in EXPORTED:
         if name not in ("lf_destroy", "lf_last_error"):
             getattr(L, name).restype = ctypes.c_int
@@ -279,3 +280,36 @@ class ForwardContext:
         v = ctypes.c_int64()
         self._check(self._lib.lf_stats(self._ctx, ctypes.byref(v)), "lf_stats")
         return int(v.value)
+
+
+def forward_noalloc(n_vertices, n_faces, n_stations, n_units, vertices, faces, front, back,
+                    stations, density, magnetization, field_dir, gz=0, tmi=0,
+                    precision="fp64", tmi_mode="reference", want=("gz", "tmi"), device=0,
+                    stream=0):
+    """lf_forward_noalloc: the forward of a model held in device memory, with no device
+    allocation and no context.  Every array argument is a DEVICE pointer (int) on
+    `device`, which must be the current device: vertices (n_vertices*3 FP64), faces
+    (n_faces*3 uint32), front/back (n_faces int32), stations (n_stations*3 FP64), density
+    (n_units FP64), magnetization (n_units*3 FP64); gz / tmi are device outputs of
+    n_stations FP64 (0: not wanted).  Enqueued on `stream` (a hipStream_t as int, 0: the
+    null stream); the caller synchronizes."""
+    L = load_library()
+    d = _Desc()
+    d.n_vertices, d.n_faces, d.n_stations, d.n_units = n_vertices, n_faces, n_stations, n_units
+    d.precision = {"fp64": LF_FP64, "fp32c": LF_FP32C}[precision]
+    d.tmi_mode = {"reference": LF_TMI_REFERENCE, "textbook": LF_TMI_TEXTBOOK}[tmi_mode]
+    d.want = (LF_WANT_GZ if "gz" in want else 0) | (LF_WANT_TMI if "tmi" in want else 0)
+    d.device = device
+    d.vertices = ctypes.cast(ctypes.c_void_p(vertices), _D)
+    d.faces = ctypes.cast(ctypes.c_void_p(faces), ctypes.POINTER(ctypes.c_uint32))
+    d.front = ctypes.cast(ctypes.c_void_p(front), ctypes.POINTER(ctypes.c_int32))
+    d.back = ctypes.cast(ctypes.c_void_p(back), ctypes.POINTER(ctypes.c_int32))
+    d.stations = ctypes.cast(ctypes.c_void_p(stations), _D)
+    d.density = ctypes.cast(ctypes.c_void_p(density), _D)
+    d.magnetization = ctypes.cast(ctypes.c_void_p(magnetization), _D)
+    d.field_dir[:] = [float(x) for x in field_dir]
+    rc = L.lf_forward_noalloc(ctypes.byref(d), ctypes.c_void_p(gz or None),
+                              ctypes.c_void_p(tmi or None), ctypes.c_void_p(stream or None))
+    if rc != 0:
+        raise LithoFwdError(f"lf_forward_noalloc failed ({rc}): {L.lf_last_error(None).decode()}")
+
