@@ -31,6 +31,13 @@ int li_model_create(void** out, int64_t n_v, const double* verts, int64_t n_f,
 void li_model_destroy(void* h);
 /* current per-unit density and susceptibility (n_units each; LithoGroup, litho_group.h) */
 int li_model_properties(void* h, double* density, double* susceptibility);
+/* LithoGroup's remanence fields (litho_group.h:18-24), n_units each (vector: 3 per unit):
+ * magnitude (A/m, reference units), inclination and declination (degrees, inclination
+ * positive down), and the remanent vector; a NULL array keeps / skips that field */
+int li_model_set_remanence(void* h, const double* magnitude, const double* inc_deg,
+                           const double* dec_deg, const double* vector);
+int li_model_remanence(void* h, double* magnitude, double* inc_deg, double* dec_deg,
+                       double* vector);
 /* fixExteriorFaces (lithology_model.cpp:369-579): vertices of model-bounding facets FIXED,
  * the rest `interior_freedom` (VertexFreedom of surface_mesh.h: 0 FIXED, 1 Z_ONLY,
  * 2 X_ONLY, 3 Y_ONLY, 8 XYZ_FREE). */
@@ -95,11 +102,22 @@ int li_problem_add_smoothness(void* h, double weight);
 int li_problem_add_boreholes(void* h, int64_t n, const double* xyz, const int32_t* above,
                              const int32_t* below, const double* sigma, double radius,
                              double weight);
-/* the property phase (runner.cpp:708-800 densities, :801-935 susceptibilities): which 0
- * densities from gravity, 1 susceptibilities from TMI; out = n_units new values, which the
- * model and every forward of the problem take */
+/* the property phase (runner.cpp:708-800 densities, :801-935 magnetic properties): which 0
+ * densities from gravity (out: n_units values), 1 magnetic properties from TMI in the
+ * problem's remanence mode (out: n_units x {chi} | {chi, remanence magnitude} |
+ * {chi, Mx, My, Mz}, runner.cpp:808-857); the model and every forward of the problem take
+ * the new values.  lower/upper bound densities and susceptibilities; remanence parameters
+ * take li_problem_set_remanence_bounds. */
 int li_problem_invert_properties(void* h, int which, double damping, double lower, double upper,
                                  int max_iter, double tol, double* out);
+/* MagneticForward::setRemanenceMode (magnetic_forward.h:61; litho_group.h:7-11): 0
+ * EffectiveSusceptibility, 1 FixedVectorPerGroup, 2 VectorPerGroup, for every magnetic
+ * forward, objective, calibration and property phase of the problem */
+int li_problem_set_remanence_mode(void* h, int mode);
+/* bounds of the remanence magnitude and of the remanent vector's components in the
+ * magnetic property phase (runner.h:102-107; defaults 0..10 and -10..10) */
+int li_problem_set_remanence_bounds(void* h, double magnitude_lower, double magnitude_upper,
+                                    double component_lower, double component_upper);
 /* JointObjective::calibrateDataWeights (joint_objective.cpp:281-393), matrix-free: sets
  * alpha_mag from the ratio of the largest eigenvalues of J^T W^2 J of gravity and of
  * magnetics at params.  out[7]: both eigenvalues, their ratio, the new alpha_mag, the two
@@ -121,8 +139,15 @@ int li_problem_forward(void* h, const double* params, double* gz, double* tmi);
 /* ForwardModel::computeJacobian (gravity_forward.cpp:269-288): which 0 gz, 1 TMI;
  * n_s x n_dof, column-major */
 int li_problem_jacobian(void* h, const double* params, int which, double* J_colmajor);
-/* unit-property response of one group at the current geometry (gravity_forward.cpp:132-145) */
+/* unit-property response of one group at the current geometry: which 0 unit density
+ * (gravity_forward.cpp:132-145), 1 unit susceptibility (magnetic_forward.cpp:148-162) */
 int li_problem_unit_response(void* h, int which, int group, double* out);
+/* MagneticForward::computeGroupUnitResponseDirection (magnetic_forward.cpp:164-178): TMI of
+ * one group magnetized with unit amplitude along direction[3] */
+int li_problem_unit_response_direction(void* h, int group, const double* direction, double* out);
+/* MagneticForward::computeGroupUnitRemanenceResponse (magnetic_forward.cpp:180-190): the
+ * same along the group's remanence inclination / declination */
+int li_problem_unit_remanence_response(void* h, int group, double* out);
 
 /* ---- the inversion loop (runner.cpp:217-641) ---- */
 

@@ -42,6 +42,23 @@ struct Problem {
     bool useFused = true;
     double inc = 0.0, dec = 0.0, field = 0.0;
     int device = 0;
+    // magnetic property phase: remanence mode and the bounds of its parameters
+    // (runner.h:98-107)
+    RemanentMagnetizationMode remMode = RemanentMagnetizationMode::EffectiveSusceptibility;
+    double remLower = 0.0, remUpper = 10.0, compLower = -10.0, compUpper = 10.0;
+
+    void applyRemanenceBounds(PropertyInversionOptions& o) const
+    {
+        o.remanenceLower = remLower;
+        o.remanenceUpper = remUpper;
+        o.componentLower = compLower;
+        o.componentUpper = compUpper;
+    }
+    MagneticForward& magnetic()
+    {
+        if (!mag) throw std::runtime_error("problem has no magnetic data");
+        return *mag;
+    }
 
     JointObjective::JointComponents components(const Vec& p)
     {
@@ -104,6 +121,40 @@ int li_model_properties(void* h, double* density, double* susceptibility)
         for (int g = 0; g < m.groupCount(); ++g) {
             density[g] = m.group(g).density;
             susceptibility[g] = m.group(g).susceptibility;
+        }
+    });
+}
+
+// LithoGroup's remanence fields (litho_group.h:18-24): per unit magnitude (A/m),
+// inclination and declination (degrees) and the vector (3 per unit); NULL keeps a field
+int li_model_set_remanence(void* h, const double* magnitude, const double* inc_deg,
+                           const double* dec_deg, const double* vector)
+{
+    return guard([&] {
+        auto& m = *((ModelHandle*)h)->m;
+        for (int g = 0; g < m.groupCount(); ++g) {
+            LithoGroup& G = m.group(g);
+            if (magnitude) G.remanence_magnitude = magnitude[g];
+            if (inc_deg) G.remanence_inclination = inc_deg[g];
+            if (dec_deg) G.remanence_declination = dec_deg[g];
+            if (vector)
+                for (int k = 0; k < 3; ++k) G.magnetization[k] = vector[3 * g + k];
+        }
+    });
+}
+
+int li_model_remanence(void* h, double* magnitude, double* inc_deg, double* dec_deg,
+                       double* vector)
+{
+    return guard([&] {
+        const auto& m = *((ModelHandle*)h)->m;
+        for (int g = 0; g < m.groupCount(); ++g) {
+            const LithoGroup& G = m.group(g);
+            if (magnitude) magnitude[g] = G.remanence_magnitude;
+            if (inc_deg) inc_deg[g] = G.remanence_inclination;
+            if (dec_deg) dec_deg[g] = G.remanence_declination;
+            if (vector)
+                for (int k = 0; k < 3; ++k) vector[3 * g + k] = G.magnetization[k];
         }
     });
 }
@@ -353,12 +404,12 @@ int li_problem_invert_properties(void* h, int which, double damping, double lowe
         o.upper = upper;
         o.maxIter = max_iter;
         o.tolerance = tol;
+        P->applyRemanenceBounds(o);
         Vec p;
         if (which == 0) {
             p = invertUnitDensities(*P->grav, P->gdata, *P->model, o);
         } else if (which == 1) {
-            if (!P->mag) throw std::runtime_error("problem has no magnetic data");
-            p = invertUnitSusceptibilities(*P->mag, P->mdata, *P->model, o);
+            p = invertUnitSusceptibilities(P->magnetic(), P->mdata, *P->model, o);
         } else {
             throw std::invalid_argument("li_problem_invert_properties: which must be 0 or 1");
         }
@@ -382,7 +433,7 @@ int li_problem_calibrate(void* h, const double* params, uint64_t seed, double ep
         const double a0 = P->joint->alphaMag();
         const CalibrationResult r = calibrateMagneticWeight(P->model, P->gdata, P->mdata, P->inc,
                                                             P->dec, P->field, p, a0, seed, eps,
-                                                            P->device);
+                                                            P->device, P->remMode);
         P->joint->setAlphaMag(r.alpha);
         if (P->fused) P->fused->setAlphaMag(r.alpha);
         const double o[7] = {r.eigGravity, r.eigMagnetic, r.ratio, r.alpha,
@@ -450,7 +501,9 @@ int li_problem_run(void* h, const LiRunnerOptions* o, void* label_grid, double* 
             tmi.clear();
             if (P->mag) tmi = P->mag->compute(p);
         };
-        rp.invertProperties = [P](int which, const PropertyInversionOptions& po) {
+        rp.invertProperties = [P](int which, const PropertyInversionOptions& po0) {
+            PropertyInversionOptions po = po0;
+            P->applyRemanenceBounds(po);
             if (which == 0)
                 invertUnitDensities(*P->grav, P->gdata, *P->model, po);
             else if (P->mag)
@@ -462,7 +515,7 @@ int li_problem_run(void* h, const LiRunnerOptions* o, void* label_grid, double* 
         rp.calibrate = [P](const Vec& p) {
             const CalibrationResult r = calibrateMagneticWeight(
                 P->model, P->gdata, P->mdata, P->inc, P->dec, P->field, p, P->joint->alphaMag(),
-                1, 1e-3, P->device);
+                1, 1e-3, P->device, P->remMode);
             P->joint->setAlphaMag(r.alpha);
             if (P->fused) P->fused->setAlphaMag(r.alpha);
             return r.alpha;
@@ -632,8 +685,48 @@ int li_problem_unit_response(void* h, int which, int group, double* out)
     return guard([&] {
         auto* P = (Problem*)h;
         const Vec r = which == 0 ? P->grav->computeGroupUnitResponse(group)
-                                 : P->mag->computeGroupUnitResponse(group);
+                                 : P->magnetic().computeGroupUnitResponse(group);
         std::memcpy(out, r.data(), sizeof(double) * r.size());
+    });
+}
+
+int li_problem_unit_response_direction(void* h, int group, const double* direction, double* out)
+{
+    return guard([&] {
+        const Vec r = ((Problem*)h)->magnetic().computeGroupUnitResponseDirection(group, direction);
+        std::memcpy(out, r.data(), sizeof(double) * r.size());
+    });
+}
+
+int li_problem_unit_remanence_response(void* h, int group, double* out)
+{
+    return guard([&] {
+        const Vec r = ((Problem*)h)->magnetic().computeGroupUnitRemanenceResponse(group);
+        std::memcpy(out, r.data(), sizeof(double) * r.size());
+    });
+}
+
+int li_problem_set_remanence_mode(void* h, int mode)
+{
+    return guard([&] {
+        auto* P = (Problem*)h;
+        if (mode < 0 || mode > 2)
+            throw std::invalid_argument("li_problem_set_remanence_mode: mode must be 0, 1 or 2");
+        P->remMode = (RemanentMagnetizationMode)mode;
+        if (P->mag) P->mag->setRemanenceMode(P->remMode);
+        if (P->fused) P->fused->setRemanenceMode(P->remMode);
+    });
+}
+
+int li_problem_set_remanence_bounds(void* h, double magnitude_lower, double magnitude_upper,
+                                    double component_lower, double component_upper)
+{
+    return guard([&] {
+        auto* P = (Problem*)h;
+        P->remLower = magnitude_lower;
+        P->remUpper = magnitude_upper;
+        P->compLower = component_lower;
+        P->compUpper = component_upper;
     });
 }
 

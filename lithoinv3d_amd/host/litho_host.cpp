@@ -748,21 +748,26 @@ MagneticForward::MagneticForward(std::shared_ptr<LithologyModel> model, const Ma
     pushProperties();
 }
 
-void MagneticForward::buildGroupMagnetization(int g, double& amp, double dir[3]) const
+void groupMagnetization(const LithoGroup& G, RemanentMagnetizationMode mode, double field_nT,
+                        const double fhat[3], double& amp, double dir[3])
 {
     // magnetic_forward.cpp:100-134
-    const LithoGroup& G = m_model->group(g);
-    const double induced = G.susceptibility * m_field / 100.0;
-    double M[3] = {induced * m_fhat[0], induced * m_fhat[1], induced * m_fhat[2]};
-    if (m_mode == RemanentMagnetizationMode::FixedVectorPerGroup) {
+    const double induced = G.susceptibility * field_nT / 100.0;
+    double M[3] = {induced * fhat[0], induced * fhat[1], induced * fhat[2]};
+    if (mode == RemanentMagnetizationMode::FixedVectorPerGroup) {
         double R[3];
         fieldUnitVector(G.remanence_inclination, G.remanence_declination, R);
         for (int k = 0; k < 3; ++k) M[k] += G.remanence_magnitude * R[k];
-    } else if (m_mode == RemanentMagnetizationMode::VectorPerGroup) {
+    } else if (mode == RemanentMagnetizationMode::VectorPerGroup) {
         for (int k = 0; k < 3; ++k) M[k] += G.magnetization[k];
     }
     amp = std::sqrt(M[0] * M[0] + M[1] * M[1] + M[2] * M[2]);
-    for (int k = 0; k < 3; ++k) dir[k] = amp < kEps ? m_fhat[k] : M[k] / amp;
+    for (int k = 0; k < 3; ++k) dir[k] = amp < kEps ? fhat[k] : M[k] / amp;
+}
+
+void MagneticForward::buildGroupMagnetization(int g, double& amp, double dir[3]) const
+{
+    groupMagnetization(m_model->group(g), m_mode, m_field, m_fhat, amp, dir);
 }
 
 void MagneticForward::buildProperties(std::vector<double>& dens, std::vector<double>& mag) const
@@ -773,9 +778,9 @@ void MagneticForward::buildProperties(std::vector<double>& dens, std::vector<dou
     for (int g = 0; g < ng; ++g) {
         double amp, dir[3];
         if (m_unitOnly >= 0) {
-            // unit susceptibility, induced (magnetic_forward.cpp:148-162)
-            amp = g == m_unitOnly ? m_field / 100.0 : 0.0;
-            for (int k = 0; k < 3; ++k) dir[k] = m_fhat[k];
+            // a unit response: only group m_unitOnly, with the requested magnetization
+            amp = g == m_unitOnly ? m_unitAmp : 0.0;
+            for (int k = 0; k < 3; ++k) dir[k] = m_unitDir[k];
         } else {
             buildGroupMagnetization(g, amp, dir);
         }
@@ -789,14 +794,48 @@ Vec MagneticForward::compute(const Vec& params)
     return forwardRaw(1);
 }
 
+Vec MagneticForward::unitResponse(int g, double amp, const double dir[3])
+{
+    if (g < 0 || g >= m_model->groupCount())
+        throw std::out_of_range("MagneticForward: group index out of range");
+    m_unitOnly = g;
+    m_unitAmp = amp;
+    for (int k = 0; k < 3; ++k) m_unitDir[k] = dir[k];
+    try {
+        pushProperties();
+        Vec r = forwardRaw(1);
+        m_unitOnly = -1;
+        pushProperties();
+        return r;
+    } catch (...) {
+        m_unitOnly = -1;
+        throw;
+    }
+}
+
 Vec MagneticForward::computeGroupUnitResponse(int g)
 {
-    m_unitOnly = g;
-    pushProperties();
-    Vec r = forwardRaw(1);
-    m_unitOnly = -1;
-    pushProperties();
-    return r;
+    // magneticClosedMesh(obs, mesh, 1.0, F-hat, F): unit susceptibility, induced
+    // (magnetic_forward.cpp:148-162)
+    return unitResponse(g, m_field / 100.0, m_fhat);
+}
+
+Vec MagneticForward::computeGroupUnitResponseDirection(int g, const double magDirection[3])
+{
+    // magneticClosedMeshVector(obs, mesh, 1.0, magDirection, F-hat)
+    // (magnetic_forward.cpp:164-178)
+    return unitResponse(g, 1.0, magDirection);
+}
+
+Vec MagneticForward::computeGroupUnitRemanenceResponse(int g)
+{
+    // magnetic_forward.cpp:180-190
+    if (g < 0 || g >= m_model->groupCount())
+        throw std::out_of_range("MagneticForward: group index out of range");
+    const LithoGroup& G = m_model->group(g);
+    double R[3];
+    fieldUnitVector(G.remanence_inclination, G.remanence_declination, R);
+    return computeGroupUnitResponseDirection(g, R);
 }
 
 // ============================================================= regularization
@@ -1193,8 +1232,9 @@ FusedJointObjective::FusedJointObjective(std::shared_ptr<LithologyModel> model,
     std::vector<double> dens(ng), mag(3 * ng);
     for (int g = 0; g < ng; ++g) {
         dens[g] = m_model->group(g).density;
-        const double amp = m_model->group(g).susceptibility * field_nT / 100.0;  // induced
-        for (int k = 0; k < 3; ++k) mag[3 * g + k] = amp * fh[k];
+        double amp, dir[3];
+        groupMagnetization(m_model->group(g), m_mode, m_field, m_fhat, amp, dir);
+        for (int k = 0; k < 3; ++k) mag[3 * g + k] = amp * dir[k];
     }
     lf_desc d{};
     d.n_vertices = (int64_t)m_model->vertexCount();
@@ -1228,8 +1268,9 @@ void FusedJointObjective::refreshProperties()
     std::vector<double> dens(ng), mag(3 * ng);
     for (int g = 0; g < ng; ++g) {
         dens[g] = m_model->group(g).density;
-        const double amp = m_model->group(g).susceptibility * m_field / 100.0;  // induced
-        for (int k = 0; k < 3; ++k) mag[3 * g + k] = amp * m_fhat[k];
+        double amp, dir[3];
+        groupMagnetization(m_model->group(g), m_mode, m_field, m_fhat, amp, dir);
+        for (int k = 0; k < 3; ++k) mag[3 * g + k] = amp * dir[k];
     }
     check(lf_set_properties(m_ctx, dens.data(), mag.data()), m_ctx, "lf_set_properties");
     m_cacheValid = false;
@@ -1252,13 +1293,14 @@ CalibrationResult calibrateMagneticWeight(std::shared_ptr<LithologyModel> model,
                                           const GravityData& gdata, const MagneticData& mdata,
                                           double inc_deg, double dec_deg, double field_nT,
                                           const Vec& params, double alpha_mag, uint64_t seed,
-                                          double eps, int device)
+                                          double eps, int device, RemanentMagnetizationMode mode)
 {
     CalibrationResult out;
     out.alpha = alpha_mag;
     if (mdata.empty() || alpha_mag <= 0.0) return out;
     FusedJointObjective f(model, gdata, mdata, inc_deg, dec_deg, field_nT, alpha_mag,
                           Precision::FP64, device);
+    if (mode != RemanentMagnetizationMode::EffectiveSusceptibility) f.setRemanenceMode(mode);
     const size_t np = params.size(), ns = gdata.size();
     Vec wgt_g(ns), wgt_m(ns);
     for (size_t i = 0; i < ns; ++i) {
@@ -1330,7 +1372,7 @@ namespace {
 // min over p in [lo, hi] of 0.5 |w (obs - U p)|^2 / s^2 + 0.5 lambda |p - p0|^2 with the
 // reference's optimizer settings; U column-major (n_obs x n_p)
 Vec bounded_unit_fit(const std::vector<double>& U, size_t nObs, const Vec& obs, const Vec& sd,
-                     const Vec& p0, const PropertyInversionOptions& o)
+                     const Vec& p0, const Vec& lo, const Vec& hi, const PropertyInversionOptions& o)
 {
     const size_t np = p0.size();
     double var = 0.0;
@@ -1372,7 +1414,6 @@ Vec bounded_unit_fit(const std::vector<double>& U, size_t nObs, const Vec& obs, 
     opt.setMaxIterations(o.maxIter);
     opt.setHistorySize(std::min<int>(5, (int)np));
     opt.setTolerance(o.tolerance);
-    const Vec lo(np, o.lower), hi(np, o.upper);
     return opt.minimize(f, g, p0, lo, hi).params;
 }
 }  // namespace
@@ -1394,7 +1435,7 @@ Vec invertUnitDensities(GravityForward& fwd, const GravityData& data, LithologyM
         sd[i] = data[i].g_std;
     }
     for (int g = 0; g < ng; ++g) p0[g] = model.group(g).density;
-    const Vec p = bounded_unit_fit(U, n, obs, sd, p0, o);
+    const Vec p = bounded_unit_fit(U, n, obs, sd, p0, Vec(ng, o.lower), Vec(ng, o.upper), o);
     for (int g = 0; g < ng; ++g) model.group(g).density = p[g];
     fwd.refreshProperties();
     return p;
@@ -1403,22 +1444,53 @@ Vec invertUnitDensities(GravityForward& fwd, const GravityData& data, LithologyM
 Vec invertUnitSusceptibilities(MagneticForward& fwd, const MagneticData& data,
                                LithologyModel& model, const PropertyInversionOptions& o)
 {
+    // runner.cpp:801-935: chi per unit, plus the remanent parameters of the mode
+    const RemanentMagnetizationMode mode = fwd.remanenceMode();
+    const int per = mode == RemanentMagnetizationMode::FixedVectorPerGroup ? 2
+                    : mode == RemanentMagnetizationMode::VectorPerGroup    ? 4
+                                                                           : 1;
     const int ng = model.groupCount();
-    const size_t n = data.size();
-    std::vector<double> U((size_t)ng * n);
+    const size_t n = data.size(), np = (size_t)per * ng;
+    std::vector<double> U(np * n);
     fwd.syncGeometry();  // the responses of the current geometry
+    auto put = [&](size_t col, const Vec& r) { std::copy(r.begin(), r.end(), U.begin() + col * n); };
+    const double axes[3][3] = {{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
+    Vec p0(np), lo(np), hi(np);
     for (int g = 0; g < ng; ++g) {
-        const Vec col = fwd.computeGroupUnitResponse(g);
-        std::copy(col.begin(), col.end(), U.begin() + (size_t)g * n);
+        const size_t c = (size_t)per * g;
+        const LithoGroup& G = model.group(g);
+        put(c, fwd.computeGroupUnitResponse(g));
+        p0[c] = G.susceptibility;
+        lo[c] = o.lower;
+        hi[c] = o.upper;
+        if (per == 2) {
+            put(c + 1, fwd.computeGroupUnitRemanenceResponse(g));
+            p0[c + 1] = G.remanence_magnitude;
+            lo[c + 1] = o.remanenceLower;
+            hi[c + 1] = o.remanenceUpper;
+        } else if (per == 4) {
+            for (int k = 0; k < 3; ++k) {
+                put(c + 1 + k, fwd.computeGroupUnitResponseDirection(g, axes[k]));
+                p0[c + 1 + k] = G.magnetization[k];
+                lo[c + 1 + k] = o.componentLower;
+                hi[c + 1 + k] = o.componentUpper;
+            }
+        }
     }
-    Vec obs(n), sd(n), p0(ng);
+    Vec obs(n), sd(n);
     for (size_t i = 0; i < n; ++i) {
         obs[i] = data[i].t_obs;
         sd[i] = data[i].t_std;
     }
-    for (int g = 0; g < ng; ++g) p0[g] = model.group(g).susceptibility;
-    const Vec p = bounded_unit_fit(U, n, obs, sd, p0, o);
-    for (int g = 0; g < ng; ++g) model.group(g).susceptibility = p[g];
+    const Vec p = bounded_unit_fit(U, n, obs, sd, p0, lo, hi, o);
+    for (int g = 0; g < ng; ++g) {
+        const size_t c = (size_t)per * g;
+        LithoGroup& G = model.group(g);
+        G.susceptibility = p[c];
+        if (per == 2) G.remanence_magnitude = p[c + 1];
+        if (per == 4)
+            for (int k = 0; k < 3; ++k) G.magnetization[k] = p[c + 1 + k];
+    }
     fwd.refreshProperties();
     return p;
 }

@@ -77,6 +77,14 @@ struct LithoGroup {
     double magnetization[3] = {0, 0, 0};
 };
 
+// magnetic_forward.cpp:100-134: the unit's magnetization M = chi F/100 along F-hat (induced),
+// plus the remanent part of the mode (FixedVectorPerGroup: remanence_magnitude along the
+// remanence inclination / declination, VectorPerGroup: the magnetization vector), returned
+// as amplitude |M| and direction (F-hat when |M| < 1e-15).  Units as in the reference:
+// the remanent part is added to the chi F/100 amplitude as it is given.
+void groupMagnetization(const LithoGroup& G, RemanentMagnetizationMode mode, double field_nT,
+                        const double fhat[3], double& amp, double dir[3]);
+
 // Lithology model: shared vertices, per-facet (front, back) units, DOF map.
 class LithologyModel {
 public:
@@ -241,17 +249,29 @@ public:
                     double inc_deg, double dec_deg, double field_nT,
                     Precision prec = Precision::FP64, bool textbook = false, int device = 0);
     Vec compute(const Vec& params) override;
+    // unit-susceptibility (induced) response of one group at the current geometry
+    // (magnetic_forward.cpp:148-162: amplitude F/100 along F-hat)
     Vec computeGroupUnitResponse(int groupIndex);
+    // response of one group magnetized with unit amplitude along magDirection
+    // (magnetic_forward.cpp:164-178)
+    Vec computeGroupUnitResponseDirection(int groupIndex, const double magDirection[3]);
+    // the same along the group's remanence direction (magnetic_forward.cpp:180-190)
+    Vec computeGroupUnitRemanenceResponse(int groupIndex);
     void buildGroupMagnetization(int g, double& amp, double dir[3]) const;
     void setRemanenceMode(RemanentMagnetizationMode m) { m_mode = m; pushProperties(); }
+    RemanentMagnetizationMode remanenceMode() const { return m_mode; }
     const double* earthFieldUnitVector() const { return m_fhat; }
+    double fieldStrength() const { return m_field; }
 
 protected:
     void buildProperties(std::vector<double>& dens, std::vector<double>& mag) const override;
+    Vec unitResponse(int g, double amp, const double dir[3]);
     double m_fhat[3];
     double m_field;
     RemanentMagnetizationMode m_mode = RemanentMagnetizationMode::EffectiveSusceptibility;
-    int m_unitOnly = -1;
+    int m_unitOnly = -1;           // >= 0: only this group is magnetized, m_unitAmp along m_unitDir
+    double m_unitAmp = 0.0;
+    double m_unitDir[3] = {0.0, 0.0, 1.0};
 };
 
 // modules/litho-regularization/include/litho_invert/regularization/regularization.h:6-15
@@ -402,6 +422,8 @@ public:
     void refreshProperties();
     double alphaMag() const { return m_alpha; }
     void setAlphaMag(double a) { m_alpha = a; }
+    // how unit magnetizations are built (groupMagnetization); re-reads the properties
+    void setRemanenceMode(RemanentMagnetizationMode m) { m_mode = m; refreshProperties(); }
     // sum_i wg[i] dgz_i/dp + wt[i] dTMI_i/dp at params (either weight vector may be empty)
     Vec jtw(const Vec& params, const Vec& wg, const Vec& wt);
 
@@ -409,6 +431,7 @@ private:
     void ensureForward(const Vec& params);
     double m_field = 50000.0;
     double m_fhat[3] = {0.0, 0.0, 1.0};
+    RemanentMagnetizationMode m_mode = RemanentMagnetizationMode::EffectiveSusceptibility;
     std::shared_ptr<LithologyModel> m_model;
     const GravityData& m_g;
     const MagneticData& m_m;
@@ -421,20 +444,29 @@ private:
 };
 
 // The property phase of the reference's inversion loop, modules/litho-inversion/src/
-// inversion/runner.cpp:708-800 (densities from gravity) and :801-935 (susceptibilities
-// from TMI, induced magnetization): unit responses U (one forward per unit with a unit
-// property), then a bounded L-BFGS on
+// inversion/runner.cpp:708-800 (densities from gravity) and :801-935 (magnetic properties
+// from TMI): unit responses U (one forward per unit and column with a unit property), then
+// a bounded L-BFGS on
 //   0.5 |W (d_obs - U p)|^2 / rms(d_obs)^2 + 0.5 damping |p - p_start|^2
-// (rms floored at 1, W = 1/std or 1), history min(5, n_units).  The result is written into
-// the model's units.
+// (rms floored at 1, W = 1/std or 1), history min(5, n_params).  The result is written into
+// the model's units.  Magnetic parameters per unit follow the forward's remanence mode
+// (runner.cpp:808-857): chi (EffectiveSusceptibility); chi and the remanence magnitude
+// (FixedVectorPerGroup, columns: unit-chi response, unit response along the unit's
+// remanence direction); chi and Mx, My, Mz (VectorPerGroup, columns: unit-chi response,
+// unit responses along x, y, z).  Bounds: [lower, upper] for densities and
+// susceptibilities, [remanenceLower, remanenceUpper] for magnitudes, [componentLower,
+// componentUpper] for components (runner.h:94-107 defaults 0..10 and -10..10).
 struct PropertyInversionOptions {
     double damping = 0.0;
     double lower = -1e300, upper = 1e300;
     int maxIter = 50;
     double tolerance = 1e-6;
+    double remanenceLower = 0.0, remanenceUpper = 10.0;
+    double componentLower = -10.0, componentUpper = 10.0;
 };
 Vec invertUnitDensities(GravityForward& fwd, const GravityData& data, LithologyModel& model,
                         const PropertyInversionOptions& opt);
+// returns the parameters in the layout above (params per unit x units)
 Vec invertUnitSusceptibilities(MagneticForward& fwd, const MagneticData& data,
                                LithologyModel& model, const PropertyInversionOptions& opt);
 
@@ -455,7 +487,9 @@ CalibrationResult calibrateMagneticWeight(std::shared_ptr<LithologyModel> model,
                                           const GravityData& gdata, const MagneticData& mdata,
                                           double inc_deg, double dec_deg, double field_nT,
                                           const Vec& params, double alpha_mag,
-                                          uint64_t seed = 1, double eps = 1e-3, int device = 0);
+                                          uint64_t seed = 1, double eps = 1e-3, int device = 0,
+                                          RemanentMagnetizationMode mode =
+                                              RemanentMagnetizationMode::EffectiveSusceptibility);
 
 // modules/litho-inversion/include/litho_invert/inversion/optimizer.h:7-36
 struct OptimizerResult {

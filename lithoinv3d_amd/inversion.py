@@ -81,6 +81,13 @@ def load_host():
                                     ctypes.c_char_p, i64]
     L.li_durbin_watson.argtypes = [i64, _P, _P, _P]
     L.li_model_properties.argtypes = [_P, _P, _P]
+    L.li_model_set_remanence.argtypes = [_P, _P, _P, _P, _P]
+    L.li_model_remanence.argtypes = [_P, _P, _P, _P, _P]
+    L.li_problem_unit_response_direction.argtypes = [_P, ctypes.c_int, _P, _P]
+    L.li_problem_unit_remanence_response.argtypes = [_P, ctypes.c_int, _P]
+    L.li_problem_set_remanence_mode.argtypes = [_P, ctypes.c_int]
+    L.li_problem_set_remanence_bounds.argtypes = [_P, ctypes.c_double, ctypes.c_double,
+                                                  ctypes.c_double, ctypes.c_double]
     L.li_model_fix_exterior_faces.argtypes = [_P, ctypes.c_int]
     L.li_model_freedom.argtypes = [_P, _P]
     L.li_model_model_bounds.argtypes = [_P, ctypes.c_double, _P, _P]
@@ -262,6 +269,25 @@ class Model:
         _chk(load_host().li_model_properties(self._h, _p(d), _p(c)), "properties")
         return d, c
 
+    def set_remanence(self, magnitude=None, inclination=None, declination=None, vector=None):
+        """Per-unit remanence (LithoGroup, litho_group.h:18-24): magnitude (reference
+        units, added to the chi F/100 amplitude), inclination / declination in degrees
+        (inclination positive down) of FixedVectorPerGroup, and the remanent vector
+        (n_units x 3) of VectorPerGroup; None keeps a field."""
+        n = self.src.n_units
+        arrs = [None if magnitude is None else _a(magnitude).reshape(n),
+                None if inclination is None else _a(inclination).reshape(n),
+                None if declination is None else _a(declination).reshape(n),
+                None if vector is None else _a(vector).reshape(n, 3)]
+        _chk(load_host().li_model_set_remanence(self._h, *[_p(a) for a in arrs]), "set_remanence")
+
+    def remanence(self):
+        """(magnitude, inclination, declination, vector) of every unit."""
+        n = self.src.n_units
+        m, i, d, v = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros((n, 3))
+        _chk(load_host().li_model_remanence(self._h, _p(m), _p(i), _p(d), _p(v)), "remanence")
+        return m, i, d, v
+
     def classify(self, points, grid=None, bottom_depth=-5000.0):
         """Unit containing each point (classifyPoint): the label grid's where it holds a
         unit, else by +x ray casting through the unit surfaces; -1 outside."""
@@ -417,18 +443,41 @@ class Problem:
         except Exception:
             pass
 
+    REMANENCE_MODES = {"effective": 0, "fixed_vector": 1, "vector": 2}
+
+    def set_remanence_mode(self, mode="effective"):
+        """MagneticForward::setRemanenceMode for every magnetic forward of the problem:
+        "effective" (induced only, chi absorbs remanence), "fixed_vector" (each unit's
+        remanence magnitude along its remanence inclination / declination) or "vector"
+        (each unit's remanent vector); litho_group.h:7-11."""
+        self.remanence_mode = mode
+        _chk(load_host().li_problem_set_remanence_mode(self._h, self.REMANENCE_MODES[mode]),
+             "set_remanence_mode")
+
+    def set_remanence_bounds(self, magnitude=(0.0, 10.0), component=(-10.0, 10.0)):
+        """Bounds of the remanence parameters in the magnetic property phase."""
+        _chk(load_host().li_problem_set_remanence_bounds(self._h, magnitude[0], magnitude[1],
+                                                         component[0], component[1]),
+             "set_remanence_bounds")
+
     def invert_properties(self, kind="density", damping=0.0, lower=-1e300, upper=1e300,
                           max_iter=50, tol=1e-6):
         """The property phase of the inversion loop at the model's current geometry:
-        per-unit densities from gravity ("density") or susceptibilities from TMI
-        ("susceptibility"), bounded, with optional damping toward the current values.
-        Returns the new values; the model and every forward of the problem take them."""
+        per-unit densities from gravity ("density"), or magnetic properties from TMI
+        ("susceptibility"): chi per unit, with the remanence magnitude ("fixed_vector"
+        mode) or the remanent vector ("vector" mode) after it, as rows per unit.  lower /
+        upper bound densities and susceptibilities.  Returns the new values (n_units or
+        n_units x k); the model and every forward of the problem take them."""
         which = {"density": 0, "susceptibility": 1}[kind]
-        out = np.zeros(self.model.src.n_units)
+        per = 1
+        if which == 1:
+            per = {"effective": 1, "fixed_vector": 2, "vector": 4}[
+                getattr(self, "remanence_mode", "effective")]
+        out = np.zeros(self.model.src.n_units * per)
         _chk(load_host().li_problem_invert_properties(self._h, which, damping, lower, upper,
                                                       max_iter, tol, _p(out)),
              "invert_properties")
-        return out
+        return out if per == 1 else out.reshape(-1, per)
 
     def calibrate(self, params=None, seed=1, eps=1e-3):
         """Sets alpha_mag from the ratio of the largest eigenvalues of J^T W^2 J of gravity
@@ -533,6 +582,22 @@ class Problem:
         out = np.zeros(self.n_s)
         _chk(load_host().li_problem_unit_response(self._h, 0 if which == "gz" else 1, group,
                                                   _p(out)), "unit_response")
+        return out
+
+    def unit_response_direction(self, group, direction):
+        """TMI of one unit magnetized with unit amplitude along `direction`
+        (computeGroupUnitResponseDirection)."""
+        out = np.zeros(self.n_s)
+        d = _a(direction).reshape(3)
+        _chk(load_host().li_problem_unit_response_direction(self._h, group, _p(d), _p(out)),
+             "unit_response_direction")
+        return out
+
+    def unit_remanence_response(self, group):
+        """The same along the unit's remanence direction (computeGroupUnitRemanenceResponse)."""
+        out = np.zeros(self.n_s)
+        _chk(load_host().li_problem_unit_remanence_response(self._h, group, _p(out)),
+             "unit_remanence_response")
         return out
 
     def minimize(self, x0, lb, ub, max_iter=50, history=10, ftol=1e-6, c1=1e-4, target=None,

@@ -27,8 +27,10 @@
  *                        central finite differences (0.01 m) of the pinned forwards on
  *                        every vertex coordinate                   (test_oracle_kat.py)
  *   lfo_group_magnetization
- *                        closed-form direction cosines; modes 1 and 2 parity unpinned
- *                        beyond that (no reference test uses remanence)
+ *                        closed-form direction cosines; in all three modes: amplitudes
+ *                        that add, subtract or meet at right angles, exact cancellation
+ *                        (the |M| < 1e-15 fallback to F-hat), each mode ignoring the
+ *                        other's inputs; linearity of the TMI in M  (test_oracle_kat.py)
  */
 #include "lithofwd_oracle.h"
 #include <math.h>

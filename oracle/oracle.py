@@ -103,6 +103,21 @@ def field_unit_vector(inc, dec):
     return out
 
 
+def group_magnetization(mode, susceptibility, field_nT, fhat, rem_magnitude=0.0, rem_inc=0.0,
+                        rem_dec=0.0, rem_vector=(0.0, 0.0, 0.0)):
+    """lfo_group_magnetization (magnetic_forward.cpp:100-134): (amplitude, direction) of a
+    unit's magnetization in remanence mode 0 (induced only), 1 (plus rem_magnitude along
+    (rem_inc, rem_dec)) or 2 (plus rem_vector)."""
+    fh = _v(fhat)
+    rv = _v(rem_vector)
+    amp = np.zeros(1)
+    mdir = np.zeros(3)
+    lib().lfo_group_magnetization(int(mode), float(susceptibility), float(field_nT), _p(fh),
+                                  float(rem_magnitude), float(rem_inc), float(rem_dec), _p(rv),
+                                  _p(amp), _p(mdir))
+    return float(amp[0]), mdir
+
+
 class OracleModel:
     """Holds the reference layout (one closed mesh per group) for the C oracle."""
 
@@ -124,11 +139,26 @@ class OracleModel:
                          _p(self.density), _p(self.mag_amp), _p(self.mag_dir))
 
     @classmethod
-    def from_litho(cls, m, field_nT=50000.0, fhat=None):
-        tris = [m.group_triangles(g) for g in range(m.n_units)]
+    def from_litho(cls, m, field_nT=50000.0, fhat=None, remanence_mode=0, rem_magnitude=None,
+                   rem_inc=None, rem_dec=None, rem_vector=None):
+        """The reference layout of a synth.LithoModel (one closed mesh per unit) with each
+        unit's magnetization from group_magnetization (the restatement of
+        buildGroupMagnetization): induced from the unit's susceptibility, plus the unit's
+        remanence in modes 1 and 2 (per-unit arrays; absent ones are zero)."""
+        n = m.n_units
+        tris = [m.group_triangles(g) for g in range(n)]
         if fhat is None:
             fhat = np.array([0.0, 0.0, 1.0])
-        amp, mdir = m.magnetization(field_nT, fhat)
+        z = np.zeros(n)
+        rm = z if rem_magnitude is None else np.asarray(rem_magnitude, float)
+        ri = z if rem_inc is None else np.asarray(rem_inc, float)
+        rd = z if rem_dec is None else np.asarray(rem_dec, float)
+        rv = np.zeros((n, 3)) if rem_vector is None else np.asarray(rem_vector, float).reshape(n, 3)
+        amp = np.zeros(n)
+        mdir = np.zeros((n, 3))
+        for g in range(n):
+            amp[g], mdir[g] = group_magnetization(remanence_mode, m.susceptibility[g], field_nT,
+                                                  fhat, rm[g], ri[g], rd[g], rv[g])
         return cls(m.vertices, tris, m.density, amp, mdir)
 
     def gravity(self, stations, threads=0):

@@ -283,6 +283,93 @@ def test_field_unit_vector_and_magnetization(oracle):
                                oracle.field_unit_vector(-60.0, 0.0), atol=1e-15)
 
 
+def test_group_magnetization_closed_forms(oracle):
+    """lfo_group_magnetization in its three modes against cases whose magnetization is
+    fixed by hand: induced only (amplitude chi F/100 along F-hat); remanence along the
+    field (amplitudes add), against it (they subtract, the direction flips when the
+    remanence wins), at right angles (Pythagoras, direction in the plane of the two);
+    a vector that cancels the induced part (|M| < 1e-15: the direction falls back to
+    F-hat); and each mode ignoring the other mode's inputs (filled with junk here)."""
+    F = 50000.0
+    inc, dec = -60.0, 20.0
+    fh = oracle.field_unit_vector(inc, dec)
+    junk_v = np.array([3.0, -7.0, 11.0])
+    # mode 0: induced only, whatever the remanence inputs hold
+    for chi in (0.02, 1e-4, 0.3):
+        a, d = oracle.group_magnetization(0, chi, F, fh, 9.0, 45.0, 77.0, junk_v)
+        assert a == pytest.approx(chi * F / 100.0, rel=1e-15)
+        np.testing.assert_allclose(d, fh, atol=1e-15)
+    a, d = oracle.group_magnetization(0, 0.0, F, fh)
+    assert a == 0.0
+    np.testing.assert_allclose(d, fh, atol=0)
+    chi = 0.01
+    ind = chi * F / 100.0  # 5
+    # mode 1, remanence along the field: amplitudes add
+    a, d = oracle.group_magnetization(1, chi, F, fh, 2.5, inc, dec, junk_v)
+    assert a == pytest.approx(ind + 2.5, rel=1e-14)
+    np.testing.assert_allclose(d, fh, atol=1e-14)
+    # against the field (inclination negated, declination turned by 180 degrees)
+    a, d = oracle.group_magnetization(1, chi, F, fh, 2.0, -inc, dec + 180.0, junk_v)
+    assert a == pytest.approx(ind - 2.0, rel=1e-14)
+    np.testing.assert_allclose(d, fh, atol=1e-14)
+    a, d = oracle.group_magnetization(1, chi, F, fh, 8.0, -inc, dec + 180.0, junk_v)
+    assert a == pytest.approx(8.0 - ind, rel=1e-14)
+    np.testing.assert_allclose(d, -fh, atol=1e-14)
+    # exact cancellation (a small induced part keeps the rounding residue below 1e-15)
+    a, d = oracle.group_magnetization(1, 1e-5, F, fh, 1e-5 * F / 100.0, -inc, dec + 180.0)
+    assert a < 1e-15
+    np.testing.assert_allclose(d, fh, atol=0)
+    # at right angles: remanence horizontal and perpendicular to F's declination plane
+    r_perp = oracle.field_unit_vector(0.0, dec + 90.0)
+    assert abs(r_perp @ fh) < 1e-15
+    a, d = oracle.group_magnetization(1, chi, F, fh, 3.0, 0.0, dec + 90.0, junk_v)
+    assert a == pytest.approx(math.hypot(ind, 3.0), rel=1e-14)
+    assert d @ fh == pytest.approx(ind / math.hypot(ind, 3.0), rel=1e-14)
+    assert d @ r_perp == pytest.approx(3.0 / math.hypot(ind, 3.0), rel=1e-14)
+    # vertical remanence: inclination +90 points down (z up)
+    a, d = oracle.group_magnetization(1, 0.0, F, fh, 4.0, 90.0, 0.0)
+    assert a == pytest.approx(4.0, rel=1e-15)
+    np.testing.assert_allclose(d, [0.0, 0.0, -1.0], atol=1e-15)
+    # mode 2: the vector, magnitude / inclination / declination ignored
+    v = 1.5 * np.cross(fh, [0.0, 0.0, 1.0]) / np.linalg.norm(np.cross(fh, [0.0, 0.0, 1.0]))
+    a, d = oracle.group_magnetization(2, chi, F, fh, 9.0, 45.0, 77.0, v)
+    assert a == pytest.approx(math.hypot(ind, 1.5), rel=1e-14)
+    np.testing.assert_allclose(d * a, ind * fh + v, atol=1e-13)
+    a, d = oracle.group_magnetization(2, chi, F, fh, 9.0, 45.0, 77.0, -ind * fh)
+    assert a < 1e-15
+    np.testing.assert_allclose(d, fh, atol=0)
+
+
+def test_magnetization_superposition(oracle):
+    """The TMI of a unit is linear in its magnetization vector M = amp * dir, in both
+    coefficient formulas: a unit magnetized along an arbitrary vector equals the sum of its
+    unit-amplitude responses along x, y, z weighted by the vector's components (the
+    property phase's remanent-vector columns rest on this, runner.cpp:815-823)."""
+    rng = np.random.default_rng(14)
+    labels = rng.integers(0, 3, size=(4, 3, 2)).astype(np.int32)
+    m = synth.label_grid_model(labels, (-50.0, -37.5, -31.0), (25.0, 25.0, 25.0),
+                               [0.2, -0.1, 0.3], [0.01, 0.002, 0.03])
+    st = synth.topo_stations(6, 150.0)
+    fh = synth.field_unit_vector(-60.0, 15.0)
+    tris = [m.group_triangles(g) for g in range(m.n_units)]
+    M = np.array([[0.7, -1.2, 2.1], [0.0, 0.0, 0.0], [-0.4, 0.9, 0.5]])
+    for textbook in (False, True):
+        amp = np.linalg.norm(M, axis=1)
+        mdir = np.where(amp[:, None] > 0, M / np.where(amp > 0, amp, 1)[:, None], fh)
+        full = oracle.OracleModel(m.vertices, tris, m.density, amp, mdir).magnetic(
+            fh, st, textbook=textbook)
+        parts = np.zeros_like(full)
+        for g in range(m.n_units):
+            for k in range(3):
+                amp_u = np.zeros(m.n_units)
+                amp_u[g] = 1.0
+                dir_u = np.tile(np.eye(3)[k], (m.n_units, 1))
+                r = oracle.OracleModel(m.vertices, tris, m.density, amp_u, dir_u).magnetic(
+                    fh, st, textbook=textbook)
+                parts += M[g, k] * r
+        np.testing.assert_allclose(full, parts, rtol=1e-10, atol=1e-9 * np.abs(full).max())
+
+
 # ------------------------------------------------------------------ FD Jacobian
 
 def test_vertex_jacobian_central_fd(oracle):
