@@ -50,8 +50,19 @@ int li_model_fix_exterior(void* h, int interior_freedom);
 int li_model_fix_exterior_faces(void* h, int freedom);
 /* the current freedom of each vertex (n_v values) */
 int li_model_freedom(void* h, uint8_t* freedom);
-/* setVertexFreedom per vertex (n_v values, same codes) */
+/* setVertexFreedom per vertex (n_v values, same codes, plus 4 ALONG_VECTOR: one
+ * parameter, the vertex's projection on its move vector, surface_mesh.cpp:125-128,
+ * 194-196) */
 int li_model_set_freedom(void* h, const uint8_t* freedom);
+/* the move vectors of ALONG_VECTOR vertices (3*n_v; used normalized, zero = no motion;
+ * surface_mesh.h Vertex::moveVector) */
+int li_model_set_move_vectors(void* h, const double* move_vectors);
+/* how non-control vertices follow the control points: 0 displacement form (default: the
+ * control points' IDW-weighted displacement along the vertex's own free directions), 1 the
+ * reference's position form (interpolateFromControlPoints, surface_mesh.cpp:610-623: every
+ * non-control vertex, FIXED ones included, becomes the IDW-weighted average of its control
+ * points' positions, even for an unchanged parameter vector) */
+int li_model_set_cp_interpolation(void* h, int mode);
 /* axis scale of the parameter vector (runner.cpp:241-248: params in units of s metres) */
 int li_model_set_axis_scale(void* h, int axis, double s);
 /* control points on a stride*cell grid (lithology_model.cpp:864-1040, IDW K = 4 of

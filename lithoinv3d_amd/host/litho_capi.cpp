@@ -159,6 +159,23 @@ int li_model_remanence(void* h, double* magnitude, double* inc_deg, double* dec_
     });
 }
 
+int li_model_set_move_vectors(void* h, const double* move_vectors)
+{
+    return guard([&] {
+        auto& m = *((ModelHandle*)h)->m;
+        for (size_t v = 0; v < m.vertexCount(); ++v) m.setMoveVector(v, move_vectors + 3 * v);
+    });
+}
+
+int li_model_set_cp_interpolation(void* h, int mode)
+{
+    return guard([&] {
+        if (mode != 0 && mode != 1)
+            throw std::invalid_argument("li_model_set_cp_interpolation: mode must be 0 or 1");
+        ((ModelHandle*)h)->m->setControlPointInterpolation((ControlPointInterpolation)mode);
+    });
+}
+
 int li_model_fix_exterior(void* h, int interior_freedom)
 {
     return guard([&] { ((ModelHandle*)h)->m->fixExteriorVertices((VertexFreedom)interior_freedom); });

@@ -58,6 +58,7 @@ LithologyModel::LithologyModel(std::vector<double> vertices, std::vector<uint32_
         throw std::invalid_argument("LithologyModel: inconsistent array sizes");
     m_rest = m_vertices;
     m_freedom.assign(vertexCount(), VertexFreedom::FIXED);
+    m_moveVector.assign(3 * vertexCount(), 0.0);
     m_isCP.assign(vertexCount(), 1);
     rebuildDofs();
 }
@@ -66,6 +67,14 @@ void LithologyModel::setVertexFreedom(size_t v, VertexFreedom f)
 {
     m_freedom.at(v) = f;
     rebuildDofs();
+}
+
+void LithologyModel::setMoveVector(size_t v, const double mv[3])
+{
+    if (v >= vertexCount()) throw std::out_of_range("setMoveVector: vertex out of range");
+    // Vector3d::normalized() (surface_mesh.cpp:195, 218); zero stays zero
+    const double n = std::sqrt(mv[0] * mv[0] + mv[1] * mv[1] + mv[2] * mv[2]);
+    for (int a = 0; a < 3; ++a) m_moveVector[3 * v + a] = n > 0.0 ? mv[a] / n : 0.0;
 }
 
 void LithologyModel::fixExteriorVertices(VertexFreedom interior)
@@ -151,6 +160,18 @@ bool axisFree(VertexFreedom f, int a)
     }
 }
 }  // namespace
+
+void LithologyModel::projectFree(size_t v, double d[3]) const
+{
+    if (m_freedom[v] == VertexFreedom::ALONG_VECTOR) {
+        const double* u = &m_moveVector[3 * v];
+        const double t = d[0] * u[0] + d[1] * u[1] + d[2] * u[2];
+        for (int a = 0; a < 3; ++a) d[a] = t * u[a];
+        return;
+    }
+    for (int a = 0; a < 3; ++a)
+        if (!axisFree(m_freedom[v], a)) d[a] = 0.0;
+}
 
 namespace {
 // body(i) for i in [0, n) on up to 16 threads
@@ -433,7 +454,13 @@ void LithologyModel::rebuildDofs()
     m_dofAxis.clear();
     for (size_t v = 0; v < vertexCount(); ++v) {
         if (m_cpStride > 0 && !m_isCP[v]) continue;
-        // one parameter per free axis, x before y before z (surface_mesh.cpp:91-172)
+        // one parameter per free axis, x before y before z; one for ALONG_VECTOR, recorded
+        // on axis 0 as the reference does (surface_mesh.cpp:91-172)
+        if (m_freedom[v] == VertexFreedom::ALONG_VECTOR) {
+            m_dofVertex.push_back((int)v);
+            m_dofAxis.push_back(0);
+            continue;
+        }
         for (int a = 0; a < 3; ++a)
             if (axisFree(m_freedom[v], a)) {
                 m_dofVertex.push_back((int)v);
@@ -442,11 +469,24 @@ void LithologyModel::rebuildDofs()
     }
 }
 
+bool LithologyModel::dofAlongVector(size_t d) const
+{
+    return m_freedom[(size_t)m_dofVertex[d]] == VertexFreedom::ALONG_VECTOR;
+}
+
 Vec LithologyModel::assembleParameterVector() const
 {
+    // extractParams (surface_mesh.cpp:211-225)
     Vec p(m_dofVertex.size());
-    for (size_t d = 0; d < p.size(); ++d)
-        p[d] = m_vertices[3 * (size_t)m_dofVertex[d] + m_dofAxis[d]] / m_axisScale[m_dofAxis[d]];
+    for (size_t d = 0; d < p.size(); ++d) {
+        const double* x = &m_vertices[3 * (size_t)m_dofVertex[d]];
+        if (dofAlongVector(d)) {
+            const double* u = &m_moveVector[3 * (size_t)m_dofVertex[d]];
+            p[d] = x[0] * u[0] + x[1] * u[1] + x[2] * u[2];
+        } else {
+            p[d] = x[m_dofAxis[d]] / m_axisScale[m_dofAxis[d]];
+        }
+    }
     return p;
 }
 
@@ -454,43 +494,70 @@ void LithologyModel::applyParameterVector(const Vec& params)
 {
     if (params.size() != m_dofVertex.size())
         throw std::invalid_argument("applyParameterVector: wrong parameter count");
-    for (size_t d = 0; d < params.size(); ++d)
-        m_vertices[3 * (size_t)m_dofVertex[d] + m_dofAxis[d]] = params[d] * m_axisScale[m_dofAxis[d]];
-    if (m_cpStride > 0) {
-        // displacement of each control point, spread to its dependants along the
-        // dependant's own free axes (a FIXED vertex never moves)
-        for (size_t v = 0; v < vertexCount(); ++v) {
-            if (m_isCP[v]) continue;
-            for (int a = 0; a < 3; ++a) {
-                if (!axisFree(m_freedom[v], a)) continue;
-                double disp = 0.0;
-                for (int64_t e = m_wOff[v]; e < m_wOff[v + 1]; ++e) {
-                    const size_t c = (size_t)m_wCp[e];
-                    disp += m_w[e] * (m_vertices[3 * c + a] - m_rest[3 * c + a]);
-                }
-                m_vertices[3 * v + a] = m_rest[3 * v + a] + disp;
+    // applyParams (surface_mesh.cpp:188-209)
+    for (size_t d = 0; d < params.size(); ++d) {
+        double* x = &m_vertices[3 * (size_t)m_dofVertex[d]];
+        if (dofAlongVector(d)) {
+            const double* u = &m_moveVector[3 * (size_t)m_dofVertex[d]];
+            const double t = params[d] - (x[0] * u[0] + x[1] * u[1] + x[2] * u[2]);
+            for (int a = 0; a < 3; ++a) x[a] += t * u[a];
+        } else {
+            x[m_dofAxis[d]] = params[d] * m_axisScale[m_dofAxis[d]];
+        }
+    }
+    if (m_cpStride <= 0) return;
+    for (size_t v = 0; v < vertexCount(); ++v) {
+        if (m_isCP[v]) continue;
+        double* x = &m_vertices[3 * v];
+        if (m_cpInterp == ControlPointInterpolation::Position) {
+            // interpolateFromControlPoints (surface_mesh.cpp:610-623)
+            double pos[3] = {0.0, 0.0, 0.0};
+            for (int64_t e = m_wOff[v]; e < m_wOff[v + 1]; ++e) {
+                const size_t c = (size_t)m_wCp[e];
+                for (int a = 0; a < 3; ++a) pos[a] += m_w[e] * m_vertices[3 * c + a];
             }
+            for (int a = 0; a < 3; ++a) x[a] = pos[a];
+        } else {
+            // the control points' weighted displacement, along the vertex's own free
+            // directions (a FIXED vertex never moves)
+            double disp[3] = {0.0, 0.0, 0.0};
+            for (int64_t e = m_wOff[v]; e < m_wOff[v + 1]; ++e) {
+                const size_t c = (size_t)m_wCp[e];
+                for (int a = 0; a < 3; ++a)
+                    disp[a] += m_w[e] * (m_vertices[3 * c + a] - m_rest[3 * c + a]);
+            }
+            projectFree(v, disp);
+            for (int a = 0; a < 3; ++a) x[a] = m_rest[3 * v + a] + disp[a];
         }
     }
 }
 
 Vec LithologyModel::pullBackVertexGradient(const Vec& vg) const
 {
-    // dPhi/dx_cp += sum over dependants w * dPhi/dx_v
+    // dPhi/dx_cp += sum over dependants w * (dPhi/dx_v, projected on the dependant's free
+    // directions in displacement form)
     std::vector<double> eff(vg.begin(), vg.end());
     if (m_cpStride > 0) {
         for (size_t v = 0; v < vertexCount(); ++v) {
             if (m_isCP[v]) continue;
+            double gv[3] = {vg[3 * v], vg[3 * v + 1], vg[3 * v + 2]};
+            if (m_cpInterp == ControlPointInterpolation::Displacement) projectFree(v, gv);
             for (int64_t e = m_wOff[v]; e < m_wOff[v + 1]; ++e) {
                 const size_t c = (size_t)m_wCp[e];
-                for (int a = 0; a < 3; ++a)
-                    if (axisFree(m_freedom[v], a)) eff[3 * c + a] += m_w[e] * vg[3 * v + a];
+                for (int a = 0; a < 3; ++a) eff[3 * c + a] += m_w[e] * gv[a];
             }
         }
     }
     Vec g(m_dofVertex.size());
-    for (size_t d = 0; d < g.size(); ++d)
-        g[d] = eff[3 * (size_t)m_dofVertex[d] + m_dofAxis[d]] * m_axisScale[m_dofAxis[d]];
+    for (size_t d = 0; d < g.size(); ++d) {
+        const double* ge = &eff[3 * (size_t)m_dofVertex[d]];
+        if (dofAlongVector(d)) {
+            const double* u = &m_moveVector[3 * (size_t)m_dofVertex[d]];
+            g[d] = ge[0] * u[0] + ge[1] * u[1] + ge[2] * u[2];
+        } else {
+            g[d] = ge[m_dofAxis[d]] * m_axisScale[m_dofAxis[d]];
+        }
+    }
     return g;
 }
 
@@ -498,6 +565,10 @@ void LithologyModel::dofVertexMap(std::vector<int64_t>& off, std::vector<int32_t
                                   std::vector<double>& w) const
 {
     const size_t nd = m_dofVertex.size();
+    for (size_t d = 0; d < nd; ++d)
+        if (dofAlongVector(d))
+            throw std::invalid_argument(
+                "dofVertexMap: ALONG_VECTOR parameters move a vertex along a vector, not one axis");
     // dependants of each control-point vertex (transpose of the IDW lists)
     std::vector<std::vector<std::pair<int32_t, double>>> dep;
     if (m_cpStride > 0) {
@@ -506,6 +577,7 @@ void LithologyModel::dofVertexMap(std::vector<int64_t>& off, std::vector<int32_t
             for (int64_t e = m_wOff[v]; e < m_wOff[v + 1]; ++e)
                 dep[(size_t)m_wCp[e]].push_back({(int32_t)v, m_w[e]});
     }
+    const bool position = m_cpInterp == ControlPointInterpolation::Position;
     off.assign(nd + 1, 0);
     vert.clear();
     w.clear();
@@ -515,7 +587,7 @@ void LithologyModel::dofVertexMap(std::vector<int64_t>& off, std::vector<int32_t
         w.push_back(s);
         if (m_cpStride > 0)
             for (auto& pr : dep[(size_t)m_dofVertex[d]]) {
-                if (!axisFree(m_freedom[(size_t)pr.first], m_dofAxis[d])) continue;
+                if (!position && !axisFree(m_freedom[(size_t)pr.first], m_dofAxis[d])) continue;
                 vert.push_back(pr.first);
                 w.push_back(pr.second * s);
             }
@@ -528,6 +600,12 @@ void LithologyModel::getBounds(const Vec& params, double margin_m, Vec& lower, V
     lower.resize(params.size());
     upper.resize(params.size());
     for (size_t d = 0; d < params.size(); ++d) {
+        if (dofAlongVector(d)) {
+            // "basically unbounded" along a vector (surface_mesh.cpp:234-237)
+            lower[d] = -1e6;
+            upper[d] = 1e6;
+            continue;
+        }
         const double m = margin_m / m_axisScale[m_dofAxis[d]];
         lower[d] = params[d] - m;
         upper[d] = params[d] + m;
@@ -548,6 +626,11 @@ void LithologyModel::getModelBounds(double marginMultiplier, Vec& lower, Vec& up
     upper.resize(m_dofVertex.size());
     for (size_t d = 0; d < m_dofVertex.size(); ++d) {
         const int a = m_dofAxis[d];
+        if (dofAlongVector(d)) {
+            lower[d] = -1e6;
+            upper[d] = 1e6;
+            continue;
+        }
         lower[d] = (lo[a] - margin) / m_axisScale[a];
         upper[d] = (hi[a] + margin) / m_axisScale[a];
     }
@@ -917,11 +1000,11 @@ double SurfaceSmoothness::penalty(std::vector<double>* vgrad) const
                 }
             if (n == 0) continue;
             double d[3];
-            for (int a = 0; a < 3; ++a) {
-                d[a] = X[3 * (size_t)v + a] - mean[a] / n;
-                if (!axisFree(fv, a)) d[a] = 0.0;
-                pen += 0.5 * d[a] * d[a];
-            }
+            for (int a = 0; a < 3; ++a) d[a] = X[3 * (size_t)v + a] - mean[a] / n;
+            // the free axes' components, or the part along the move vector
+            // (smoothness.cpp:54-98)
+            m_model->projectFree((size_t)v, d);
+            for (int a = 0; a < 3; ++a) pen += 0.5 * d[a] * d[a];
             if (!vgrad) continue;
             // own term d, and -d/n to each same-freedom neighbour
             for (int a = 0; a < 3; ++a) (*vgrad)[3 * (size_t)v + a] += d[a];

@@ -57,12 +57,22 @@ struct MagneticPoint {
 using GravityData = std::vector<GravityPoint>;
 using MagneticData = std::vector<MagneticPoint>;
 
-// modules/litho-surface/include/litho_invert/surface/surface_mesh.h (VertexFreedom)
-// (ALONG_VECTOR = 4 of the reference is not supported)
+// modules/litho-surface/include/litho_invert/surface/surface_mesh.h (VertexFreedom);
+// ALONG_VECTOR: one parameter, the vertex's projection on its (normalized) move vector
+// (surface_mesh.cpp:125-128, 194-196, 217-218)
 enum class VertexFreedom : uint8_t {
-    FIXED = 0, Z_ONLY = 1, X_ONLY = 2, Y_ONLY = 3, XY_FREE = 5, XZ_FREE = 6, YZ_FREE = 7,
-    XYZ_FREE = 8
+    FIXED = 0, Z_ONLY = 1, X_ONLY = 2, Y_ONLY = 3, ALONG_VECTOR = 4, XY_FREE = 5, XZ_FREE = 6,
+    YZ_FREE = 7, XYZ_FREE = 8
 };
+
+// How vertices that are not control points follow the control points:
+//   Displacement: the IDW-weighted displacement of their control points, along their own
+//                 free directions (a FIXED vertex stays put); unchanged parameters leave
+//                 the geometry as it is
+//   Position:     the reference's interpolateFromControlPoints (surface_mesh.cpp:610-623):
+//                 every non-control vertex, FIXED ones included, becomes the IDW-weighted
+//                 average of its control points' positions
+enum class ControlPointInterpolation : uint8_t { Displacement = 0, Position = 1 };
 
 // modules/litho-model/include/litho_invert/litho/litho_group.h:7-34
 enum class RemanentMagnetizationMode { EffectiveSusceptibility = 0, FixedVectorPerGroup = 1, VectorPerGroup = 2 };
@@ -104,6 +114,15 @@ public:
 
     void setVertexFreedom(size_t v, VertexFreedom f);
     VertexFreedom vertexFreedom(size_t v) const { return m_freedom[v]; }
+    // the move vector of an ALONG_VECTOR vertex (surface_mesh.h Vertex::moveVector; used
+    // normalized); a zero vector gives the vertex no motion
+    void setMoveVector(size_t v, const double mv[3]);
+    const double* moveVector(size_t v) const { return &m_moveVector[3 * v]; }
+    // d -> the part of d the vertex may move along: the free axes' components, or the
+    // projection on the unit move vector (ALONG_VECTOR), or 0 (FIXED)
+    void projectFree(size_t v, double d[3]) const;
+    void setControlPointInterpolation(ControlPointInterpolation m) { m_cpInterp = m; }
+    ControlPointInterpolation controlPointInterpolation() const { return m_cpInterp; }
     // Every vertex of a facet that bounds the model (back == -1) becomes FIXED, the
     // rest takes `interior` (the role of LithologyModel::fixExteriorFaces,
     // lithology_model.cpp:369-579, for closed voxel models).
@@ -135,12 +154,17 @@ public:
     int dofVertex(uint32_t d) const { return m_dofVertex[d]; }
     int dofAxis(uint32_t d) const { return m_dofAxis[d]; }
     Vec assembleParameterVector() const;
-    // param -> vertices (lithology_model.cpp:210-248 in displacement form)
+    // param -> vertices (lithology_model.cpp:210-248, surface_mesh.cpp:188-209): a vertex
+    // moves along each of its parameters' unit directions u (an axis, or an ALONG_VECTOR
+    // move vector) until x.u = p s (s: the axis scale, 1 along a vector), then the
+    // control-point interpolation of the chosen form
     void applyParameterVector(const Vec& params);
     // vertex gradient (3*n_v) -> parameter gradient (transpose of the map above)
     Vec pullBackVertexGradient(const Vec& vgrad) const;
     // The linear map itself, per parameter: parameter d moves coordinate dofAxis(d) of
     // vertex vert[e] by w[e] (metres per parameter unit) for e in [off[d], off[d+1]).
+    // Defined for axis parameters only (an ALONG_VECTOR parameter moves a vertex along a
+    // vector, not one coordinate): throws if the model has one.
     void dofVertexMap(std::vector<int64_t>& off, std::vector<int32_t>& vert,
                       std::vector<double>& w) const;
     // params -/+ margin/axisScale (runner.cpp:241-248)
@@ -152,11 +176,14 @@ public:
 
 private:
     void rebuildDofs();
+    bool dofAlongVector(size_t d) const;
     std::vector<double> m_vertices, m_rest;  // current and initial positions
     std::vector<uint32_t> m_faces;
     std::vector<int32_t> m_front, m_back;
     std::vector<LithoGroup> m_groups;
     std::vector<VertexFreedom> m_freedom;
+    std::vector<double> m_moveVector;  // 3 per vertex, unit length or zero
+    ControlPointInterpolation m_cpInterp = ControlPointInterpolation::Displacement;
     double m_axisScale[3] = {1.0, 1.0, 1.0};
     int m_cpStride = 0;
     std::vector<uint8_t> m_isCP;

@@ -17,7 +17,8 @@ from . import lithofwd
 _HERE = os.path.dirname(os.path.abspath(__file__))
 HOST_PATH = os.path.join(_HERE, "liblithohost.so")
 
-FIXED, Z_ONLY, X_ONLY, Y_ONLY, XY_FREE, XZ_FREE, YZ_FREE, XYZ_FREE = 0, 1, 2, 3, 5, 6, 7, 8
+FIXED, Z_ONLY, X_ONLY, Y_ONLY, ALONG_VECTOR, XY_FREE, XZ_FREE, YZ_FREE, XYZ_FREE = (
+    0, 1, 2, 3, 4, 5, 6, 7, 8)
 
 _P = ctypes.c_void_p
 _D = ctypes.POINTER(ctypes.c_double)
@@ -82,6 +83,8 @@ def load_host():
     L.li_durbin_watson.argtypes = [i64, _P, _P, _P]
     L.li_model_properties.argtypes = [_P, _P, _P]
     L.li_model_set_remanence.argtypes = [_P, _P, _P, _P, _P]
+    L.li_model_set_move_vectors.argtypes = [_P, _P]
+    L.li_model_set_cp_interpolation.argtypes = [_P, ctypes.c_int]
     L.li_model_remanence.argtypes = [_P, _P, _P, _P, _P]
     L.li_problem_unit_response_direction.argtypes = [_P, ctypes.c_int, _P, _P]
     L.li_problem_unit_remanence_response.argtypes = [_P, ctypes.c_int, _P]
@@ -268,6 +271,19 @@ class Model:
         d, c = np.zeros(n), np.zeros(n)
         _chk(load_host().li_model_properties(self._h, _p(d), _p(c)), "properties")
         return d, c
+
+    def set_move_vectors(self, mv):
+        """Move vectors (n_v x 3) of ALONG_VECTOR vertices, used normalized."""
+        a = _a(mv).reshape(self.n_v, 3)
+        _chk(load_host().li_model_set_move_vectors(self._h, _p(a)), "set_move_vectors")
+
+    def set_cp_interpolation(self, mode="displacement"):
+        """How non-control vertices follow the control points: "displacement" (their
+        control points' weighted displacement along their own free directions) or
+        "position" (the reference's interpolateFromControlPoints: the weighted average of
+        the control points' positions, for every non-control vertex)."""
+        _chk(load_host().li_model_set_cp_interpolation(
+            self._h, {"displacement": 0, "position": 1}[mode]), "set_cp_interpolation")
 
     def set_remanence(self, magnitude=None, inclination=None, declination=None, vector=None):
         """Per-unit remanence (LithoGroup, litho_group.h:18-24): magnitude (reference

@@ -340,3 +340,100 @@ def test_global_dofs_reference_cases():
                   inversion.Z_ONLY).astype(np.uint8)
     M2.set_freedom(fr)
     assert M2.n_dof == 16
+
+
+@pytest.mark.parametrize("stride", [2, 3])
+def test_position_form_control_points_against_reference(stride):
+    """The reference's control-point interpolation (interpolateFromControlPoints,
+    surface_mesh.cpp:610-623, with the copies of a shared vertex averaged as in
+    lithology_model.cpp:230-247): every vertex that is not a control point becomes the
+    weighted average of its control points' positions -- FIXED exterior vertices included,
+    and already for the unchanged parameter vector -- with the weights of the reference's
+    rule restated above; the parameter gradient is the exact transpose of that map."""
+    m = small_model()
+    M = inversion.Model(m)
+    M.fix_exterior(inversion.Z_ONLY)
+    M.set_axis_scale(10.0)
+    M.set_cp_stride(stride, 25.0)
+    M.set_cp_interpolation("position")
+    cp, W = reference_cp_weights(m, stride, 25.0)
+    p0 = M.params()
+    rng = np.random.default_rng(40 + stride)
+    for p in (p0, p0 + rng.normal(size=len(p0))):
+        X = M.apply(p)
+        np.testing.assert_allclose(X[~cp], (W @ X)[~cp], rtol=0, atol=1e-9)
+    # the unchanged parameters already move vertices (voxel corners collapse onto averages)
+    assert np.abs(M.apply(p0) - m.vertices).max() > 1.0
+    # FIXED exterior vertices that are not control points follow too
+    ext = np.zeros(m.n_vertices, bool)
+    ext[m.faces[m.back < 0].ravel()] = True
+    moved = np.abs(M.apply(p0) - m.vertices).max(1) > 1e-9
+    assert (moved & ext & ~cp).any()
+    # the map is affine in p: the pullback is its transpose
+    dp = rng.normal(size=len(p0))
+    dX = M.apply(p0 + dp) - M.apply(p0)
+    vg = rng.normal(size=(m.n_vertices, 3))
+    assert np.sum(vg * dX) == pytest.approx(M.pullback(vg) @ dp, rel=1e-12)
+    # the displacement form is still there and leaves the start geometry unchanged
+    M.set_cp_interpolation("displacement")
+    np.testing.assert_allclose(M.apply(p0), m.vertices, atol=1e-12)
+
+
+def test_along_vector_freedom():
+    """ALONG_VECTOR (surface_mesh.cpp:125-128, 188-237; the reference's own check is
+    test_vertex_freedom.cpp:50-52: such a vertex contributes one parameter): the parameter
+    is the vertex's projection on its normalized move vector (no axis scale), applying a
+    value moves the vertex along that vector only, the parameter's bounds are +-1e6, mixed
+    freedoms add up, and the gradient is the transpose of the map, also through the
+    smoothness term (the part of the offset from the neighbours along the vector,
+    smoothness.cpp:72-77)."""
+    m = small_model()
+    M = inversion.Model(m)
+    M.fix_exterior(inversion.Z_ONLY)
+    M.set_axis_scale(10.0)
+    fr = M.freedoms()
+    n_z = int((fr == inversion.Z_ONLY).sum())
+    assert M.n_dof == n_z
+    rng = np.random.default_rng(8)
+    interior = np.nonzero(fr == inversion.Z_ONLY)[0]
+    along = rng.choice(interior, 40, replace=False)
+    fr2 = fr.copy()
+    fr2[along] = inversion.ALONG_VECTOR
+    mv = np.zeros((m.n_vertices, 3))
+    mv[along] = rng.normal(size=(40, 3)) * 3.0  # not unit: used normalized
+    M.set_move_vectors(mv)
+    M.set_freedom(fr2)
+    assert M.n_dof == n_z  # one parameter each, as Z_ONLY had
+    fr3 = fr2.copy()
+    fr3[interior[:5]] = inversion.XYZ_FREE
+    M.set_freedom(fr3)
+    assert M.n_dof == n_z + 2 * 5  # each of the five goes from one parameter to three
+    M.set_freedom(fr2)
+    v, a, _ = M.dofs()
+    sel = np.isin(v, along)
+    assert sel.sum() == 40 and (a[sel] == 0).all()  # recorded on axis 0 like the reference
+    u = mv / np.maximum(np.linalg.norm(mv, axis=1), 1e-300)[:, None]
+    p0 = M.params()
+    np.testing.assert_allclose(p0[sel], np.sum(m.vertices[v[sel]] * u[v[sel]], 1), rtol=1e-14)
+    np.testing.assert_allclose(p0[~sel], m.vertices[v[~sel], 2] / 10.0, rtol=1e-14)
+    np.testing.assert_allclose(M.apply(p0), m.vertices, atol=1e-9)
+    dp = np.where(sel, 7.5, 0.0)
+    X = M.apply(p0 + dp)
+    d = X - m.vertices
+    np.testing.assert_allclose(d[v[sel]], 7.5 * u[v[sel]], atol=1e-9)  # along the vector
+    assert np.abs(np.delete(d, v[sel], 0)).max() == 0.0
+    lo, hi = M.bounds(p0, 200.0)
+    assert (lo[sel] == -1e6).all() and (hi[sel] == 1e6).all()
+    # gradient: transpose of the map, plain and through the smoothness term
+    vg = rng.normal(size=(m.n_vertices, 3))
+    dp = rng.normal(size=len(p0))
+    dX = M.apply(p0 + dp) - M.apply(p0)
+    assert np.sum(vg * dX) == pytest.approx(M.pullback(vg) @ dp, rel=1e-12)
+    p = p0 + rng.normal(0, 1.0, len(p0))
+    val, g = M.smoothness(p, 1.0)
+    h = 1e-4
+    for j in list(np.nonzero(sel)[0][:6]) + list(np.nonzero(~sel)[0][:4]):
+        e = np.zeros(len(p))
+        e[j] = h
+        fd = (M.smoothness(p + e, 1.0)[0] - M.smoothness(p - e, 1.0)[0]) / (2 * h)
+        assert fd == pytest.approx(g[j], rel=1e-6, abs=1e-9 * np.abs(g).max())
