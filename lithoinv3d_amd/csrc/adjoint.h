@@ -9,20 +9,22 @@
 //
 // Structure.  A facet's contribution at station o is F(p; theta): p = R (o - a) is the
 // station offset in the facet frame R = (u, v, n) and theta are the facet constants
-// (nine in-plane shape numbers, 2A, and six coefficients).  Per pair the kernel runs the
-// frame-form forward and its reverse, and accumulates over the stations of a chunk:
+// (seven in-plane shape numbers: l0, cu, cv and the length and angle of edges 1 and 2;
+// 2A; five coefficients K1, W0..W3).  Per pair the kernel runs the frame-form forward and
+// its reverse, and accumulates over the stations of a chunk:
 //   Gp    = sum w dF/dp                                  (3)
 //   A_jk  = sum w (dF/dp_j p_k - dF/dp_k p_j), j<k     (3)  -- the frame's rotation
-//   tbar  = sum w dF/dtheta                              (16)
-// 20 numbers per facet, lane = facet, stations wave-uniform (scalar loads).  A finalize
+//   tbar  = sum w dF/dtheta                              (13)
+// 19 numbers per facet, lane = facet, stations wave-uniform (scalar loads).  A finalize
 // kernel then pushes them through R(a,b,c) and theta(a,b,c) once per facet with
 // forward-mode duals (9 directions):
 //   dPhi/dx_k = sum_{j<k'} (dR_j/dx_k . R_k') A_jk' + tbar . dtheta/dx_k - [x_k in a] (R^T Gp)_k
 // and a gather kernel sums facet corners into vertices through a vertex -> corner CSR
 // list built once from the topology (deterministic, no atomics).
 //
-// The edge log uses the cancellation-free three-case form of the forward; the FP32
-// adjoint has no FP64 edge branch.
+// Edge logs: 2 atanh(ell / (d1 + d2)) by its series for edges more than about five edge
+// lengths from the station, the cancellation-free three-case form of the forward for the
+// nearer ones; the FP32 adjoint has no FP64 edge branch.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -131,6 +133,104 @@ __device__ __forceinline__ double atanh2_dz(double z, double& g)
     return 2.0 * atanh(z);
 }
 
+// Forward quantities of a pair that the reverse sweep reads.
+template <class T>
+struct PairFwd {
+    T px, py, pz, t, dcx, dcy, rda, rdb, rdc, s1b, s1c, dp1, dp2;
+};
+
+// The part of the reverse sweep that every seed source shares: from the seeds of the
+// edge-frame numbers (s1, dperp of edges 1 and 2), of the solid angle's products
+// rab, rac, rbc and of the three distances down to p and the in-plane shape numbers,
+// adding the pair's p, torque and shape seeds into acc.  The sweep is linear in its
+// seeds, so it runs once on the seeds of the common path (FAR: no near-edge seeds, whose
+// zero terms are left out rather than computed) and, in the rare branch with an edge near
+// the station, once more on that branch's own seeds (no solid-angle seeds).
+template <class T, bool FAR>
+__device__ __forceinline__ void reverse_tail(const FaceFrame<T>& F, const PairFwd<T>& p,
+                                             T spx, T spy, T spz, T st_, T sda, T sdb,
+                                             T sdc, T srho0, T spz2, T ss1b, T ss1c,
+                                             T sdp1, T sdp2, T srab, T srac, T srbc, T sL0,
+                                             T sL1, T sL2, T* acc)
+{
+    T sdcx, sdcy, Sda2, scu, scv, sl0c;
+    if constexpr (FAR) {
+        // s1b = t e1u - py e1v, dp1 = t e1v + py e1u; s1c = dcx e2u + dcy e2v,
+        // dp2 = dcx e2v - dcy e2u (only dp1, dp2 carry seeds here)
+        st_ = sdp1 * F.e1v;
+        spy += sdp1 * F.e1u;
+        sdcx = sdp2 * F.e2v;
+        sdcy = -sdp2 * F.e2u;
+        acc[S_PHI1] = fma(sdp1, p.s1b, acc[S_PHI1]);
+        acc[S_PHI2] = fma(sdp2, p.s1c, acc[S_PHI2]);
+        // rbc = t dcx - py dcy + pz2
+        st_ += srbc * p.dcx;
+        sdcx += srbc * p.t;
+        spy -= srbc * p.dcy;
+        sdcy -= srbc * p.py;
+        spz2 = srbc;
+        // rac = da2 - cu px - cv py ; rab = da2 - l0 px
+        scu = -srac * p.px;
+        scv = -srac * p.py;
+        spx = -(srac * F.cu + srab * F.l0);
+        spy -= srac * F.cv;
+        sl0c = -srab * p.px;
+        Sda2 = T(2) * (srac + srab) + sda * p.rda;
+    } else {
+        st_ += ss1b * F.e1u + sdp1 * F.e1v;
+        spy += sdp1 * F.e1u - ss1b * F.e1v;
+        sdcx = ss1c * F.e2u + sdp2 * F.e2v;
+        sdcy = ss1c * F.e2v - sdp2 * F.e2u;
+        // edge angles: (s_e1u, s_e1v) = ss1b (t, -py) + sdp1 (py, t) projected on
+        // (-e1v, e1u) is sdp1 s1b - ss1b dp1; likewise sdp2 s1c - ss1c dp2 for edge 2
+        acc[S_PHI1] = fma(sdp1, p.s1b, acc[S_PHI1]);
+        acc[S_PHI1] = fma(-ss1b, p.dp1, acc[S_PHI1]);
+        acc[S_PHI2] = fma(sdp2, p.s1c, acc[S_PHI2]);
+        acc[S_PHI2] = fma(-ss1c, p.dp2, acc[S_PHI2]);
+        Sda2 = sda * p.rda;
+        scu = T(0);
+        scv = T(0);
+        sl0c = T(0);
+    }
+    // square roots and squares, with the seeds of the squared quantities doubled
+    // (2 dF/d(d^2) = dF/dd / d, and d(x^2)/dx = 2x takes the factor back): hb = 2 s_db2,
+    // hc = 2 s_dc2, Sda2 = 2 s_da2, Srho0 = 2 s_rho0, Spz2 = 2 s_pz2
+    const T hb = sdb * p.rdb;
+    const T hc = sdc * p.rdc;
+    // dc2 = dcx^2 + dcy^2 + pz2
+    sdcx += p.dcx * hc;
+    sdcy += p.dcy * hc;
+    // dcx = cu - px, dcy = cv - py
+    scu += sdcx;
+    spx -= sdcx;
+    scv += sdcy;
+    spy -= sdcy;
+    // db2 = t^2 + rho0
+    st_ += p.t * hb;
+    // t = l0 - px
+    sl0c += st_;
+    spx -= st_;
+    // da2 = px^2 + rho0 ; rho0 = py^2 + pz2 ; pz2 = pz^2
+    spx += p.px * Sda2;
+    T Srho0 = hb + Sda2;
+    if constexpr (!FAR) Srho0 += T(2) * srho0;
+    spy += p.py * Srho0;
+    const T Spz2 = T(2) * spz2 + hc + Srho0;
+    spz += p.pz * Spz2;
+    // ---- accumulate
+    acc[S_GPX] += spx;
+    acc[S_GPY] += spy;
+    acc[S_GPZ] += spz;
+    acc[S_AXY] = fma(-spy, p.px, fma(spx, p.py, acc[S_AXY]));
+    acc[S_AXZ] = fma(-spz, p.px, fma(spx, p.pz, acc[S_AXZ]));
+    acc[S_AYZ] = fma(-spz, p.py, fma(spy, p.pz, acc[S_AYZ]));
+    acc[S_L0] += sl0c + sL0;
+    acc[S_CU] += scu;
+    acc[S_CV] += scv;
+    acc[S_L1] += sL1;
+    acc[S_L2] += sL2;
+}
+
 // One (station, facet) pair: frame-form forward and its reverse sweep for the weights
 // (lam on gz, mu on TMI), adding the pair's 19 facet seeds into acc (see the layout in
 // SeedIdx).  Linear in (lam, mu).  Edge logs are L = 2 atanh(z), z = ell / (d1 + d2),
@@ -143,9 +243,11 @@ __device__ __forceinline__ void pair_seeds(const FaceFrame<T>& F, double ox, dou
                                            double oz, T lam, T mu, T* acc)
 {
     const T Px = T(ox - F.ax), Py = T(oy - F.ay), Pz = T(oz - F.az);
-    const T px = Px * F.ux + Py * F.uy + Pz * F.uz + F.du;
-    const T py = Px * F.vx + Py * F.vy + Pz * F.vz + F.dv;
-    const T pz = Px * F.nx + Py * F.ny + Pz * F.nz + F.dn;
+    PairFwd<T> p;
+    p.px = Px * F.ux + Py * F.uy + Pz * F.uz + F.du;
+    p.py = Px * F.vx + Py * F.vy + Pz * F.vz + F.dv;
+    p.pz = Px * F.nx + Py * F.ny + Pz * F.nz + F.dn;
+    const T px = p.px, py = p.py, pz = p.pz;
     // ---- forward
     const T pz2 = pz * pz;
     const T rho0 = py * py + pz2;
@@ -154,8 +256,14 @@ __device__ __forceinline__ void pair_seeds(const FaceFrame<T>& F, double ox, dou
     const T db2 = t * t + rho0;
     const T dcx = F.cu - px, dcy = F.cv - py;
     const T dc2 = dcx * dcx + dcy * dcy + pz2;
+    p.t = t;
+    p.dcx = dcx;
+    p.dcy = dcy;
     // one reciprocal square root per distance gives both d and the 1/d of the reverse
     const T rda = fast_rsqrt(da2), rdb = fast_rsqrt(db2), rdc = fast_rsqrt(dc2);
+    p.rda = rda;
+    p.rdb = rdb;
+    p.rdc = rdc;
     const T da = da2 * rda, db = db2 * rdb, dc = dc2 * rdc;
     const T rab = da2 - F.l0 * px;
     const T rac = da2 - (F.cu * px + F.cv * py);
@@ -168,6 +276,10 @@ __device__ __forceinline__ void pair_seeds(const FaceFrame<T>& F, double ox, dou
     const T dp2 = dcx * F.e2v - dcy * F.e2u;
     const T s1b = t * F.e1u - py * F.e1v;
     const T s1c = dcx * F.e2u + dcy * F.e2v;
+    p.dp1 = dp1;
+    p.dp2 = dp2;
+    p.s1b = s1b;
+    p.s1c = s1c;
     // edges: z = ell q, q = 1 / (d1 + d2)
     const T q0 = fast_rcp(da + db), q1 = fast_rcp(db + dc), q2 = fast_rcp(dc + da);
     const T z0 = F.l0 * q0, z1 = F.l1 * q1, z2 = F.l2 * q2;
@@ -194,33 +306,40 @@ __device__ __forceinline__ void pair_seeds(const FaceFrame<T>& F, double ox, dou
     // ---- reverse
     const T ses = lam * F.K1;
     const T sth = T(2) * ses * pz + mu * F.W0;
-    T spz = T(2) * ses * th;
     const T sl0 = ses * py + mu * F.W1;
     const T sl1 = ses * dp1 + mu * F.W2;
     const T sl2 = ses * dp2 + mu * F.W3;
-    T spy = ses * lam0;
-    T sdp1 = ses * lam1;
-    T sdp2 = ses * lam2;
     // far edges: L(ell, d1, d2); w = sL dL/dz q, dz/dell = q, dz/dd = -z q
     const T w0 = n0 ? T(0) : sl0 * g0 * q0;
     const T w1 = n1 ? T(0) : sl1 * g1 * q1;
     const T w2 = n2 ? T(0) : sl2 * g2 * q2;
     const T wz0 = w0 * z0, wz1 = w1 * z1, wz2 = w2 * z2;
-    T sda = -(wz0 + wz2);
-    T sdb = -(wz0 + wz1);
-    T sdc = -(wz1 + wz2);
-    T sL0 = w0, sL1 = w1, sL2 = w2;
-    T spx = T(0), st_ = T(0), srho0 = T(0), spz2 = T(0);
-    T ss1b = T(0), ss1c = T(0);
+    // th = atan2(N, D)
+    const T q = fast_rcp(N * N + D * D);
+    const T sN = sth * D * q;
+    const T sD = -sth * N * q;
+    acc[S_TWOA] -= sN * pz;
+    // D = da db dc + rab dc + rac db + rbc da
+    const T sda = sD * (db * dc + rbc) - (wz0 + wz2);
+    const T sdb = sD * (da * dc + rac) - (wz0 + wz1);
+    const T sdc = sD * (da * db + rab) - (wz1 + wz2);
+    const T spz = T(2) * ses * th - sN * twoA;
+    reverse_tail<T, true>(F, p, T(0), ses * lam0, spz, T(0), sda, sdb, sdc, T(0), T(0), T(0),
+                          T(0), ses * lam1, ses * lam2, sD * dc, sD * db, sD * da, w0, w1, w2,
+                          acc);
     if (any_near) {
+        // the near edges' seeds, through the same tail
+        T spx = T(0), st_ = T(0), sdan = T(0), sdbn = T(0), sdcn = T(0), srho0 = T(0);
+        T spz2 = T(0), ss1b = T(0), ss1c = T(0), sdp1 = T(0), sdp2 = T(0);
+        T sL1 = T(0), sL2 = T(0);
         T gs1, gs2, gd1, gd2, gr;
         if (n0) {
             // s1 = -px, s2 = t, d1 = da, d2 = db
             edge_ln_grad(-px, F.l0, da, db, rho0, gs1, gs2, gd1, gd2, gr);
             spx -= sl0 * gs1;
             st_ += sl0 * gs2;
-            sda += sl0 * gd1;
-            sdb += sl0 * gd2;
+            sdan += sl0 * gd1;
+            sdbn += sl0 * gd2;
             srho0 += sl0 * gr;
         }
         if (n1) {
@@ -228,8 +347,8 @@ __device__ __forceinline__ void pair_seeds(const FaceFrame<T>& F, double ox, dou
             edge_ln_grad(s1b, F.l1, db, dc, dp1 * dp1 + pz2, gs1, gs2, gd1, gd2, gr);
             ss1b += sl1 * (gs1 + gs2);
             sL1 += sl1 * gs2;
-            sdb += sl1 * gd1;
-            sdc += sl1 * gd2;
+            sdbn += sl1 * gd1;
+            sdcn += sl1 * gd2;
             sdp1 += T(2) * dp1 * (sl1 * gr);
             spz2 += sl1 * gr;
         }
@@ -237,88 +356,14 @@ __device__ __forceinline__ void pair_seeds(const FaceFrame<T>& F, double ox, dou
             edge_ln_grad(s1c, F.l2, dc, da, dp2 * dp2 + pz2, gs1, gs2, gd1, gd2, gr);
             ss1c += sl2 * (gs1 + gs2);
             sL2 += sl2 * gs2;
-            sdc += sl2 * gd1;
-            sda += sl2 * gd2;
+            sdcn += sl2 * gd1;
+            sdan += sl2 * gd2;
             sdp2 += T(2) * dp2 * (sl2 * gr);
             spz2 += sl2 * gr;
         }
+        reverse_tail<T, false>(F, p, spx, T(0), T(0), st_, sdan, sdbn, sdcn, srho0, spz2,
+                               ss1b, ss1c, sdp1, sdp2, T(0), T(0), T(0), T(0), sL1, sL2, acc);
     }
-    // s1b = t e1u - py e1v, dp1 = t e1v + py e1u
-    st_ += ss1b * F.e1u + sdp1 * F.e1v;
-    spy += sdp1 * F.e1u - ss1b * F.e1v;
-    // s1c = dcx e2u + dcy e2v, dp2 = dcx e2v - dcy e2u
-    T sdcx = ss1c * F.e2u + sdp2 * F.e2v;
-    T sdcy = ss1c * F.e2v - sdp2 * F.e2u;
-    // edge angles: (s_e1u, s_e1v) = ss1b (t, -py) + sdp1 (py, t) projected on (-e1v, e1u)
-    // is sdp1 s1b - ss1b dp1; likewise sdp2 s1c - ss1c dp2 for edge 2
-    acc[S_PHI1] = fma(sdp1, s1b, acc[S_PHI1]);
-    acc[S_PHI1] = fma(-ss1b, dp1, acc[S_PHI1]);
-    acc[S_PHI2] = fma(sdp2, s1c, acc[S_PHI2]);
-    acc[S_PHI2] = fma(-ss1c, dp2, acc[S_PHI2]);
-    // th = atan2(N, D)
-    const T q = fast_rcp(N * N + D * D);
-    const T sN = sth * D * q;
-    const T sD = -sth * N * q;
-    const T stwoA = -sN * pz;
-    spz -= sN * twoA;
-    // D = da db dc + rab dc + rac db + rbc da
-    sda += sD * (db * dc + rbc);
-    sdb += sD * (da * dc + rac);
-    sdc += sD * (da * db + rab);
-    const T srab = sD * dc;
-    const T srac = sD * db;
-    const T srbc = sD * da;
-    // rbc = t dcx - py dcy + pz2
-    st_ += srbc * dcx;
-    sdcx += srbc * t;
-    spy -= srbc * dcy;
-    sdcy -= srbc * py;
-    spz2 += srbc;
-    // rac = da2 - cu px - cv py ; rab = da2 - l0 px
-    T sda2 = srac + srab;
-    T scu = -srac * px;
-    T scv = -srac * py;
-    spx -= srac * F.cu + srab * F.l0;
-    spy -= srac * F.cv;
-    T sl0c = -srab * px;
-    // square roots
-    sda2 += T(0.5) * sda * rda;
-    const T sdb2 = T(0.5) * sdb * rdb;
-    const T sdc2 = T(0.5) * sdc * rdc;
-    // dc2 = dcx^2 + dcy^2 + pz2
-    sdcx += T(2) * dcx * sdc2;
-    sdcy += T(2) * dcy * sdc2;
-    spz2 += sdc2;
-    // dcx = cu - px, dcy = cv - py
-    scu += sdcx;
-    spx -= sdcx;
-    scv += sdcy;
-    spy -= sdcy;
-    // db2 = t^2 + rho0
-    st_ += T(2) * t * sdb2;
-    srho0 += sdb2;
-    // t = l0 - px
-    sl0c += st_;
-    spx -= st_;
-    // da2 = px^2 + rho0 ; rho0 = py^2 + pz2 ; pz2 = pz^2
-    spx += T(2) * px * sda2;
-    srho0 += sda2;
-    spy += T(2) * py * srho0;
-    spz2 += srho0;
-    spz += T(2) * pz * spz2;
-    // ---- accumulate
-    acc[S_GPX] += spx;
-    acc[S_GPY] += spy;
-    acc[S_GPZ] += spz;
-    acc[S_AXY] = fma(-spy, px, fma(spx, py, acc[S_AXY]));
-    acc[S_AXZ] = fma(-spz, px, fma(spx, pz, acc[S_AXZ]));
-    acc[S_AYZ] = fma(-spz, py, fma(spy, pz, acc[S_AYZ]));
-    acc[S_L0] += sl0c + sL0;
-    acc[S_CU] += scu;
-    acc[S_CV] += scv;
-    acc[S_L1] += sL1;
-    acc[S_L2] += sL2;
-    acc[S_TWOA] += stwoA;
 }
 
 // Seed sums.  The per-pair seeds of one facet over the stations of a chunk have random
@@ -356,9 +401,14 @@ __device__ __forceinline__ void adjoint_pairs_body(const FaceFrame<T>* __restric
     // 32-bit counts keep the loop tests on the scalar unit (it has no 64-bit less-than)
     const StationW* __restrict__ sp = st + (i0 < i1 ? i0 : 0);
     const int cnt = i0 < i1 ? (int)(i1 - i0) : 0;
-    // each thread reads and writes only its own column: no barrier needed
+    // each thread reads and writes only its own column: no barrier needed.  A compiler
+    // memory barrier after each flush keeps the column in LDS: promoted to registers
+    // (legal, since no other thread reads it) the 19 doubles would take 38 VGPRs from the
+    // pair loop.
+    double(*col)[kAdjointBlock] = sum64;
 #pragma unroll
-    for (int k = 0; k < kSeeds; ++k) sum64[k][threadIdx.x] = 0.0;
+    for (int k = 0; k < kSeeds; ++k) col[k][threadIdx.x] = 0.0;
+    __asm__ volatile("" ::: "memory");
     for (int b0 = 0; b0 < cnt; b0 += kFlushStations) {
         const int b1 = min(cnt, b0 + kFlushStations);
         T acc[kSeeds];
@@ -375,12 +425,13 @@ __device__ __forceinline__ void adjoint_pairs_body(const FaceFrame<T>* __restric
             pair_seeds(F, S.x, S.y, S.z, T(S.wg), T(S.wt), acc);
         }
 #pragma unroll
-        for (int k = 0; k < kSeeds; ++k) sum64[k][threadIdx.x] += (double)acc[k];
+        for (int k = 0; k < kSeeds; ++k) col[k][threadIdx.x] += (double)acc[k];
+        __asm__ volatile("" ::: "memory");
     }
     if (active) {
         double* __restrict__ out = seeds + (int64_t)chunk * kSeeds * n_f + f;
 #pragma unroll
-        for (int k = 0; k < kSeeds; ++k) out[(int64_t)k * n_f] = sum64[k][threadIdx.x];
+        for (int k = 0; k < kSeeds; ++k) out[(int64_t)k * n_f] = col[k][threadIdx.x];
     }
 }
 

@@ -170,8 +170,9 @@ __device__ __forceinline__ void forward_pair(const FaceF32& F, const double* __r
     const float D = fmaf(da * db, dc, fmaf(rab, dc, fmaf(rac, db, rbc * da)));
     const float N = -(F.l0 * F.cv) * pz;
     const float th = atan2_t(N, D);  // Omega = 2*th
-    // edge logs 2 atanh(ell / (d1 + d2)): the series for every edge more than about 2.5
-    // edge lengths away; the few nearer ones from (s1, rho^2) of the edge
+    // edge logs 2 atanh(ell / (d1 + d2)): the series for every edge more than about five
+    // edge lengths away (d1 + d2 >= 10 ell); the few nearer ones from (s1, rho^2) of the
+    // edge
     const float z0 = F.l0 * rcp_nr(da + db);
     const float z1 = F.l1 * rcp_nr(db + dc);
     const float z2 = F.l2 * rcp_nr(dc + da);

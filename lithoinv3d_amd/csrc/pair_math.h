@@ -17,7 +17,8 @@
 //    (u along edge a->b, v = n x u, n): p = (px, py, pz).  The three vertex distances,
 //    the solid-angle numerator (2A*h, h = -pz) and denominator, and every edge's
 //    (s1, s2, dperp, rho^2) follow from p and nine in-plane shape numbers, and the
-//    edge log is evaluated without cancellation (edge_ln).
+//    edge log is evaluated without cancellation (edge_ratio, atanh2_series,
+//    ln_ratio_near).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -137,18 +138,17 @@ __device__ __forceinline__ float log1p_t(float y)
 }
 __device__ __forceinline__ double log1p_t(double y) { return log1p(y); }
 
-// ln((s2+d2)/(s1+d1)) for an edge of length ell (s2 = s1 + ell), X = |s| + d, without
-// cancellation even when the edge is far away (the ratio is then 1 + O(ell/d)).  With
+// The edge log Q = (s2+d2)/(s1+d1) for an edge of length ell (s2 = s1 + ell), X = |s| + d,
+// is 1 + O(ell/d) for a distant edge, so ln of a rounded ratio would lose eps*d/ell.  With
 // dd = d2 - d1 = ell (2 s1 + ell) / (d1 + d2) (exact, since d2^2 - d1^2 = 2 s1 ell + ell^2):
 //   s1 >= 0      : Q - 1 = (ell + dd)/X1 = ell (X1 + X2) / (X1 (d1 + d2))
 //   s2 <  0      : Q = X1/X2 (identity (s+d)(d-s) = rho^2 at both ends),
 //                  Q - 1 = (ell - dd)/X2 = ell (X1 + X2) / (X2 (d1 + d2))
 //   s1 < 0 <= s2 : Q = X1 X2 / rho^2 (station projects inside the edge)
 // so every case is ln(1 + y) with y >= 0 formed from positive products.
-
-// Numerator and denominator of Q - 1 = num / den for the forward's edge logarithm, with
-// the middle case in a form without cancellation: from (|s| + d)(d - |s|) = rho^2 and
-// d2 - d1 = ell (s1 + s2) / (d1 + d2),
+//
+// Numerator and denominator of Q - 1 = num / den, with the middle case in a form without
+// cancellation: from (|s| + d)(d - |s|) = rho^2 and d2 - d1 = ell (s1 + s2) / (d1 + d2),
 //   X1 X2 - rho^2 = ell (X1 X2 + rho^2) / (d1 + d2),
 // so every case is num = ell * A, den = (d1 + d2) * B with A, B sums or products of
 // positive numbers:

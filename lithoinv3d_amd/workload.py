@@ -28,9 +28,10 @@ OPS = {
     "forward_f32c": dict(fma=62, addmul=50, special=7),
     # adjoint, gz + TMI seeds: the forward part (plain rsq distances and rcp edge ratios),
     # the series of 2 atanh(z) and its derivative, coefficient seeds, reverse sweep through
-    # the edge ratios, solid angle and distances, accumulation of the 19 facet seeds;
+    # the edge ratios, solid angle and distances (the near-edge seeds left to their own
+    # branch, doubled seeds for the squared distances), accumulation of the 19 facet seeds;
     # 3 rsq + 5 rcp (three edge ratios, atan2 and its derivative)
-    "adjoint_pairs_f32": dict(fma=110, addmul=102, special=8),
+    "adjoint_pairs_f32": dict(fma=103, addmul=91, special=8),
 }
 
 
