@@ -380,75 +380,8 @@ def large_inversion(name="contact256x256x96_st512", stride=8, max_iter=3, ref_we
     return out
 
 
-def main(argv=None):
-    ap = argparse.ArgumentParser()
-    sub = ap.add_subparsers(dest="cmd", required=True)
-    a = sub.add_parser("learned-h0")
-    a.add_argument("--train", type=int, default=200)
-    a.add_argument("--test", type=int, default=20)
-    a.add_argument("--shape", type=int, nargs=3, default=(64, 64, 32))
-    a.add_argument("--n-st", type=int, default=128)
-    a.add_argument("--steps", type=int, default=3000)
-    a.add_argument("--max-iter", type=int, default=40)
-    a.add_argument("--h0-power", type=float, default=H0_POWER)
-    a.add_argument("--extra-powers", type=float, nargs="*", default=())
-    a.add_argument("--out", default=None)
-    b = sub.add_parser("invert")
-    b.add_argument("--seed", type=int, default=1)
-    b.add_argument("--shape", type=int, nargs=3, default=(64, 64, 32))
-    b.add_argument("--n-st", type=int, default=128)
-    b.add_argument("--max-iter", type=int, default=40)
-    b.add_argument("--out", default=None)
-    c = sub.add_parser("large")
-    c.add_argument("--workload", default="contact256x256x96_st512")
-    c.add_argument("--stride", type=int, default=8)
-    c.add_argument("--max-iter", type=int, default=3)
-    c.add_argument("--boreholes", type=int, default=50)
-    c.add_argument("--h0", choices=("learned", "scaled"), default="learned")
-    c.add_argument("--train-models", type=int, default=40)
-    c.add_argument("--out", default=None)
-    args = ap.parse_args(argv)
-    if args.cmd == "large":
-        res = large_inversion(args.workload, args.stride, args.max_iter,
-                              n_boreholes=args.boreholes, h0=args.h0,
-                              train_models=args.train_models)
-        if args.out:
-            json.dump(res, open(args.out, "w"), indent=1)
-        print(json.dumps(res))
-        return
-    shape = tuple(args.shape)
-    if args.cmd == "invert":
-        prob = contact_problem(args.seed, shape, args.n_st)
-        res = compare_solvers(prob, max_iter=args.max_iter)
-        log(json.dumps(res))
-        if args.out:
-            json.dump(res, open(args.out, "w"), indent=1)
-        print(json.dumps(res))
-        return
-    t0 = time.time()
-    X, t = training_set(range(1000, 1000 + args.train), shape, args.n_st)
-    t_data = time.time() - t0
-    t1 = time.time()
-    net, norm, losses = train_mlp(X, t, steps=args.steps)
-    t_train = time.time() - t1
-    log(f"trained on {len(t)} parameters from {args.train} models: loss {losses[0]:.4f} -> "
-        f"{losses[-1]:.4f} ({t_train:.1f}s; data {t_data:.1f}s)")
-    results = []
-    for k, s in enumerate(range(2000, 2000 + args.test)):
-        prob = contact_problem(s, shape, args.n_st)
-        # prediction quality on the held-out model (a constant offset does not change
-        # the solver's path, gamma absorbs it: it is removed before the RMS)
-        _, th, base = features_targets(prob)
-        pred, _ = predict_log_diag(prob, net, norm)
-        r = compare_solvers(prob, net, norm, max_iter=args.max_iter, exact_diag=True,
-                            h0_power=args.h0_power, extra_powers=args.extra_powers)
-        e, eb = pred - th, base - th
-        r["log_diag_rmse"] = float(np.std(e))
-        r["log_diag_rmse_subset_only"] = float(np.std(eb))
-        r["log_diag_std"] = float(np.std(th))
-        results.append(r)
-        log(f"held-out {k + 1}/{args.test}: {json.dumps(r)}")
-
+def summarize_h0(results, meta, extra_powers=()):
+    """Medians over the held-out models of compare_solvers' results."""
     def med(key, field):
         vals = [r[key][field] for r in results if key in r]
         return float(np.median(vals)) if vals else math.nan
@@ -458,9 +391,7 @@ def main(argv=None):
         return float(np.median(vals)) if vals else math.nan
 
     summary = {
-        "train_models": args.train, "test_models": args.test, "train_samples": int(len(t)),
-        "train_loss_first_last": [float(losses[0]), float(losses[-1])],
-        "train_seconds": t_train, "data_seconds": t_data,
+        **meta,
         "median_fp64_seconds": med("fp64", "seconds"),
         "median_fp64_iterations": med("fp64", "iterations"),
         "median_fp32c_seconds_to_target": med_to("fp32c", 1),
@@ -482,13 +413,128 @@ def main(argv=None):
             for r in results if r["learned_fp32c"]["to_target"] and r["fp32c"]["to_target"]]))
         if results else math.nan,
         **{f"median_learned_fp32c_power{pw:g}_iterations_to_target":
-           med_to(f"learned_fp32c_power{pw:g}", 0) for pw in args.extra_powers},
+           med_to(f"learned_fp32c_power{pw:g}", 0) for pw in extra_powers},
         "median_subset_only_fp32c_iterations_to_target": med_to("subset_only_fp32c", 0),
         "median_log_diag_rmse": float(np.median([r["log_diag_rmse"] for r in results])),
         "median_log_diag_rmse_subset_only": float(np.median(
             [r["log_diag_rmse_subset_only"] for r in results])),
         "median_log_diag_std": float(np.median([r["log_diag_std"] for r in results])),
     }
+    return summary
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    a = sub.add_parser("learned-h0")
+    a.add_argument("--train", type=int, default=200)
+    a.add_argument("--test", type=int, default=20)
+    a.add_argument("--shape", type=int, nargs=3, default=(64, 64, 32))
+    a.add_argument("--n-st", type=int, default=128)
+    a.add_argument("--steps", type=int, default=3000)
+    a.add_argument("--max-iter", type=int, default=300,
+                   help="iterations of the plain FP64 run whose final misfit is the target "
+                        "(300: within about 0.5 %% of its value after 1000)")
+    a.add_argument("--h0-power", type=float, default=H0_POWER)
+    a.add_argument("--extra-powers", type=float, nargs="*", default=())
+    a.add_argument("--out", default=None)
+    a.add_argument("--test-first", type=int, default=2000, help="first held-out seed")
+    a.add_argument("--save-net", default=None, help="write the trained network (.npz)")
+    a.add_argument("--load-net", default=None, help="use this network instead of training")
+    a.add_argument("--jsonl", default=None, help="append each held-out model's result")
+    b = sub.add_parser("invert")
+    b.add_argument("--seed", type=int, default=1)
+    b.add_argument("--shape", type=int, nargs=3, default=(64, 64, 32))
+    b.add_argument("--n-st", type=int, default=128)
+    b.add_argument("--max-iter", type=int, default=40)
+    b.add_argument("--out", default=None)
+    d = sub.add_parser("summarize-h0", help="medians over the results of learned-h0 runs")
+    d.add_argument("--jsonl", required=True)
+    d.add_argument("--net", required=True, help="the network file of the runs (.npz)")
+    d.add_argument("--out", default=None)
+    c = sub.add_parser("large")
+    c.add_argument("--workload", default="contact256x256x96_st512")
+    c.add_argument("--stride", type=int, default=8)
+    c.add_argument("--max-iter", type=int, default=3)
+    c.add_argument("--boreholes", type=int, default=50)
+    c.add_argument("--h0", choices=("learned", "scaled"), default="learned")
+    c.add_argument("--train-models", type=int, default=40)
+    c.add_argument("--out", default=None)
+    args = ap.parse_args(argv)
+    if args.cmd == "large":
+        res = large_inversion(args.workload, args.stride, args.max_iter,
+                              n_boreholes=args.boreholes, h0=args.h0,
+                              train_models=args.train_models)
+        if args.out:
+            json.dump(res, open(args.out, "w"), indent=1)
+        print(json.dumps(res))
+        return
+    if args.cmd == "summarize-h0":
+        results = [json.loads(line) for line in open(args.jsonl) if line.strip()]
+        z = np.load(args.net)
+        summary = summarize_h0(results, dict(
+            train_samples=int(z["n_samples"]), test_models=len(results),
+            train_loss_first_last=[float(z["losses"][0]), float(z["losses"][-1])],
+            train_seconds=float(z["train_seconds"]), data_seconds=float(z["data_seconds"])))
+        out = {"summary": summary, "models": results}
+        if args.out:
+            json.dump(out, open(args.out, "w"), indent=1)
+        print(json.dumps(summary, indent=1))
+        return
+    shape = tuple(args.shape)
+    if args.cmd == "invert":
+        prob = contact_problem(args.seed, shape, args.n_st)
+        res = compare_solvers(prob, max_iter=args.max_iter)
+        log(json.dumps(res))
+        if args.out:
+            json.dump(res, open(args.out, "w"), indent=1)
+        print(json.dumps(res))
+        return
+    if args.load_net:
+        z = np.load(args.load_net)
+        net = mlp.MLP(seed=0)
+        net.params = z["params"]
+        norm = mlp.Normalizer(z["mean"], z["std"], float(z["t_mean"]), float(z["t_std"]))
+        losses = z["losses"]
+        t = np.zeros(int(z["n_samples"]))
+        t_data, t_train = float(z["data_seconds"]), float(z["train_seconds"])
+        log(f"network from {args.load_net}")
+    else:
+        t0 = time.time()
+        X, t = training_set(range(1000, 1000 + args.train), shape, args.n_st)
+        t_data = time.time() - t0
+        t1 = time.time()
+        net, norm, losses = train_mlp(X, t, steps=args.steps)
+        t_train = time.time() - t1
+        log(f"trained on {len(t)} parameters from {args.train} models: loss {losses[0]:.4f} "
+            f"-> {losses[-1]:.4f} ({t_train:.1f}s; data {t_data:.1f}s)")
+        if args.save_net:
+            np.savez(args.save_net, params=net.params, mean=norm.mean, std=norm.std,
+                     t_mean=norm.t_mean, t_std=norm.t_std, losses=np.asarray(losses),
+                     n_samples=len(t), data_seconds=t_data, train_seconds=t_train)
+    results = []
+    for k, s in enumerate(range(args.test_first, args.test_first + args.test)):
+        prob = contact_problem(s, shape, args.n_st)
+        # prediction quality on the held-out model (a constant offset does not change
+        # the solver's path, gamma absorbs it: it is removed before the RMS)
+        _, th, base = features_targets(prob)
+        pred, _ = predict_log_diag(prob, net, norm)
+        r = compare_solvers(prob, net, norm, max_iter=args.max_iter, exact_diag=True,
+                            h0_power=args.h0_power, extra_powers=args.extra_powers)
+        e, eb = pred - th, base - th
+        r["log_diag_rmse"] = float(np.std(e))
+        r["log_diag_rmse_subset_only"] = float(np.std(eb))
+        r["log_diag_std"] = float(np.std(th))
+        results.append(r)
+        log(f"held-out {k + 1}/{args.test}: {json.dumps(r)}")
+        if args.jsonl:
+            with open(args.jsonl, "a") as fh:
+                fh.write(json.dumps(r) + "\n")
+
+    summary = summarize_h0(results, dict(
+        train_models=args.train, test_models=len(results), train_samples=int(len(t)),
+        train_loss_first_last=[float(losses[0]), float(losses[-1])],
+        train_seconds=t_train, data_seconds=t_data), args.extra_powers)
     out = {"summary": summary, "models": results}
     if args.out:
         json.dump(out, open(args.out, "w"), indent=1)
