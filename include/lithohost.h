@@ -212,6 +212,11 @@ int li_durbin_watson(int64_t n, const double* xyz, const double* residuals, doub
 
 /* ---- L-BFGS-B (lbfgsb_optimizer.cpp:13-263, optimizer.h:7-36) ---- */
 
+/* Iteration callback of a problem (InversionRunner::setIterationCallback, runner.h:200-201):
+ * called on the calling thread with each history row as it is recorded -- from
+ * li_problem_minimize the 7 values of its hist rows, from li_problem_run the 10 values of
+ * its hist rows; kind is 0 for a minimize row and 1 for a runner row. */
+typedef void (*li_iter_cb)(int32_t kind, const double* row, int64_t n_row, void* user);
 typedef double (*li_obj_cb)(const double* x, int64_t n, void* user);
 typedef void (*li_grad_cb)(const double* x, int64_t n, double* g, void* user);
 typedef struct {
@@ -229,6 +234,8 @@ int li_minimize_callbacks(int64_t n, const double* x0, const double* lb, const d
                           li_obj_cb f, li_grad_cb g, void* user, const LiOptions* o,
                           const double* h0, double* x_out, double* hist, int64_t hist_cap,
                           int64_t* n_hist, int32_t* iterations, char* msg, int64_t msg_cap);
+/* sets (cb non-NULL) or clears the problem's iteration callback */
+int li_problem_set_iteration_callback(void* h, li_iter_cb cb, void* user);
 int li_problem_minimize(void* h, const double* x0, const double* lb, const double* ub,
                         const LiOptions* o, const double* h0, double* x_out, double* hist,
                         int64_t hist_cap, int64_t* n_hist, int32_t* iterations, char* msg,

@@ -452,6 +452,17 @@ struct lf_ctx {
     unsigned long long* d_fb = nullptr;
     // adjoint work
     AdjointWork adj;
+    // column-norm work (lf_colnorm2): the model-sized buffers are allocated on the first
+    // call, the per-parameter ones grow to the largest parameter map seen
+    struct ColnormWork {
+        void* dT = nullptr;  // per-facet derivative matrix restricted to one axis
+        StationW* dst = nullptr;
+        float *Gg = nullptr, *Gt = nullptr, *Dg = nullptr, *Dt = nullptr;
+        int64_t* doff = nullptr;
+        int32_t* dvert = nullptr;
+        double *dw = nullptr, *og = nullptr, *ot = nullptr;
+        int64_t cap_dof = -1, cap_ent = -1;
+    } cn;
     // timing
     bool timing = false;
     struct Ev { int cls; hipEvent_t a, b; };
@@ -627,6 +638,10 @@ void lf_destroy(lf_ctx* c)
                     c->d_f64,   c->d_fr64, c->d_geo64, c->d_part_gz, c->d_part_tmi, c->d_out_gz,
                     c->d_out_tmi, c->d_fb};
     for (void* p : bufs)
+        if (p) hipFree(p);
+    void* cnb[] = {c->cn.dT, c->cn.dst, c->cn.Gg,  c->cn.Gt, c->cn.Dg, c->cn.Dt,
+                   c->cn.doff, c->cn.dvert, c->cn.dw, c->cn.og, c->cn.ot};
+    for (void* p : cnb)
         if (p) hipFree(p);
     adjoint_free(c->adj);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -876,34 +891,58 @@ int lf_colnorm2(lf_ctx* c, int axis, int64_t n_dof, const int64_t* dof_off,
     if (rc) return rc;
     const int kBlk = 64;
     const size_t tsz = c->precision == LF_FP32C ? sizeof(float) : sizeof(double);
-    void* dT = nullptr;
-    StationW* dst = nullptr;
-    float *Gg = nullptr, *Gt = nullptr, *Dg = nullptr, *Dt = nullptr;
-    int64_t* doff = nullptr;
-    int32_t* dvert = nullptr;
-    double *dw = nullptr, *og = nullptr, *ot = nullptr;
-    auto cleanup = [&] {
-        void* b[] = {dT, dst, Gg, Gt, Dg, Dt, doff, dvert, dw, og, ot};
-        for (void* p : b)
-            if (p) hipFree(p);
-    };
-    auto fail = [&](hipError_t e, const char* what) {
-        cleanup();
-        return set_err(c, std::string("lf_colnorm2 ") + what + ": " + hipGetErrorString(e), LF_EHIP);
-    };
+    auto& W = c->cn;
     hipError_t e;
+    auto fail = [&](hipError_t err, const char* what) {
+        return set_err(c, std::string("lf_colnorm2 ") + what + ": " + hipGetErrorString(err), LF_EHIP);
+    };
 #define CN(expr, what) do { e = (expr); if (e != hipSuccess) return fail(e, what); } while (0)
-    CN(hipMalloc(&dT, tsz * kSeeds * 3 * std::max<int64_t>(c->n_f, 1)), "alloc");
-    CN(hipMalloc((void**)&dst, sizeof(StationW) * kBlk), "alloc");
-    CN(hipMalloc((void**)&Gg, sizeof(float) * kBlk * 3 * std::max<int64_t>(c->n_f, 1)), "alloc");
-    CN(hipMalloc((void**)&Gt, sizeof(float) * kBlk * 3 * std::max<int64_t>(c->n_f, 1)), "alloc");
-    CN(hipMalloc((void**)&Dg, sizeof(float) * kBlk * std::max<int64_t>(c->n_v, 1)), "alloc");
-    CN(hipMalloc((void**)&Dt, sizeof(float) * kBlk * std::max<int64_t>(c->n_v, 1)), "alloc");
-    CN(hipMalloc((void**)&doff, sizeof(int64_t) * (n_dof + 1)), "alloc");
-    CN(hipMalloc((void**)&dvert, sizeof(int32_t) * std::max<int64_t>(n_ent, 1)), "alloc");
-    CN(hipMalloc((void**)&dw, sizeof(double) * std::max<int64_t>(n_ent, 1)), "alloc");
-    CN(hipMalloc((void**)&og, sizeof(double) * std::max<int64_t>(n_dof, 1)), "alloc");
-    CN(hipMalloc((void**)&ot, sizeof(double) * std::max<int64_t>(n_dof, 1)), "alloc");
+    if (!(W.dT && W.dst && W.Gg && W.Gt && W.Dg && W.Dt)) {
+        // first call, or a first call that failed part-way
+        for (void* q : {W.dT, (void*)W.dst, (void*)W.Gg, (void*)W.Gt, (void*)W.Dg, (void*)W.Dt})
+            if (q) hipFree(q);
+        W.dT = nullptr;
+        W.dst = nullptr;
+        W.Gg = W.Gt = W.Dg = W.Dt = nullptr;
+        const int64_t nf = std::max<int64_t>(c->n_f, 1), nv = std::max<int64_t>(c->n_v, 1);
+        CN(hipMalloc(&W.dT, tsz * kSeeds * 3 * nf), "alloc");
+        CN(hipMalloc((void**)&W.dst, sizeof(StationW) * kBlk), "alloc");
+        CN(hipMalloc((void**)&W.Gg, sizeof(float) * kBlk * 3 * nf), "alloc");
+        CN(hipMalloc((void**)&W.Gt, sizeof(float) * kBlk * 3 * nf), "alloc");
+        CN(hipMalloc((void**)&W.Dg, sizeof(float) * kBlk * nv), "alloc");
+        CN(hipMalloc((void**)&W.Dt, sizeof(float) * kBlk * nv), "alloc");
+    }
+    if (n_dof > W.cap_dof) {
+        // grow; the previous call ended with a stream synchronization, so nothing reads
+        // the old buffers
+        CN(hipStreamSynchronize(c->stream), "sync");
+        for (void* q : {(void*)W.doff, (void*)W.og, (void*)W.ot})
+            if (q) hipFree(q);
+        W.doff = nullptr;
+        W.og = W.ot = nullptr;
+        W.cap_dof = -1;
+        CN(hipMalloc((void**)&W.doff, sizeof(int64_t) * (n_dof + 1)), "alloc");
+        CN(hipMalloc((void**)&W.og, sizeof(double) * std::max<int64_t>(n_dof, 1)), "alloc");
+        CN(hipMalloc((void**)&W.ot, sizeof(double) * std::max<int64_t>(n_dof, 1)), "alloc");
+        W.cap_dof = n_dof;
+    }
+    if (n_ent > W.cap_ent) {
+        CN(hipStreamSynchronize(c->stream), "sync");
+        if (W.dvert) hipFree(W.dvert);
+        if (W.dw) hipFree(W.dw);
+        W.dvert = nullptr;
+        W.dw = nullptr;
+        W.cap_ent = -1;
+        CN(hipMalloc((void**)&W.dvert, sizeof(int32_t) * std::max<int64_t>(n_ent, 1)), "alloc");
+        CN(hipMalloc((void**)&W.dw, sizeof(double) * std::max<int64_t>(n_ent, 1)), "alloc");
+        W.cap_ent = n_ent;
+    }
+    void* dT = W.dT;
+    StationW* dst = W.dst;
+    float *Gg = W.Gg, *Gt = W.Gt, *Dg = W.Dg, *Dt = W.Dt;
+    int64_t* doff = W.doff;
+    int32_t* dvert = W.dvert;
+    double *dw = W.dw, *og = W.og, *ot = W.ot;
     if (n_dof > 0) {
         CN(hipMemcpyAsync(doff, dof_off, sizeof(int64_t) * (n_dof + 1), hipMemcpyHostToDevice, c->stream), "copy");
         if (n_ent) {
@@ -957,7 +996,6 @@ int lf_colnorm2(lf_ctx* c, int axis, int64_t n_dof, const int64_t* dof_off,
     }
     CN(hipStreamSynchronize(c->stream), "sync");
 #undef CN
-    cleanup();
     return LF_OK;
 }
 

@@ -283,7 +283,7 @@ def learned_log_diag_xyz(m, M, stations, net, norm, field_dir, field_nT):
 
 
 def large_inversion(name="contact256x256x96_st512", stride=8, max_iter=3, ref_weight=1e-2,
-                    seed=5, n_boreholes=50, h0="learned", train_models=40):
+                    seed=5, n_boreholes=50, h0="learned", train_models=40, target=None):
     """Joint gz + TMI inversion at the large workload, everything resident on one GPU:
     XYZ_FREE control points (every ``stride``-th cell), FP32C forward and adjoint, the
     reference-model term, n_boreholes seeded vertical boreholes with depth-to-contact
@@ -292,7 +292,8 @@ def large_inversion(name="contact256x256x96_st512", stride=8, max_iter=3, ref_we
     training time is reported); "scaled": the identity.  The first step is scaled so the
     largest parameter moves one unit (an unscaled step would cost a long series of
     halvings, each a full forward); the diagonal's shape is kept.  Reports setup,
-    per-iteration times, misfit and device memory in use."""
+    per-iteration times, misfit and device memory in use.  target: stop once the
+    objective is at or below it (seconds to another run's final misfit)."""
     import torch
 
     out = {"workload": name, "h0": h0}
@@ -368,7 +369,13 @@ def large_inversion(name="contact256x256x96_st512", stride=8, max_iter=3, ref_we
     h0 = D / max(float(np.abs(D * g0).max()), 1e-300)
     log(f"large: first gradient {out['first_gradient_s']:.1f}s; "
         f"{out['device_memory_in_use_gb']:.1f} GB in use")
-    r = P.minimize(p0, lo, hi, max_iter=max_iter, h0=h0)
+    P.set_iteration_callback(lambda kind, row: log(
+        f"large: iteration {int(row[0])}: f {row[1]:.5e}, |pg| {row[2]:.3e}, {row[4]:.1f}s, "
+        f"{int(row[5])} objective / {int(row[6])} gradient calls"))
+    t4 = time.perf_counter()
+    r = P.minimize(p0, lo, hi, max_iter=max_iter, h0=h0, target=target)
+    out["solve_s"] = time.perf_counter() - t4
+    out["target"] = target
     comp = P.evaluate(r.x)
     P.close()
     h = r.history
@@ -459,12 +466,14 @@ def main(argv=None):
     c.add_argument("--boreholes", type=int, default=50)
     c.add_argument("--h0", choices=("learned", "scaled"), default="learned")
     c.add_argument("--train-models", type=int, default=40)
+    c.add_argument("--target", type=float, default=None,
+                   help="stop at this objective value (another run's final misfit)")
     c.add_argument("--out", default=None)
     args = ap.parse_args(argv)
     if args.cmd == "large":
         res = large_inversion(args.workload, args.stride, args.max_iter,
                               n_boreholes=args.boreholes, h0=args.h0,
-                              train_models=args.train_models)
+                              train_models=args.train_models, target=args.target)
         if args.out:
             json.dump(res, open(args.out, "w"), indent=1)
         print(json.dumps(res))

@@ -46,6 +46,8 @@ struct Problem {
     // (runner.h:98-107)
     RemanentMagnetizationMode remMode = RemanentMagnetizationMode::EffectiveSusceptibility;
     double remLower = 0.0, remUpper = 10.0, compLower = -10.0, compUpper = 10.0;
+    li_iter_cb iterCb = nullptr;  // li_problem_set_iteration_callback
+    void* iterUser = nullptr;
 
     void applyRemanenceBounds(PropertyInversionOptions& o) const
     {
@@ -476,6 +478,15 @@ void copy_text(const std::string& t, char* out, int64_t cap)
 // DW magnetic x/y.  summary[8]: converged, geometry iterations, final objective, final
 // gravity RMS, alpha_mag after calibration (0 if none), validation retries, rejections,
 // history entries in total.
+// a runner history entry as the 10 values of li_problem_run's hist rows
+static void runner_row(const RunnerIteration& it, double* row)
+{
+    const double v[10] = {(double)it.iteration, it.gravityMisfit, it.magneticMisfit,
+                          it.regularization, it.total, it.rms, it.dwGravityX,
+                          it.dwGravityY, it.dwMagneticX, it.dwMagneticY};
+    std::memcpy(row, v, sizeof(v));
+}
+
 int li_problem_run(void* h, const LiRunnerOptions* o, void* label_grid, double* params_out,
                    double* hist, int64_t hist_cap, int64_t* n_hist, double* summary, char* log,
                    int64_t log_cap)
@@ -537,17 +548,18 @@ int li_problem_run(void* h, const LiRunnerOptions* o, void* label_grid, double* 
             if (P->fused) P->fused->setAlphaMag(r.alpha);
             return r.alpha;
         };
+        if (P->iterCb) {
+            rp.onIteration = [P](const RunnerIteration& it) {
+                double row[10];
+                runner_row(it, row);
+                P->iterCb(1, row, 10, P->iterUser);
+            };
+        }
         const LabelGridData* grid = label_grid ? &((GridHandle*)label_grid)->g : nullptr;
         const RunnerResult r = runInversion(P->model, rp, ro, grid);
         std::memcpy(params_out, r.params.data(), sizeof(double) * r.params.size());
         const int64_t nh = std::min<int64_t>((int64_t)r.history.size(), hist_cap);
-        for (int64_t k = 0; k < nh; ++k) {
-            const auto& it = r.history[(size_t)k];
-            const double row[10] = {(double)it.iteration, it.gravityMisfit, it.magneticMisfit,
-                                    it.regularization, it.total, it.rms, it.dwGravityX,
-                                    it.dwGravityY, it.dwMagneticX, it.dwMagneticY};
-            std::memcpy(hist + 10 * k, row, sizeof(row));
-        }
+        for (int64_t k = 0; k < nh; ++k) runner_row(r.history[(size_t)k], hist + 10 * k);
         *n_hist = nh;
         const double sm[8] = {r.converged ? 1.0 : 0.0, (double)r.totalIterations, r.finalMisfit,
                               r.finalRMS, r.alphaMag, (double)r.validationRetries,
@@ -748,14 +760,33 @@ int li_problem_set_remanence_bounds(void* h, double magnitude_lower, double magn
 }
 
 // hist rows: iteration, f, |proj grad|, alpha, seconds, objective calls, gradient calls
+// an optimizer record as the 7 values of the minimize hist rows
+static void minimize_row(const LBFGSBOptimizer::IterationRecord& r, double* row)
+{
+    row[0] = r.iteration;
+    row[1] = r.f;
+    row[2] = r.pgNorm;
+    row[3] = r.alpha;
+    row[4] = r.seconds;
+    row[5] = r.objectiveCalls;
+    row[6] = r.gradientCalls;
+}
+
 static int run_minimize(LBFGSBOptimizer::Objective obj, LBFGSBOptimizer::Gradient grad,
                         int64_t n, const double* x0, const double* lb, const double* ub,
                         const LiOptions* o, const double* h0, double* x_out, double* hist,
                         int64_t hist_cap, int64_t* n_hist, int32_t* iterations, char* msg,
-                        int64_t msg_cap)
+                        int64_t msg_cap, li_iter_cb cb = nullptr, void* cb_user = nullptr)
 {
     return guard([&] {
         LBFGSBOptimizer opt;
+        if (cb) {
+            opt.setIterationCallback([cb, cb_user](const LBFGSBOptimizer::IterationRecord& r) {
+                double row[7];
+                minimize_row(r, row);
+                cb(0, row, 7, cb_user);
+            });
+        }
         opt.setMaxIterations(o->max_iter);
         opt.setHistorySize(o->history);
         opt.setTolerance(o->ftol);
@@ -767,16 +798,7 @@ static int run_minimize(LBFGSBOptimizer::Objective obj, LBFGSBOptimizer::Gradien
         std::memcpy(x_out, r.params.data(), sizeof(double) * n);
         const auto& H = opt.history();
         const int64_t nh = std::min<int64_t>((int64_t)H.size(), hist_cap);
-        for (int64_t k = 0; k < nh; ++k) {
-            double* row = hist + 7 * k;
-            row[0] = H[k].iteration;
-            row[1] = H[k].f;
-            row[2] = H[k].pgNorm;
-            row[3] = H[k].alpha;
-            row[4] = H[k].seconds;
-            row[5] = H[k].objectiveCalls;
-            row[6] = H[k].gradientCalls;
-        }
+        for (int64_t k = 0; k < nh; ++k) minimize_row(H[k], hist + 7 * k);
         *n_hist = nh;
         *iterations = r.iterations;
         if (msg && msg_cap > 0) {
@@ -811,7 +833,15 @@ int li_problem_minimize(void* h, const double* x0, const double* lb, const doubl
     auto obj = [&](const Vec& x) { return P->components(x).total; };
     auto grad = [&](const Vec& x) { return P->grad(x); };
     return run_minimize(obj, grad, n, x0, lb, ub, o, h0, x_out, hist, hist_cap, n_hist, iterations,
-                        msg, msg_cap);
+                        msg, msg_cap, P->iterCb, P->iterUser);
+}
+
+int li_problem_set_iteration_callback(void* h, li_iter_cb cb, void* user)
+{
+    auto* P = (Problem*)h;
+    P->iterCb = cb;
+    P->iterUser = cb ? user : nullptr;
+    return 0;
 }
 
 // ------------------------------------------------------------------ file formats

@@ -1724,6 +1724,7 @@ OptimizerResult LBFGSBOptimizer::minimize(Objective objective, Gradient gradient
     };
     m_hist.push_back({0, f, pgnorm(x, g), 0.0,
                       std::chrono::duration<double>(clk::now() - t0).count(), nObj, nGrad});
+    if (m_onIteration) m_onIteration(m_hist.back());
     for (int k = 0; k < m_maxIter; ++k) {
         m_iter = k + 1;
         if (m_hasTarget && f <= m_target) {
@@ -1815,6 +1816,7 @@ OptimizerResult LBFGSBOptimizer::minimize(Objective objective, Gradient gradient
         f = fn;
         m_hist.push_back({m_iter, f, pgnorm(x, g), alpha,
                           std::chrono::duration<double>(clk::now() - t0).count(), nObj, nGrad});
+        if (m_onIteration) m_onIteration(m_hist.back());
     }
     if (res.message.empty()) res.message = "Maximum iterations reached";
     res.params = x;

@@ -554,11 +554,15 @@ public:
     // two-loop recursion and the step (off by default: the reference projects the full
     // L-BFGS step, which can stop descending once bounds are active).
     void setActiveSetAware(bool on) { m_activeSet = on; }
+    // called with each iteration's record as it is appended (record 0: the start)
+    using IterationCallback = std::function<void(const IterationRecord&)>;
+    void setIterationCallback(IterationCallback cb) { m_onIteration = std::move(cb); }
     int iterationCount() const { return m_iter; }
     const std::vector<IterationRecord>& history() const { return m_hist; }
 
 private:
     Vec direction(const Vec& g, const std::vector<uint8_t>& freeMask) const;
+    IterationCallback m_onIteration;
     std::vector<Vec> m_s, m_y;
     std::vector<double> m_rho;
     Vec m_h0;

@@ -421,6 +421,7 @@ RunnerResult runInversion(std::shared_ptr<LithologyModel> model, RunnerProblem& 
     };
 
     res.history.push_back(entry(params, problem.components(params)));
+    if (problem.onIteration) problem.onIteration(res.history.back());
     if (hasMag && opt.eigenvalueScaling && problem.calibrate) {
         res.alphaMag = problem.calibrate(params);
         log << "calibrated alpha_mag = " << res.alphaMag << "\n";
@@ -434,6 +435,7 @@ RunnerResult runInversion(std::shared_ptr<LithologyModel> model, RunnerProblem& 
         if (res.history.empty() || d2 > 1e-14) {
             lastRecorded = p;
             res.history.push_back(entry(p, c));
+            if (problem.onIteration) problem.onIteration(res.history.back());
         }
         return c.total;
     };

@@ -130,6 +130,9 @@ struct RunnerProblem {
     std::function<double(const Vec&)> calibrate;   // returns the new alpha_mag
     const GravityData* gdata = nullptr;
     const MagneticData* mdata = nullptr;           // null: gravity only
+    // called with each history entry as it is appended (InversionRunner's iteration
+    // callback, runner.h:200-201, runner.cpp:297, 447, 472)
+    std::function<void(const RunnerIteration&)> onIteration;
 };
 
 // InversionRunner::run (runner.cpp:217-641): start parameters and bounds, history entry

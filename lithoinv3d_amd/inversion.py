@@ -44,6 +44,8 @@ class _RunnerOptions(ctypes.Structure):
 
 OBJ_CB = ctypes.CFUNCTYPE(ctypes.c_double, _D, ctypes.c_int64, ctypes.c_void_p)
 GRAD_CB = ctypes.CFUNCTYPE(None, _D, ctypes.c_int64, _D, ctypes.c_void_p)
+# li_iter_cb: (kind 0 = minimize row of 7, 1 = runner row of 10, row, n_row, user)
+ITER_CB = ctypes.CFUNCTYPE(None, ctypes.c_int32, _D, ctypes.c_int64, ctypes.c_void_p)
 
 
 def load_host():
@@ -89,6 +91,7 @@ def load_host():
     L.li_problem_unit_response_direction.argtypes = [_P, ctypes.c_int, _P, _P]
     L.li_problem_unit_remanence_response.argtypes = [_P, ctypes.c_int, _P]
     L.li_problem_set_remanence_mode.argtypes = [_P, ctypes.c_int]
+    L.li_problem_set_iteration_callback.argtypes = [_P, ITER_CB, _P]
     L.li_problem_set_remanence_bounds.argtypes = [_P, ctypes.c_double, ctypes.c_double,
                                                   ctypes.c_double, ctypes.c_double]
     L.li_model_fix_exterior_faces.argtypes = [_P, ctypes.c_int]
@@ -615,6 +618,23 @@ class Problem:
         _chk(load_host().li_problem_unit_remanence_response(self._h, group, _p(out)),
              "unit_remanence_response")
         return out
+
+    def set_iteration_callback(self, fn):
+        """fn(kind, row) with each history row as it is recorded (InversionRunner's
+        iteration callback): kind "minimize" rows are (iteration, f, |projected gradient|,
+        step, seconds, objective calls, gradient calls), kind "run" rows are
+        li_problem_run's history rows; None removes it."""
+        L = load_host()
+        if fn is None:
+            self._iter_cb = None
+            _chk(L.li_problem_set_iteration_callback(self._h, ITER_CB(), None), "callback")
+            return
+
+        def cb(kind, row, n_row, user):
+            fn("minimize" if kind == 0 else "run", np.ctypeslib.as_array(row, (n_row,)).copy())
+
+        self._iter_cb = ITER_CB(cb)  # kept alive with the problem
+        _chk(L.li_problem_set_iteration_callback(self._h, self._iter_cb, None), "callback")
 
     def minimize(self, x0, lb, ub, max_iter=50, history=10, ftol=1e-6, c1=1e-4, target=None,
                  h0=None, active_set=False):

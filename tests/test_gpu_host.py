@@ -162,9 +162,17 @@ def test_small_inversion_recovers(prec):
                           alpha_mag=1.0, precision=prec)
     lo, hi = M.bounds(p0, 200.0)
     f0 = P.evaluate(p0)[3]
+    rows = []
+    P.set_iteration_callback(lambda kind, row: rows.append((kind, row)))
     r = P.minimize(p0, lo, hi, max_iter=40)
     print(f"{prec}: f0 {f0:.3e} -> {r.f:.3e} in {r.iterations} it ({r.message})")
     assert r.f < 1e-2 * f0
+    # the iteration callback saw every history row as it was recorded
+    assert [k for k, _ in rows] == ["minimize"] * len(r.history)
+    np.testing.assert_array_equal(np.array([row for _, row in rows]), r.history)
+    P.set_iteration_callback(None)
+    P.minimize(p0, lo, hi, max_iter=2)
+    assert len(rows) == len(r.history)
     P.close()
 
 

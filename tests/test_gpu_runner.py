@@ -59,9 +59,15 @@ def test_runner_geometry_with_calibration_and_gate():
     P.close()
     say("runner: geometry, calibration, validation gate")
     P = inversion.Problem(M, st, gz, np.full(n, 0.05), tm, np.full(n, 1.0), alpha_mag=1.0)
+    rows = []
+    P.set_iteration_callback(lambda kind, row: rows.append((kind, row)))
     r = P.run(max_iterations=30, property_interval=10, validate=True, eigenvalue_scaling=True,
               bounds_margin=200.0)
     print(r["log"])
+    # the iteration callback (runner.h:200-201) saw every history entry as it was recorded
+    assert len(rows) == r["total_entries"] and all(k == "run" for k, _ in rows)
+    np.testing.assert_array_equal(np.array([row for _, row in rows])[:len(r["history"])],
+                                  r["history"])
     print({k: v for k, v in r.items() if k not in ("history", "params", "log")})
     h = r["history"]
     np.testing.assert_allclose(h[0, 1:5], [c0[0], c0[1], c0[2], c0[3]], rtol=1e-12)
