@@ -249,7 +249,7 @@ __device__ __forceinline__ void pair_seeds(const FaceFrame<T>& F, double ox, dou
     p.pz = Px * F.nx + Py * F.ny + Pz * F.nz + F.dn;
     const T px = p.px, py = p.py, pz = p.pz;
     // ---- forward
-    const T pz2 = pz * pz + T(kDist2Floor);
+    const T pz2 = pz * pz;
     const T rho0 = py * py + pz2;
     const T da2 = px * px + rho0;
     const T t = F.l0 - px;

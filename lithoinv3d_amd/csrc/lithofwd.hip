@@ -155,7 +155,7 @@ __device__ __forceinline__ void forward_pair(const FaceF32& F, const double* __r
     const float px = fmaf(Px, F.ux, fmaf(Py, F.uy, fmaf(Pz, F.uz, F.du)));
     const float py = fmaf(Px, F.vx, fmaf(Py, F.vy, fmaf(Pz, F.vz, F.dv)));
     const float pz = fmaf(Px, F.nx, fmaf(Py, F.ny, fmaf(Pz, F.nz, F.dn)));  // h = -pz
-    const float pz2 = fmaf(pz, pz, kDist2Floor);
+    const float pz2 = pz * pz;
     const float rho0 = fmaf(py, py, pz2);
     const float da2 = fmaf(px, px, rho0);
     const float t = F.l0 - px;  // (b - o).u

@@ -76,16 +76,15 @@ __device__ __forceinline__ double fast_rsqrt(double x) { return 1.0 / sqrt(x); }
 // rounding of the final fma, which is unbiased.
 __device__ __forceinline__ float sqrt_nr(float x)
 {
-    // x > 0 (the pair code's squared distances carry kDist2Floor)
-    const float rs = fast_rsqrt(x);
+    // x >= 0; x = 0 gives 0 (the clamp keeps rsq finite).  The clamp is not folded into
+    // the squared distance as an fma floor (pz * pz + 1e-30): a positive addend breaks
+    // every exact rounding tie of pz * pz upwards, and ties are common where station and
+    // vertex coordinates sit on grids (offsets in quarter metres: pz with 13 significant
+    // bits has a square exactly halfway between two floats), which biased the sums.
+    const float rs = fast_rsqrt(fmaxf(x, 1e-30f));
     const float d = x * rs;
     return fmaf(fmaf(-d, d, x), 0.5f * rs, d);
 }
-// Floor added to the squared distance from the facet plane, pz^2, which enters all three
-// squared vertex distances: they stay positive for a station exactly at a vertex, so the
-// reciprocal square roots stay finite (1e-30 m^2 changes no distance above 1e-15 m), at
-// the cost of no instruction (pz^2 + floor is one fma where pz^2 was one multiply).
-constexpr float kDist2Floor = 1e-30f;
 __device__ __forceinline__ float rcp_nr(float x)
 {
     const float r = fast_rcp(x);

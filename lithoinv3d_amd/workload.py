@@ -21,11 +21,11 @@ TRANSCENDENTAL_FLOP_EQ = 8
 # not counted); the three FP64 subtractions of the station offset count as add/mul
 OPS = {
     # forward, gz + TMI fused, FP32C: station offset from the record's origin and its frame
-    # projection, squared distances (with their floor), three Newton-refined square roots
-    # (rsq + 2 fmas), solid-angle N and D, atan2 polynomial, z = ell / (d1 + d2) with a
-    # Newton-refined reciprocal per edge, the 2 atanh(z) series per edge, gz and TMI
-    # combine with Kahan sums; 3 rsq + 4 rcp
-    "forward_f32c": dict(fma=63, addmul=49, special=7),
+    # projection, squared distances, three Newton-refined square roots (rsq + 2 fmas),
+    # solid-angle N and D, atan2 polynomial, z = ell / (d1 + d2) with a Newton-refined
+    # reciprocal per edge, the 2 atanh(z) series per edge, gz and TMI combine with Kahan
+    # sums; 3 rsq + 4 rcp
+    "forward_f32c": dict(fma=62, addmul=50, special=7),
     # adjoint, gz + TMI seeds: the forward part (plain rsq distances and rcp edge ratios),
     # the series of 2 atanh(z) and its derivative, coefficient seeds, reverse sweep through
     # the edge ratios, solid angle and distances (the near-edge seeds left to their own
