@@ -8,8 +8,9 @@
 //                     magnetic_forward.cpp:317-341 hoisted out of the station loop).
 //   k_forward_f32c    lane = station, wave-uniform facet: facet records stream through
 //                     LDS in stages of 64 and are read at a uniform address; each lane
-//                     keeps its own (Kahan) accumulators, so the facet reduction needs
-//                     no cross-lane traffic.  Grid = station tiles x facet chunks.
+//                     keeps its own accumulators (FP32 sums over groups of 16 facets,
+//                     Kahan-compensated across groups), so the facet reduction needs no
+//                     cross-lane traffic.  Grid = station tiles x facet chunks.
 //   k_forward_f64     the same map in FP64 with the reference's operation sequence;
 //                     records arrive through scalar loads.
 //   k_reduce_chunks   sums the per-chunk partials of each station in a fixed order.
@@ -139,13 +140,13 @@ __device__ __forceinline__ void block_coords(int n_tiles, int& tile, int& chunk)
 #define LF_FORWARD_OCC
 #endif
 
-// One (station, facet) pair of the FP32C forward, accumulated into the station's Kahan
-// sums.  F is the facet record (wave-uniform), f its index (for the FP64 branch).
+// One (station, facet) pair of the FP32C forward, added into the station's FP32 sums of
+// the current facet group (gs, ts).  F is the facet record (wave-uniform), f its index
+// (for the FP64 branch).
 template <bool GZ, bool TMI>
 __device__ __forceinline__ void forward_pair(const FaceF32& F, const double* __restrict__ geo64,
                                              int64_t f, double ox, double oy, double oz,
-                                             float& gs, float& gc, float& ts, float& tc,
-                                             unsigned& nfb)
+                                             float& gs, float& ts, unsigned& nfb)
 {
     // station offset from the record's origin: FP64 difference, one rounding; in the
     // frame, plus the origin's offset d, it is the offset from vertex a
@@ -169,20 +170,27 @@ __device__ __forceinline__ void forward_pair(const FaceF32& F, const double* __r
     const float rbc = fmaf(t, dcx, fmaf(-py, dcy, pz2));
     const float D = fmaf(da * db, dc, fmaf(rab, dc, fmaf(rac, db, rbc * da)));
     const float N = -(F.l0 * F.cv) * pz;
-    const float th = atan2_t(N, D);  // Omega = 2*th
-    // edge logs 2 atanh(ell / (d1 + d2)): the series for every edge more than about five
+    // edge logs 2 atanh(ell / (d1 + d2)) by the series for every edge more than about five
     // edge lengths away (d1 + d2 >= 10 ell); the few nearer ones from (s1, rho^2) of the
     // edge
     const float z0 = F.l0 * rcp_nr(da + db);
     const float z1 = F.l1 * rcp_nr(db + dc);
     const float z2 = F.l2 * rcp_nr(dc + da);
-    float lam0 = atanh2_series(z0);
-    float lam1 = atanh2_series(z1);
-    float lam2 = atanh2_series(z2);
     // in-plane distances of the station from the lines of edges 1 and 2 (gz weights)
     const float dp1 = fmaf(t, F.e1v, py * F.e1u);
     const float dp2 = fmaf(dcx, F.e2v, -dcy * F.e2u);
-    if (fmaxf(z0, fmaxf(z1, z2)) > kZSeries) {
+    float th, lam0, lam1, lam2;  // Omega = 2 th
+    if (fmaxf(z0, fmaxf(z1, z2)) <= kZSeries && fabsf(N) <= kSmallAngle * D) {
+        // the common case: a station far from the facet compared with its size
+        th = atan_small(N * fast_rcp(D));
+        lam0 = atanh2_series(z0);
+        lam1 = atanh2_series(z1);
+        lam2 = atanh2_series(z2);
+    } else {
+        th = atan2_t(N, D);
+        lam0 = atanh2_series(z0);
+        lam1 = atanh2_series(z1);
+        lam2 = atanh2_series(z2);
         const float s1b = fmaf(t, F.e1u, -py * F.e1v);
         const float s1c = fmaf(dcx, F.e2u, dcy * F.e2v);
         const float rho1 = fmaf(dp1, dp1, pz2);
@@ -220,11 +228,9 @@ __device__ __forceinline__ void forward_pair(const FaceF32& F, const double* __r
     }
     if (GZ) {
         const float es = fmaf(py, lam0, fmaf(dp1, lam1, dp2 * lam2));
-        kahan_add(gs, gc, F.K1 * fmaf(pz + pz, th, es));
+        gs = fmaf(F.K1, fmaf(pz + pz, th, es), gs);
     }
-    if (TMI) {
-        kahan_add(ts, tc, fmaf(F.W0, th, fmaf(F.W1, lam0, fmaf(F.W2, lam1, F.W3 * lam2))));
-    }
+    if (TMI) ts += fmaf(F.W0, th, fmaf(F.W1, lam0, fmaf(F.W2, lam1, F.W3 * lam2)));
 }
 
 // Lane = station; the facet records of the block's chunk stream through LDS: the block
@@ -236,6 +242,15 @@ __device__ __forceinline__ void forward_pair(const FaceF32& F, const double* __r
 // traffic.
 constexpr int kStage = 64;
 static_assert(kStage * sizeof(FaceF32) == kBlock * 32, "one 32-byte piece per thread");
+// Facet sums: each lane adds the pairs of kGroup consecutive facets in plain FP32 and
+// adds each group's sum into its Kahan-compensated FP32 total.  The group sum is small
+// against the total, so its own roundings are small against one rounding of the total,
+// and the compensation costs four instructions per group instead of per pair.
+#ifndef LF_FORWARD_GROUP
+#define LF_FORWARD_GROUP 16
+#endif
+constexpr int kGroup = LF_FORWARD_GROUP;
+static_assert(kStage % kGroup == 0, "groups tile a stage");
 // stations per lane: each record read from LDS serves this many pairs
 #ifndef LF_FORWARD_SPL
 #define LF_FORWARD_SPL 1
@@ -282,6 +297,7 @@ __global__ __launch_bounds__(256) LF_FORWARD_OCC void k_forward_f32c(
         dst[1] = q1;
     };
 
+    // per station: Kahan-compensated FP32 sums (s, c) of the facet-group sums
     float gs[kSPL], gc[kSPL], ts[kSPL], tc[kSPL];
 #pragma unroll
     for (int k = 0; k < kSPL; ++k) gs[k] = gc[k] = ts[k] = tc[k] = 0.f;
@@ -296,12 +312,22 @@ __global__ __launch_bounds__(256) LF_FORWARD_OCC void k_forward_f32c(
         const bool more = fs + kStage < f1;
         if (more) fetch(fs + kStage);
         const int nst = (int)min((int64_t)kStage, f1 - fs);
-        for (int j = 0; j < nst; ++j) {
-            const FaceF32 F = stage[b][j];
+        for (int j0 = 0; j0 < nst; j0 += kGroup) {
+            const int j1 = min(nst, j0 + kGroup);
+            float g[kSPL], m[kSPL];
 #pragma unroll
-            for (int k = 0; k < kSPL; ++k)
-                forward_pair<GZ, TMI>(F, geo64, fs + j, ox[k], oy[k], oz[k], gs[k], gc[k], ts[k],
-                                      tc[k], nfb);
+            for (int k = 0; k < kSPL; ++k) g[k] = m[k] = 0.f;
+            for (int j = j0; j < j1; ++j) {
+                const FaceF32 F = stage[b][j];
+#pragma unroll
+                for (int k = 0; k < kSPL; ++k)
+                    forward_pair<GZ, TMI>(F, geo64, fs + j, ox[k], oy[k], oz[k], g[k], m[k], nfb);
+            }
+#pragma unroll
+            for (int k = 0; k < kSPL; ++k) {
+                if (GZ) kahan_add(gs[k], gc[k], g[k]);
+                if (TMI) kahan_add(ts[k], tc[k], m[k]);
+            }
         }
         // the other buffer was last read before the previous barrier
         if (more) park(b ^ 1);

@@ -116,6 +116,32 @@ __device__ __forceinline__ float atan2_t(float y, float x)
 }
 __device__ __forceinline__ double atan2_t(double y, double x) { return atan2(y, x); }
 
+// atan(a) for |a| <= kSmallAngle: the Taylor series a - a^3/3 + ... + a^9/9 (the first
+// omitted term, a^11/11, is below 1e-10 of a at |a| = 1/8), written as a + (a s) q so that
+// the result carries one final rounding.  The solid angle of a facet seen from more than
+// a few edge lengths away has tan(Omega/2) = N/D far inside this range, where atan2_t's
+// octant reduction and its minimax polynomial (whose constant term 1 - 6.6e-7 biases
+// small angles) are not needed.
+constexpr float kSmallAngle = 0.125f;
+__device__ __forceinline__ float atan_small(float a)
+{
+    const float s = a * a;
+    float q = fmaf(s, 1.0f / 9.0f, -1.0f / 7.0f);
+    q = fmaf(s, q, 1.0f / 5.0f);
+    q = fmaf(s, q, -1.0f / 3.0f);
+    return fmaf(a * s, q, a);
+}
+// d atan(a) / da = 1 / (1 + s), s = a^2 <= 1/64, by its series to s^4 (truncation below
+// 1e-9 relative).
+__device__ __forceinline__ float datan_small(float s)
+{
+    float g = s - 1.0f;
+    g = fmaf(s, g, 1.0f);
+    g = fmaf(s, g, -1.0f);
+    g = fmaf(s, g, 1.0f);
+    return g;
+}
+
 // Kahan-compensated accumulation: s holds the sum, c the running compensation;
 // the compensated total is s - c.
 __device__ __forceinline__ void kahan_add(float& s, float& c, float x)
@@ -181,12 +207,13 @@ __device__ __forceinline__ void edge_ratio(float s1, float ell, float d1, float 
 constexpr float kZSeries = 0.1f;
 __device__ __forceinline__ float atanh2_series(float z)
 {
+    // z (2 + 2z^2/3 + ... + 2z^8/9): the factor 2 sits in the coefficients
     const float s = z * z;
-    float p = fmaf(s, 1.0f / 9.0f, 1.0f / 7.0f);
-    p = fmaf(s, p, 1.0f / 5.0f);
-    p = fmaf(s, p, 1.0f / 3.0f);
-    p = fmaf(s, p, 1.0f);
-    return (z + z) * p;
+    float p = fmaf(s, 2.0f / 9.0f, 2.0f / 7.0f);
+    p = fmaf(s, p, 2.0f / 5.0f);
+    p = fmaf(s, p, 2.0f / 3.0f);
+    p = fmaf(s, p, 2.0f);
+    return z * p;
 }
 __device__ __forceinline__ float ln_ratio_near(float num, float den)
 {

@@ -16,22 +16,28 @@ from . import synth
 
 TRANSCENDENTAL_FLOP_EQ = 8
 
-# per (station, facet) pair, counted over the common path of the compiled loops (the
-# branch for edges within about five edge lengths of the station, taken by few waves, is
-# not counted); the three FP64 subtractions of the station offset count as add/mul
+# per (station, facet) pair, counted over the common path of the compiled loops: a wave
+# whose stations are all more than about five edge lengths from its facet (forward) or
+# whose facets are all that far from its station (adjoint), with the solid angle's
+# tan(Omega/2) = N/D below 1/8 -- nearly every wave of the named workloads.  The branch
+# for nearer pairs (octant-reduced atan2, case-form edge logs, FP64 edges) is not counted;
+# the three FP64 subtractions of the station offset count as add/mul
 OPS = {
     # forward, gz + TMI fused, FP32C: station offset from the record's origin and its frame
-    # projection, squared distances, three Newton-refined square roots (rsq + 2 fmas),
-    # solid-angle N and D, atan2 polynomial, z = ell / (d1 + d2) with a Newton-refined
-    # reciprocal per edge, the 2 atanh(z) series per edge, gz and TMI combine with Kahan
-    # sums; 3 rsq + 4 rcp
-    "forward_f32c": dict(fma=62, addmul=50, special=7),
-    # adjoint, gz + TMI seeds: the forward part (plain rsq distances and rcp edge ratios),
-    # the series of 2 atanh(z) and its derivative, coefficient seeds, reverse sweep through
-    # the edge ratios, solid angle and distances (the near-edge seeds left to their own
-    # branch, doubled seeds for the squared distances), accumulation of the 19 facet seeds;
-    # 3 rsq + 5 rcp (three edge ratios, atan2 and its derivative)
-    "adjoint_pairs_f32": dict(fma=103, addmul=91, special=8),
+    # projection (nested fmas), squared distances, three Newton-refined square roots (rsq +
+    # 2 fmas), solid-angle N and D, small-angle atan series of N/D, z = ell / (d1 + d2) with
+    # a Newton-refined reciprocal per edge, the 2 atanh(z) series per edge, gz and TMI
+    # combined into the station's FP32 sums of the 16-facet group (the group sums' Kahan
+    # update, 8 operations per 16 pairs, is not counted); 3 rsq + 4 rcp
+    "forward_f32c": dict(fma=58, addmul=41, special=7),
+    # adjoint, gz + TMI seeds: the forward part (plain rsq distances and rcp edge ratios,
+    # nested-fma projection), the small-angle atan series and its derivative, the series of
+    # 2 atanh(z) and its derivative, coefficient seeds, reverse sweep through the edge
+    # ratios, solid angle and distances (the near-edge seeds left to their own branch,
+    # doubled seeds for the squared distances), accumulation of the 19 facet seeds (their
+    # FP64 flush every 16 stations is not counted); 3 rsq + 4 rcp (three edge ratios and
+    # the solid angle's N/D)
+    "adjoint_pairs_f32": dict(fma=102, addmul=82, special=7),
 }
 
 
