@@ -173,9 +173,15 @@ __device__ __forceinline__ void forward_pair(const FaceF32& F, const double* __r
     // edge logs 2 atanh(ell / (d1 + d2)) by the series for every edge more than about five
     // edge lengths away (d1 + d2 >= 10 ell); the few nearer ones from (s1, rho^2) of the
     // edge
-    const float z0 = F.l0 * rcp_nr(da + db);
-    const float z1 = F.l1 * rcp_nr(db + dc);
-    const float z2 = F.l2 * rcp_nr(dc + da);
+    // the three 1 / (d1 + d2) from one Newton-refined reciprocal of their product
+    // (S0 S1 S2 <= (2R)^3 stays far inside FP32 range for any survey geometry)
+    const float S0 = da + db, S1 = db + dc, S2 = dc + da;
+    const float S01 = S0 * S1;
+    const float r012 = rcp_nr(S01 * S2);
+    const float r01 = S2 * r012;  // 1 / (S0 S1)
+    const float z0 = F.l0 * (S1 * r01);
+    const float z1 = F.l1 * (S0 * r01);
+    const float z2 = F.l2 * (S01 * r012);
     // in-plane distances of the station from the lines of edges 1 and 2 (gz weights)
     const float dp1 = fmaf(t, F.e1v, py * F.e1u);
     const float dp2 = fmaf(dcx, F.e2v, -dcy * F.e2u);
@@ -251,9 +257,12 @@ static_assert(kStage * sizeof(FaceF32) == kBlock * 32, "one 32-byte piece per th
 #endif
 constexpr int kGroup = LF_FORWARD_GROUP;
 static_assert(kStage % kGroup == 0, "groups tile a stage");
-// stations per lane: each record read from LDS serves this many pairs
+// stations per lane: each record read from LDS serves this many pairs.  Two halve the
+// LDS-to-VGPR traffic of the records (8 uniform 16-byte reads per record per wave) at 142
+// VGPRs (3 waves per SIMD instead of 4): measured on the 192x192x64 contact model with
+// 65,536 stations, 378 ms with one station per lane, 347 ms with two, 366 ms with three.
 #ifndef LF_FORWARD_SPL
-#define LF_FORWARD_SPL 1
+#define LF_FORWARD_SPL 2
 #endif
 constexpr int kSPL = LF_FORWARD_SPL;
 

@@ -25,11 +25,13 @@ TRANSCENDENTAL_FLOP_EQ = 8
 OPS = {
     # forward, gz + TMI fused, FP32C: station offset from the record's origin and its frame
     # projection (nested fmas), squared distances, three Newton-refined square roots (rsq +
-    # 2 fmas), solid-angle N and D, small-angle atan series of N/D, z = ell / (d1 + d2) with
-    # a Newton-refined reciprocal per edge, the 2 atanh(z) series per edge, gz and TMI
-    # combined into the station's FP32 sums of the 16-facet group (the group sums' Kahan
-    # update, 8 operations per 16 pairs, is not counted); 3 rsq + 4 rcp
-    "forward_f32c": dict(fma=58, addmul=41, special=7),
+    # 2 fmas), solid-angle N and D, small-angle atan series of N/D, z = ell / (d1 + d2) for
+    # the three edges from one Newton-refined reciprocal of the product of the three sums,
+    # the 2 atanh(z) series per edge, gz and TMI combined into the station's FP32 sums of
+    # the 16-facet group (the group sums' Kahan update, 8 operations per 16 pairs, is not
+    # counted); 3 rsq + 2 rcp.  The compiled loop runs two stations per lane and computes
+    # the facet's 2A once for both (108 FMA + 93 add/mul + 10 transcendentals per two pairs)
+    "forward_f32c": dict(fma=54, addmul=47, special=5),
     # adjoint, gz + TMI seeds: the forward part (plain rsq distances and rcp edge ratios,
     # nested-fma projection), the small-angle atan series and its derivative, the series of
     # 2 atanh(z) and its derivative, coefficient seeds, reverse sweep through the edge
