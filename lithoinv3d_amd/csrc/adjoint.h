@@ -129,7 +129,10 @@ __device__ __forceinline__ float atanh2_dz(float z, float& g)
 }
 __device__ __forceinline__ double atanh2_dz(double z, double& g)
 {
-    g = 2.0 / (1.0 - z * z);
+    // the derivative of a near edge (z > kZSeries) is taken out again by the near branch,
+    // so it only has to be finite: z = 1 for a station on the edge itself
+    const double zc = fmin(z, 0.5);
+    g = 2.0 / (1.0 - zc * zc);
     return 2.0 * atanh(z);
 }
 
@@ -339,10 +342,12 @@ __device__ __forceinline__ void pair_seeds(const FaceFrame<T>& F, double ox, dou
     const T sl0 = ses * py + mu * F.W1;
     const T sl1 = ses * dp1 + mu * F.W2;
     const T sl2 = ses * dp2 + mu * F.W3;
-    // far edges: L(ell, d1, d2); w = sL dL/dz q, dz/dell = q, dz/dd = -z q
-    const T w0 = n0 ? T(0) : sl0 * g0 * q0;
-    const T w1 = n1 ? T(0) : sl1 * g1 * q1;
-    const T w2 = n2 ? T(0) : sl2 * g2 * q2;
+    // far edges: L(ell, d1, d2); w = sL dL/dz q, dz/dell = q, dz/dd = -z q.  Taken for
+    // every edge (finite for near ones too); the near branch removes the near edges' share
+    // again, which keeps selects off the common path
+    const T w0 = sl0 * g0 * q0;
+    const T w1 = sl1 * g1 * q1;
+    const T w2 = sl2 * g2 * q2;
     const T wz0 = w0 * z0, wz1 = w1 * z1, wz2 = w2 * z2;
     // th = atan2(N, D)
     const T sN = sth * dthN;
@@ -357,41 +362,43 @@ __device__ __forceinline__ void pair_seeds(const FaceFrame<T>& F, double ox, dou
                           T(0), ses * lam1, ses * lam2, sD * dc, sD * db, sD * da, w0, w1, w2,
                           acc);
     if (any_near) {
-        // the near edges' seeds, through the same tail
+        // the near edges' seeds, through the same tail, less the far form's seeds that the
+        // common path took for those edges
         T spx = T(0), st_ = T(0), sdan = T(0), sdbn = T(0), sdcn = T(0), srho0 = T(0);
         T spz2 = T(0), ss1b = T(0), ss1c = T(0), sdp1 = T(0), sdp2 = T(0);
-        T sL1 = T(0), sL2 = T(0);
+        T sL0 = T(0), sL1 = T(0), sL2 = T(0);
         T gs1, gs2, gd1, gd2, gr;
         if (n0) {
             // s1 = -px, s2 = t, d1 = da, d2 = db
             edge_ln_grad(-px, F.l0, da, db, rho0, gs1, gs2, gd1, gd2, gr);
             spx -= sl0 * gs1;
             st_ += sl0 * gs2;
-            sdan += sl0 * gd1;
-            sdbn += sl0 * gd2;
+            sdan += sl0 * gd1 + wz0;
+            sdbn += sl0 * gd2 + wz0;
             srho0 += sl0 * gr;
+            sL0 -= w0;
         }
         if (n1) {
             // s1 = s1b, s2 = s1b + l1, d1 = db, d2 = dc, rho^2 = dp1^2 + pz^2
             edge_ln_grad(s1b, F.l1, db, dc, dp1 * dp1 + pz2, gs1, gs2, gd1, gd2, gr);
             ss1b += sl1 * (gs1 + gs2);
-            sL1 += sl1 * gs2;
-            sdbn += sl1 * gd1;
-            sdcn += sl1 * gd2;
+            sL1 += sl1 * gs2 - w1;
+            sdbn += sl1 * gd1 + wz1;
+            sdcn += sl1 * gd2 + wz1;
             sdp1 += T(2) * dp1 * (sl1 * gr);
             spz2 += sl1 * gr;
         }
         if (n2) {
             edge_ln_grad(s1c, F.l2, dc, da, dp2 * dp2 + pz2, gs1, gs2, gd1, gd2, gr);
             ss1c += sl2 * (gs1 + gs2);
-            sL2 += sl2 * gs2;
-            sdcn += sl2 * gd1;
-            sdan += sl2 * gd2;
+            sL2 += sl2 * gs2 - w2;
+            sdcn += sl2 * gd1 + wz2;
+            sdan += sl2 * gd2 + wz2;
             sdp2 += T(2) * dp2 * (sl2 * gr);
             spz2 += sl2 * gr;
         }
         reverse_tail<T, false>(F, p, spx, T(0), T(0), st_, sdan, sdbn, sdcn, srho0, spz2,
-                               ss1b, ss1c, sdp1, sdp2, T(0), T(0), T(0), T(0), sL1, sL2, acc);
+                               ss1b, ss1c, sdp1, sdp2, T(0), T(0), T(0), sL0, sL1, sL2, acc);
     }
 }
 

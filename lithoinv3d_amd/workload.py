@@ -36,10 +36,11 @@ OPS = {
     # nested-fma projection), the small-angle atan series and its derivative, the series of
     # 2 atanh(z) and its derivative, coefficient seeds, reverse sweep through the edge
     # ratios, solid angle and distances (the near-edge seeds left to their own branch,
-    # doubled seeds for the squared distances), accumulation of the 19 facet seeds (their
+    # doubled seeds for the squared distances; the far-form edge seeds taken for every
+    # edge and removed again in the near branch), accumulation of the 19 facet seeds (their
     # FP64 flush every 16 stations is not counted); 3 rsq + 4 rcp (three edge ratios and
     # the solid angle's N/D)
-    "adjoint_pairs_f32": dict(fma=102, addmul=82, special=7),
+    "adjoint_pairs_f32": dict(fma=105, addmul=79, special=7),
 }
 
 
