@@ -146,8 +146,8 @@ int lf_stats(lf_ctx* ctx, int64_t* fp64_edges);
  * buffer of LF_NOALLOC_SCRATCH_BYTES per GPU, part of the library's code object (held from
  * the library's first use on a GPU until the process ends).  A model needs
  *   ~ n_faces * (200 B FP32C | 288 B FP64) + n_stations * 24 B
- *     + 16 B * n_stations * n_chunks   (FP32C: n_chunks ~ 4096 / ceil(n_stations / 512);
- *                                        FP64: 4096 / ceil(n_stations / 256))
+ *     + 16 B * n_stations * n_chunks   (n_chunks <= n_faces / 512 and <= 8 MiB / (16 B *
+ *                                        n_stations): at most 8 MiB of partial sums)
  * and a larger one is refused with LF_ENOMEM (lf_last_error(NULL) gives the bytes).
  * Every call on one GPU uses the same buffer: calls that overlap on a GPU race, so
  * serialize them (one stream, or wait for the previous call).  Facet vertex and unit ids

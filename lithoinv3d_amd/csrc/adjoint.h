@@ -754,15 +754,45 @@ inline int adjoint_init(AdjointWork& w, int64_t n_v, int64_t n_f, int64_t n_s,
     w.n_f = n_f;
     w.n_s = n_s;
     w.precision = precision;
-    // station chunks: enough blocks (facet blocks x chunks, 65536 or more) that the
-    // last partial round of resident blocks costs little, at least 16 stations per chunk,
-    // and at most 8 GiB of FP64 seeds (a seed buffer of one chunk always fits)
-    const int64_t face_blocks = std::max<int64_t>(1, (n_f + 255) / 256);
-    int64_t chunks = std::max<int64_t>(1, (65536 + face_blocks - 1) / face_blocks);
-    chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, (n_s + 15) / 16));
-    chunks = std::min<int64_t>(chunks, 65535);
+    // station chunks (grid = facet blocks x chunks): at least 16 stations per chunk and at
+    // most 8 GiB of FP64 seeds (a seed buffer of one chunk always fits), and among those
+    // the count that minimizes a cost model in units of one station's work for one round
+    // of resident blocks:
+    //   rounds(c) * (stations per chunk + kBlockFixed) + c * n_f * kFinalizePerFacet,
+    // rounds(c) = ceil(facet blocks * c / resident blocks).  The first term is the pair
+    // kernel (whose last round may be partly idle, and which pays a fixed set-up per
+    // block), the second the finalize's reads of every chunk's seeds.  Measured at
+    // 140,384 facets x 16,384 stations: 120 chunks of 137 stations (the earlier rule of
+    // at least 65,536 blocks) took 9.8 ms in the pair kernel and 3.0 ms in the finalize.
+    const int64_t face_blocks = std::max<int64_t>(1, (n_f + kAdjointBlock - 1) / kAdjointBlock);
     const size_t seed_bytes = (size_t)kSeeds * std::max<int64_t>(n_f, 1) * sizeof(double);
-    chunks = std::max<int64_t>(1, std::min<int64_t>(chunks, (int64_t)((8ull << 30) / seed_bytes)));
+    int64_t c_max = std::min<int64_t>(std::max<int64_t>(1, (n_s + 15) / 16), 65535);
+    c_max = std::max<int64_t>(1, std::min<int64_t>(c_max, (int64_t)((8ull << 30) / seed_bytes)));
+    int dev = 0, cus = 256, per_cu = 0;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+            cus = n;
+    }
+    if (precision == 1)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_adjoint_pairs_f32, kAdjointBlock, 0);
+    else
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_adjoint_pairs_f64, kAdjointBlock, 0);
+    const int64_t slots = (int64_t)cus * std::max(1, per_cu);
+    constexpr double kBlockFixed = 4.0;          // record load, column set-up, seed write
+    constexpr double kFinalizePerFacet = 1.8e-4; // 152 bytes of seeds per facet and chunk
+    int64_t chunks = 1;
+    double best = 1e300;
+    for (int64_t c = 1; c <= c_max; ++c) {
+        const int64_t spc = (n_s + c - 1) / c;
+        const int64_t rounds = (face_blocks * c + slots - 1) / slots;
+        const double cost = (double)rounds * ((double)spc + kBlockFixed) +
+                            (double)c * (double)n_f * kFinalizePerFacet;
+        if (cost < best) {
+            best = cost;
+            chunks = c;
+        }
+    }
     w.st_per_chunk = std::max<int64_t>(1, (n_s + chunks - 1) / chunks);
     w.n_chunks = (int)std::max<int64_t>(1, (n_s + w.st_per_chunk - 1) / w.st_per_chunk);
     const size_t nseed = (size_t)w.n_chunks * kSeeds * std::max<int64_t>(n_f, 1);

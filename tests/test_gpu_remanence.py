@@ -166,7 +166,11 @@ def test_property_phase_with_remanence_matches_reference_algorithm(oracle, mode)
     p_gpu = out.ravel()
     print(f"{mode}: product {p_gpu}\n reference algorithm {p_ref}\n"
           f" objective {f(p0):.3e} -> {f(p_gpu):.3e}")
-    np.testing.assert_allclose(p_gpu, p_ref, rtol=1e-6, atol=1e-9)
+    # the GPU's FP64 unit responses differ from the oracle's only in the last bits (the
+    # facet sums run in another order), and this least-squares problem has a condition
+    # number of about 1e6, so the two runs of the same algorithm agree to about 1e-8 in
+    # absolute terms: 2e-8 on a susceptibility of 0.006 was seen with one facet-chunk plan
+    np.testing.assert_allclose(p_gpu, p_ref, rtol=1e-6, atol=1e-7)
     assert f(p_gpu) < 1e-3 * f(p0)
     d, c = M.properties()
     mag, _, _, vec = M.remanence()
