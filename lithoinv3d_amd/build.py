@@ -48,7 +48,7 @@ def build_host(verbose: bool = False) -> str:
     linked against liblithofwd.so."""
     srcs = [os.path.join(HOST, f) for f in ("litho_host.cpp", "litho_io.cpp", "litho_runner.cpp",
                                             "litho_capi.cpp")]
-    cmd = [CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra",
+    cmd = [CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-Wall", "-Wextra",
            "-Wno-unused-parameter"] + srcs + [
         "-L" + HERE, "-llithofwd", "-Wl,-rpath,$ORIGIN", "-o", HOST_LIB + ".tmp"]
     if verbose:

@@ -191,6 +191,11 @@ private:
     std::vector<int64_t> m_wOff;
     std::vector<int32_t> m_wCp;
     std::vector<double> m_w;
+    // its transpose: CP vertex -> (dependent vertex, weight), CSR (for the gradient's pull-back)
+    std::vector<int64_t> m_depOff;
+    std::vector<int32_t> m_depV;
+    std::vector<double> m_depW;
+    void buildDependants();
     std::vector<int32_t> m_dofVertex, m_dofAxis;
 };
 
