@@ -437,3 +437,33 @@ def test_along_vector_freedom():
         e[j] = h
         fd = (M.smoothness(p + e, 1.0)[0] - M.smoothness(p - e, 1.0)[0]) / (2 * h)
         assert fd == pytest.approx(g[j], rel=1e-6, abs=1e-9 * np.abs(g).max())
+
+
+@pytest.mark.parametrize("mode", ["displacement", "position"])
+def test_threaded_parameter_map_and_pullback(mode):
+    """Models of 16 384 vertices or more apply parameters and pull vertex gradients back
+    on host threads (LithologyModel::applyParameterVector, pullBackVertexGradient).  On a
+    20 801-vertex contact model with XYZ_FREE control points every 4th cell: the pull-back
+    is the transpose of the applied map, and it equals the contraction of the explicit
+    parameter -> vertex map (li_model_dof_map, built by separate code) with the gradient."""
+    from lithoinv3d_amd import synth
+
+    m = synth.config_contact(shape=(40, 40, 24), n_st=8, seed=5)["model"]
+    assert m.n_vertices >= 16384
+    M = inversion.Model(m)
+    M.fix_exterior(inversion.XYZ_FREE)
+    M.set_axis_scale(10.0)
+    n = M.set_cp_stride(4, 25.0)
+    M.set_cp_interpolation(mode)
+    _, axis, _ = M.dofs()
+    p0 = M.params()
+    rng = np.random.default_rng(7)
+    dp = rng.normal(size=n)
+    dX = M.apply(p0 + dp) - M.apply(p0)
+    vg = rng.normal(size=(m.n_vertices, 3))
+    g = M.pullback(vg)
+    assert np.sum(vg * dX) == pytest.approx(g @ dp, rel=1e-12)
+    off, vert, w = M.dof_map()
+    g_map = np.array([np.sum(w[off[d]:off[d + 1]] * vg[vert[off[d]:off[d + 1]], axis[d]])
+                      for d in range(n)])
+    np.testing.assert_allclose(g, g_map, rtol=1e-12, atol=1e-12 * np.abs(g_map).max())
