@@ -497,18 +497,21 @@ __global__ __launch_bounds__(kAdjointBlock) void k_adjoint_pairs_f64(
 // corner order) is sum_m seed_m * T[m][k] for the per-pair (or accumulated) seeds of
 // SeedIdx.  Built from the facet constants with forward-mode duals along the nine
 // directions, in the corner order of the facet's frame record (frame_rotation), with the
-// columns returned in the facet's own order.  Returns false for a degenerate facet (T = 0).
-__device__ __forceinline__ bool facet_T(const double P0[3][3], double drho, const double Md[3],
-                                        const double Ms[3], const double Fd[3], bool textbook,
-                                        double Tm[kSeeds][9])
+// columns returned in the facet's own order.  facet_T_columns hands each column to
+// column(k, col) as soon as it is formed (col[m] = T[m][k]), so that a caller that only
+// contracts T with seeds keeps no 19 x 9 matrix; it returns false, calling nothing, for a
+// degenerate facet (T = 0).
+template <class Column>
+__device__ __forceinline__ bool facet_T_columns(const double P0[3][3], double drho,
+                                                const double Md[3], const double Ms[3],
+                                                const double Fd[3], bool textbook,
+                                                Column column)
 {
     const int rot = frame_rotation(P0);
     double P[3][3];
     rotate_corners(P0, rot, P);
     FrameVals<double> fv;
     facet_frame(P, drho, Md, Ms, Fd, textbook, fv);
-    for (int m = 0; m < kSeeds; ++m)
-        for (int k = 0; k < 9; ++k) Tm[m][k] = 0.0;
     if (fv.bad) return false;
     for (int kr = 0; kr < 9; ++kr) {
         // direction kr of the record's corners is column k of the facet's corners
@@ -524,31 +527,46 @@ __device__ __forceinline__ bool facet_T(const double P0[3][3], double drho, cons
             wxz += fd.u[j].d * fv.n[j];
             wyz += fd.v[j].d * fv.n[j];
         }
+        double col[kSeeds];
         if (kr < 3) {
             // p = R (o - a): moving a moves p by -R da
-            Tm[S_GPX][k] = -fv.u[kr];
-            Tm[S_GPY][k] = -fv.v[kr];
-            Tm[S_GPZ][k] = -fv.n[kr];
+            col[S_GPX] = -fv.u[kr];
+            col[S_GPY] = -fv.v[kr];
+            col[S_GPZ] = -fv.n[kr];
+        } else {
+            col[S_GPX] = col[S_GPY] = col[S_GPZ] = 0.0;
         }
-        Tm[S_AXY][k] = wxy;
-        Tm[S_AXZ][k] = wxz;
-        Tm[S_AYZ][k] = wyz;
-        Tm[S_L0][k] = fd.l0.d;
-        Tm[S_CU][k] = fd.cu.d;
-        Tm[S_CV][k] = fd.cv.d;
+        col[S_AXY] = wxy;
+        col[S_AXZ] = wxz;
+        col[S_AYZ] = wyz;
+        col[S_L0] = fd.l0.d;
+        col[S_CU] = fd.cu.d;
+        col[S_CV] = fd.cv.d;
         // d(angle) of the unit edge directions: (e_u, e_v) x d(e_u, e_v)
-        Tm[S_PHI1][k] = fv.e1u * fd.e1v.d - fv.e1v * fd.e1u.d;
-        Tm[S_L1][k] = fd.l1.d;
-        Tm[S_PHI2][k] = fv.e2u * fd.e2v.d - fv.e2v * fd.e2u.d;
-        Tm[S_L2][k] = fd.l2.d;
-        Tm[S_TWOA][k] = fd.twoA.d;
-        Tm[S_K1][k] = fd.K1.d;
-        Tm[S_W0][k] = fd.W0.d;
-        Tm[S_W1][k] = fd.W1.d;
-        Tm[S_W2][k] = fd.W2.d;
-        Tm[S_W3][k] = fd.W3.d;
+        col[S_PHI1] = fv.e1u * fd.e1v.d - fv.e1v * fd.e1u.d;
+        col[S_L1] = fd.l1.d;
+        col[S_PHI2] = fv.e2u * fd.e2v.d - fv.e2v * fd.e2u.d;
+        col[S_L2] = fd.l2.d;
+        col[S_TWOA] = fd.twoA.d;
+        col[S_K1] = fd.K1.d;
+        col[S_W0] = fd.W0.d;
+        col[S_W1] = fd.W1.d;
+        col[S_W2] = fd.W2.d;
+        col[S_W3] = fd.W3.d;
+        column(k, col);
     }
     return true;
+}
+
+__device__ __forceinline__ bool facet_T(const double P0[3][3], double drho, const double Md[3],
+                                        const double Ms[3], const double Fd[3], bool textbook,
+                                        double Tm[kSeeds][9])
+{
+    for (int m = 0; m < kSeeds; ++m)
+        for (int k = 0; k < 9; ++k) Tm[m][k] = 0.0;
+    return facet_T_columns(P0, drho, Md, Ms, Fd, textbook, [&](int k, const double* col) {
+        for (int m = 0; m < kSeeds; ++m) Tm[m][k] = col[m];
+    });
 }
 
 __device__ __forceinline__ void load_facet(int64_t f, const double* __restrict__ verts,
@@ -591,14 +609,15 @@ __global__ __launch_bounds__(128) void k_adjoint_finalize(
     double P[3][3], drho, Md[3], Ms[3];
     load_facet(f, verts, tri, front, back, density, mag, P, drho, Md, Ms);
     const double Fd[3] = {fx, fy, fz};
-    double Tm[kSeeds][9];
-    facet_T(P, drho, Md, Ms, Fd, tmi_mode == 1, Tm);
     double* out = face_grad + 9 * f;
-    for (int k = 0; k < 9; ++k) {
-        double g = 0.0;
-        for (int m = 0; m < kSeeds; ++m) g += s[m] * Tm[m][k];
-        out[k] = g;
-    }
+    const bool ok = facet_T_columns(P, drho, Md, Ms, Fd, tmi_mode == 1,
+                                    [&](int k, const double* col) {
+                                        double g = 0.0;
+                                        for (int m = 0; m < kSeeds; ++m) g += s[m] * col[m];
+                                        out[k] = g;
+                                    });
+    if (!ok)
+        for (int k = 0; k < 9; ++k) out[k] = 0.0;
 }
 
 // ------------------------------------------------- Jacobian column norms (per DOF)
