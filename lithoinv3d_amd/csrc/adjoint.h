@@ -409,10 +409,11 @@ __device__ __forceinline__ void pair_seeds(const FaceFrame<T>& F, double ox, dou
 // vertex sensitivity of size (L/R)^2).  The FP32 sums therefore run over at most
 // kFlushStations stations and are then added into FP64 sums, one LDS column per thread
 // (19 x 256 doubles, 38 KiB per block), written to the chunk's seeds at the end.  On the
-// 192x192x64 contact model with 65,536 stations the FP32 adjoint's largest deviation from
-// the FP64 one is 8.9e-6 of the largest component with 32 stations per flush, 6.5e-6 with
-// 16 (1 % more time) and 6.0e-6 with 8 (3.5 % more): below 16 the per-pair roundings,
-// not the sums, set the error.
+// 192x192x64 contact model with 65,536 stations in 8 station chunks the FP32 adjoint's
+// largest deviation from the FP64 one is 8.9e-6 of the largest component with 32 stations
+// per flush, 6.5e-6 with 16 (1 % more time) and 6.0e-6 with 8 (3.5 % more): below 16 the
+// per-pair roundings, not the sums, set the error (with 9 chunks and 16 stations: 7.4e-6;
+// the chunk boundaries decide which stations share an FP32 run).
 #ifndef LF_ADJOINT_FLUSH
 #define LF_ADJOINT_FLUSH 16
 #endif

@@ -10,7 +10,8 @@
 //                     LDS in stages of 64 and are read at a uniform address; each lane
 //                     keeps its own accumulators (FP32 sums over groups of 16 facets,
 //                     Kahan-compensated across groups), so the facet reduction needs no
-//                     cross-lane traffic.  Grid = station tiles x facet chunks.
+//                     cross-lane traffic; two stations per lane share each record
+//                     read.  Grid = station tiles x facet chunks, about 128 blocks per CU.
 //   k_forward_f64     the same map in FP64 with the reference's operation sequence;
 //                     records arrive through scalar loads.
 //   k_reduce_chunks   sums the per-chunk partials of each station in a fixed order.
