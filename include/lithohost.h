@@ -186,6 +186,11 @@ typedef struct {
     double density_lower, density_upper, density_damping;
     double susceptibility_lower, susceptibility_upper, susceptibility_damping;
     double property_tolerance;
+    int32_t solver;              /* geometry solver (runner.h:49-51, runner.cpp:192-212):
+                                    0 L-BFGS-B, 1 Gauss-Newton CG (li_gncg_*) */
+    int32_t gncg_cg_max_iter;    /* inner CG iterations (runner.h:54, default 50) */
+    double gncg_cg_tolerance;    /* CG residual tolerance (runner.h:55, default 1e-6) */
+    double fd_step;              /* Hessian-vector difference step (runner.h:39, default 1) */
 } LiRunnerOptions;
 /* hist rows of 10: iteration, gravity misfit, magnetic misfit, regularization, total,
  * gravity RMS, Durbin-Watson gravity x, y, magnetic x, y (InversionIteration,
@@ -240,6 +245,37 @@ int li_problem_minimize(void* h, const double* x0, const double* lb, const doubl
                         const LiOptions* o, const double* h0, double* x_out, double* hist,
                         int64_t hist_cap, int64_t* n_hist, int32_t* iterations, char* msg,
                         int64_t msg_cap);
+
+/* ---- Gauss-Newton CG (gncg_optimizer.cpp:13-194, gncg_optimizer.h:6-65) ---- */
+
+/* GNCGOptimizer's settings: outer iterations, inner CG iterations and residual tolerance,
+ * the step h = fd_step / |v| of the central-difference Hessian-vector product
+ * (g(x + h v) - g(x - h v)) / 2h, the projected-gradient tolerance (relative to
+ * max(1, its first value)), the backtracking limit; armijo_c1 is kept for the
+ * interface, the reference's line search accepts any step that does not raise f */
+typedef struct {
+    int32_t max_iter;
+    int32_t cg_max_iter;
+    double cg_tolerance;
+    double fd_step;
+    double ftol;
+    double armijo_c1;
+    int32_t line_search_max_iter;
+} LiGncgOptions;
+/* GNCGOptimizer::minimize (gncg_optimizer.cpp:105-194) on callbacks.  hist rows of 7 as
+ * li_minimize_callbacks, one per iterate once f and the gradient there are known (a final
+ * iterate whose gradient was never evaluated has NaN as its projected-gradient norm);
+ * cg_iters[k]: the inner CG iterations of outer iteration k (up to cg_cap) */
+int li_gncg_minimize_callbacks(int64_t n, const double* x0, const double* lb, const double* ub,
+                               li_obj_cb f, li_grad_cb g, void* user, const LiGncgOptions* o,
+                               double* x_out, double* hist, int64_t hist_cap, int64_t* n_hist,
+                               int32_t* cg_iters, int64_t cg_cap, int32_t* iterations, char* msg,
+                               int64_t msg_cap);
+/* the same on the problem's objective and GPU gradient; rows go to its iteration callback */
+int li_problem_gncg_minimize(void* h, const double* x0, const double* lb, const double* ub,
+                             const LiGncgOptions* o, double* x_out, double* hist,
+                             int64_t hist_cap, int64_t* n_hist, int32_t* cg_iters,
+                             int64_t cg_cap, int32_t* iterations, char* msg, int64_t msg_cap);
 
 /* ---- file formats (modules/litho-io) ---- */
 

@@ -17,7 +17,12 @@
                           large workload with 50 boreholes' depth-to-contact picks and the
                           learned H0, resident on one GPU.
 
+  * ``compare_gncg``      the runner's two geometry solvers, L-BFGS-B and Gauss-Newton CG,
+                          from the same start on one model: objective against seconds and
+                          gradient evaluations.
+
 CLI:  python -m lithoinv3d_amd.experiments learned-h0 --train 200 --test 20 --out FILE
+      python -m lithoinv3d_amd.experiments solvers --seed 2000 --out FILE
       python -m lithoinv3d_amd.experiments large --max-iter 3
 """
 from __future__ import annotations
@@ -92,12 +97,12 @@ def synth_data(prob, precision="fp64"):
     return gz, tm
 
 
-def make_problem(prob, precision):
+def make_problem(prob, precision, smooth_weight=0.0):
     st = prob["stations"]
     n = len(st)
     return inversion.Problem(prob["M"], st, prob["gz_obs"], np.full(n, SIGMA_GZ),
                              prob["tmi_obs"], np.full(n, SIGMA_TMI), alpha_mag=1.0,
-                             precision=precision)
+                             precision=precision, smooth_weight=smooth_weight)
 
 
 def features_targets(prob, with_targets=True):
@@ -220,6 +225,51 @@ def compare_solvers(prob, net=None, norm=None, max_iter=40, exact_diag=False,
                                        diag_seconds=t_diag, reached=hit is not None,
                                        to_target=None if hit is None else
                                        (hit[0], hit[1] + t_diag))
+    return out
+
+
+def compare_gncg(prob, lbfgs_iter=100, gncg_iter=8, cg_max_iter=50, fd_step=1.0,
+                 precision="fp32c", smooth_weight=0.0):
+    """The runner's two geometry solvers from the same start on one problem: L-BFGS-B
+    (history 10) and Gauss-Newton CG with the runner's settings (50 CG iterations,
+    tolerance 1e-6, fd_step 1 parameter unit), each Hessian-vector product two gradients.
+    Reports each GN-CG iterate's objective, gradient count and seconds beside the first
+    L-BFGS-B iterate at or below that objective.  smooth_weight: the Laplacian smoothness
+    term of the objective (the runner's lambda, 1 by default there)."""
+    synth_data(prob)
+    out = {"seed": prob["seed"], "n_dof": prob["n_dof"], "precision": precision,
+           "cg_max_iter": cg_max_iter, "fd_step": fd_step, "smooth_weight": smooth_weight}
+    runs = {}
+    for name in ("lbfgsb", "gncg"):
+        P = make_problem(prob, precision, smooth_weight)
+        P.gradient(prob["p0"])  # context set-up outside the timed solve
+        log(f"{name}: solving")
+        if name == "lbfgsb":
+            r = P.minimize(prob["p0"], prob["lo"], prob["hi"], max_iter=lbfgs_iter)
+        else:
+            r = P.gncg_minimize(prob["p0"], prob["lo"], prob["hi"], max_iter=gncg_iter,
+                                cg_max_iter=cg_max_iter, fd_step=fd_step)
+        P.close()
+        runs[name] = r
+        out[name] = dict(iterations=r.iterations, message=r.message, f0=float(r.history[0, 1]),
+                         f_final=r.f, seconds=r.seconds,
+                         gradients=int(r.history[-1, 6]),
+                         history=r.history[:, [0, 1, 3, 4, 5, 6]].tolist())
+        if name == "gncg":
+            out[name]["cg_iterations"] = [int(c) for c in r.cg_iterations]
+        log(f"{name}: {r.iterations} iterations, f {r.history[0, 1]:.4e} -> {r.f:.4e} in "
+            f"{r.seconds:.2f} s ({r.message})")
+    rows = []
+    lb = runs["lbfgsb"]
+    for it, f, sec, _, ng in runs["gncg"].history[:, [0, 1, 4, 5, 6]]:
+        hit = time_to(lb, f)
+        k = None if hit is None else int(np.nonzero(lb.history[:, 1] <= f)[0][0])
+        rows.append(dict(gncg_iteration=int(it), f=float(f), gncg_seconds=float(sec),
+                         gncg_gradients=int(ng),
+                         lbfgsb_iteration=None if hit is None else hit[0],
+                         lbfgsb_seconds=None if hit is None else hit[1],
+                         lbfgsb_gradients=None if k is None else int(lb.history[k, 6])))
+    out["at_equal_objective"] = rows
     return out
 
 
@@ -459,6 +509,17 @@ def main(argv=None):
     d.add_argument("--jsonl", required=True)
     d.add_argument("--net", required=True, help="the network file of the runs (.npz)")
     d.add_argument("--out", default=None)
+    sv = sub.add_parser("solvers", help="L-BFGS-B against Gauss-Newton CG on one model")
+    sv.add_argument("--seed", type=int, default=2000)
+    sv.add_argument("--shape", type=int, nargs=3, default=(64, 64, 32))
+    sv.add_argument("--n-st", type=int, default=128)
+    sv.add_argument("--lbfgs-iter", type=int, default=100)
+    sv.add_argument("--gncg-iter", type=int, default=8)
+    sv.add_argument("--cg-max-iter", type=int, default=50)
+    sv.add_argument("--fd-step", type=float, default=1.0)
+    sv.add_argument("--precision", choices=("fp64", "fp32c"), default="fp32c")
+    sv.add_argument("--smooth", type=float, default=0.0, help="Laplacian smoothness weight")
+    sv.add_argument("--out", default=None)
     c = sub.add_parser("large")
     c.add_argument("--workload", default="contact256x256x96_st512")
     c.add_argument("--stride", type=int, default=8)
@@ -477,6 +538,14 @@ def main(argv=None):
         if args.out:
             json.dump(res, open(args.out, "w"), indent=1)
         print(json.dumps(res))
+        return
+    if args.cmd == "solvers":
+        prob = contact_problem(args.seed, tuple(args.shape), args.n_st)
+        res = compare_gncg(prob, args.lbfgs_iter, args.gncg_iter, args.cg_max_iter,
+                           args.fd_step, args.precision, args.smooth)
+        if args.out:
+            json.dump(res, open(args.out, "w"), indent=1)
+        print(json.dumps({k: v for k, v in res.items() if k != "lbfgsb"}))
         return
     if args.cmd == "summarize-h0":
         results = [json.loads(line) for line in open(args.jsonl) if line.strip()]

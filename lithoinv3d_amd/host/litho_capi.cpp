@@ -510,6 +510,12 @@ int li_problem_run(void* h, const LiRunnerOptions* o, void* label_grid, double* 
         ro.tolerance = o->tolerance;
         ro.armijoC1 = o->armijo_c1;
         ro.lineSearchMaxIter = o->line_search_max_iter;
+        if (o->solver != 0 && o->solver != 1)
+            throw std::invalid_argument("li_problem_run: solver must be 0 (lbfgsb) or 1 (gncg)");
+        ro.solver = (RunnerOptions::Solver)o->solver;
+        ro.gncgCGMaxIter = o->gncg_cg_max_iter;
+        ro.gncgCGTolerance = o->gncg_cg_tolerance;
+        ro.fdStep = o->fd_step;
         ro.densityOptions = {o->density_damping, o->density_lower, o->density_upper,
                              o->property_max_iter, o->property_tolerance};
         ro.susceptibilityOptions = {o->susceptibility_damping, o->susceptibility_lower,
@@ -834,6 +840,77 @@ int li_problem_minimize(void* h, const double* x0, const double* lb, const doubl
     auto grad = [&](const Vec& x) { return P->grad(x); };
     return run_minimize(obj, grad, n, x0, lb, ub, o, h0, x_out, hist, hist_cap, n_hist, iterations,
                         msg, msg_cap, P->iterCb, P->iterUser);
+}
+
+static int run_gncg(GNCGOptimizer::Objective obj, GNCGOptimizer::Gradient grad, int64_t n,
+                    const double* x0, const double* lb, const double* ub,
+                    const LiGncgOptions* o, double* x_out, double* hist, int64_t hist_cap,
+                    int64_t* n_hist, int32_t* cg_iters, int64_t cg_cap, int32_t* iterations,
+                    char* msg, int64_t msg_cap, li_iter_cb cb = nullptr, void* cb_user = nullptr)
+{
+    return guard([&] {
+        if (o->max_iter < 0 || o->cg_max_iter < 0 || o->line_search_max_iter < 0)
+            throw std::invalid_argument("li_gncg: iteration limits must be >= 0");
+        if (!(o->fd_step > 0.0)) throw std::invalid_argument("li_gncg: fd_step must be > 0");
+        GNCGOptimizer opt;
+        if (cb) {
+            opt.setIterationCallback([cb, cb_user](const GNCGOptimizer::IterationRecord& r) {
+                double row[7];
+                minimize_row(r, row);
+                cb(0, row, 7, cb_user);
+            });
+        }
+        opt.setMaxIterations(o->max_iter);
+        opt.setCGMaxIter(o->cg_max_iter);
+        opt.setCGTolerance(o->cg_tolerance);
+        opt.setFDStep(o->fd_step);
+        opt.setTolerance(o->ftol);
+        opt.setArmijoC1(o->armijo_c1);
+        opt.setLineSearchMaxIter(o->line_search_max_iter);
+        const auto r = opt.minimize(obj, grad, Vec(x0, x0 + n), Vec(lb, lb + n), Vec(ub, ub + n));
+        std::memcpy(x_out, r.params.data(), sizeof(double) * n);
+        const auto& H = opt.history();
+        const int64_t nh = std::min<int64_t>((int64_t)H.size(), hist_cap);
+        for (int64_t k = 0; k < nh; ++k) minimize_row(H[k], hist + 7 * k);
+        *n_hist = nh;
+        const auto& C = opt.cgIterations();
+        for (int64_t k = 0; k < std::min<int64_t>((int64_t)C.size(), cg_cap); ++k)
+            cg_iters[k] = C[k];
+        *iterations = r.iterations;
+        if (msg && msg_cap > 0) {
+            std::strncpy(msg, r.message.c_str(), (size_t)msg_cap - 1);
+            msg[msg_cap - 1] = 0;
+        }
+    });
+}
+
+int li_gncg_minimize_callbacks(int64_t n, const double* x0, const double* lb, const double* ub,
+                               li_obj_cb f, li_grad_cb g, void* user, const LiGncgOptions* o,
+                               double* x_out, double* hist, int64_t hist_cap, int64_t* n_hist,
+                               int32_t* cg_iters, int64_t cg_cap, int32_t* iterations, char* msg,
+                               int64_t msg_cap)
+{
+    auto obj = [&](const Vec& x) { return f(x.data(), (int64_t)x.size(), user); };
+    auto grad = [&](const Vec& x) {
+        Vec out(x.size());
+        g(x.data(), (int64_t)x.size(), out.data(), user);
+        return out;
+    };
+    return run_gncg(obj, grad, n, x0, lb, ub, o, x_out, hist, hist_cap, n_hist, cg_iters, cg_cap,
+                    iterations, msg, msg_cap);
+}
+
+int li_problem_gncg_minimize(void* h, const double* x0, const double* lb, const double* ub,
+                             const LiGncgOptions* o, double* x_out, double* hist,
+                             int64_t hist_cap, int64_t* n_hist, int32_t* cg_iters,
+                             int64_t cg_cap, int32_t* iterations, char* msg, int64_t msg_cap)
+{
+    auto* P = (Problem*)h;
+    const int64_t n = P->model->totalDofCount();
+    auto obj = [&](const Vec& x) { return P->components(x).total; };
+    auto grad = [&](const Vec& x) { return P->grad(x); };
+    return run_gncg(obj, grad, n, x0, lb, ub, o, x_out, hist, hist_cap, n_hist, cg_iters, cg_cap,
+                    iterations, msg, msg_cap, P->iterCb, P->iterUser);
 }
 
 int li_problem_set_iteration_callback(void* h, li_iter_cb cb, void* user)

@@ -1856,4 +1856,160 @@ OptimizerResult LBFGSBOptimizer::minimize(Objective objective, Gradient gradient
     return res;
 }
 
+// =================================================================== GN-CG
+
+Vec GNCGOptimizer::cgSolve(const std::function<Vec(const Vec&)>& Hv, const Vec& b,
+                           int& iters) const
+{
+    // gncg_optimizer.cpp:39-77: CG from x = 0.  A direction with p.Hp <= 1e-16 takes the
+    // step |r|^2 / |p|^2 and still updates the residual with Hp.  Hv runs before the
+    // |p|^2 check, as there.
+    const size_t n = b.size();
+    Vec x(n, 0.0), r = b, p = b;
+    double rs = dot(r, r);
+    iters = 0;
+    if (rs < 1e-30) return x;
+    for (int j = 0; j < m_cgMaxIter; ++j) {
+        const Vec Hp = Hv(p);
+        ++iters;
+        const double p2 = dot(p, p);
+        if (p2 < 1e-30) break;
+        const double pHp = dot(p, Hp);
+        const double a = pHp > 1e-16 ? rs / pHp : rs / p2;
+        for (size_t i = 0; i < n; ++i) {
+            x[i] += a * p[i];
+            r[i] -= a * Hp[i];
+        }
+        const double rsNew = dot(r, r);
+        if (rsNew < m_cgTol * m_cgTol) break;
+        const double beta = rsNew / rs;
+        for (size_t i = 0; i < n; ++i) p[i] = r[i] + beta * p[i];
+        rs = rsNew;
+    }
+    return x;
+}
+
+OptimizerResult GNCGOptimizer::minimize(Objective objective, Gradient gradient, const Vec& x0,
+                                        const Vec& lb, const Vec& ub)
+{
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    OptimizerResult res;
+    m_iter = 0;
+    m_cgIters.clear();
+    m_hist.clear();
+    int nObj = 0, nGrad = 0;
+    auto fobj = [&](const Vec& x) { ++nObj; return objective(x); };
+    auto fgrad = [&](const Vec& x) { ++nGrad; return gradient(x); };
+    const size_t n = x0.size();
+    // gncg_optimizer.cpp:24-37: a component at a bound (within 1e-12) whose gradient
+    // points out of the box does not count
+    auto pgnorm = [&](const Vec& x, const Vec& g) {
+        double s = 0.0;
+        for (size_t i = 0; i < n; ++i) {
+            double pg = g[i];
+            if (x[i] <= lb[i] + 1e-12 && g[i] > 0) pg = 0.0;
+            if (x[i] >= ub[i] - 1e-12 && g[i] < 0) pg = 0.0;
+            s += pg * pg;
+        }
+        return std::sqrt(s);
+    };
+    auto record = [&](int k, double f, double pg, double alpha) {
+        m_hist.push_back({k, f, pg, alpha,
+                          std::chrono::duration<double>(clk::now() - t0).count(), nObj, nGrad});
+        if (m_onIteration) m_onIteration(m_hist.back());
+    };
+    Vec x = project(x0, lb, ub);
+    double pg0 = 0.0, fx = 0.0, lastAlpha = 0.0;
+    bool recorded = false;  // x's row is in the history
+    int stall = 0;
+    for (int k = 0; k < m_maxIter; ++k) {
+        m_iter = k + 1;
+        const Vec g = fgrad(x);
+        const double pg = pgnorm(x, g);
+        if (k == 0) pg0 = pg;
+        if (k > 0) {
+            record(k, fx, pg, lastAlpha);
+            recorded = true;
+            if (pg < m_ftol * std::max(1.0, pg0)) {
+                res.converged = true;
+                res.message = "Converged: projected gradient below tolerance";
+                break;
+            }
+        }
+        // H v by central differences of the gradient (:134-147)
+        auto Hv = [&](const Vec& v) {
+            const double vn = std::sqrt(dot(v, v));
+            if (vn < 1e-15) return Vec(n, 0.0);
+            const double h = m_fdStep / vn;
+            Vec xp(n), xm(n);
+            for (size_t i = 0; i < n; ++i) {
+                xp[i] = x[i] + h * v[i];
+                xm[i] = x[i] - h * v[i];
+            }
+            const Vec gp = fgrad(project(xp, lb, ub));
+            const Vec gm = fgrad(project(xm, lb, ub));
+            Vec out(n);
+            for (size_t i = 0; i < n; ++i) out[i] = (gp[i] - gm[i]) / (2.0 * h);
+            return out;
+        };
+        Vec b(n);
+        for (size_t i = 0; i < n; ++i) b[i] = -g[i];
+        int cgIt = 0;
+        Vec d = cgSolve(Hv, b, cgIt);
+        m_cgIters.push_back(cgIt);
+        if (dot(d, g) >= 0.0) d = b;  // not a descent direction: steepest descent (:151-157)
+        // backtracking from 1 by halves until the projected point does not raise the
+        // objective at x, evaluated afresh (:79-103)
+        const double f0 = fobj(x);
+        if (k == 0) {
+            record(0, f0, pg, 0.0);
+            recorded = true;
+        }
+        double alpha = 1.0, fn = f0;
+        Vec xn;
+        bool found = false;
+        for (int i = 0; i < m_lsMax; ++i) {
+            Vec xt(n);
+            for (size_t j = 0; j < n; ++j) xt[j] = x[j] + alpha * d[j];
+            xt = project(xt, lb, ub);
+            const double ft = fobj(xt);
+            if (ft <= f0) {
+                xn = std::move(xt);
+                fn = ft;
+                found = true;
+                break;
+            }
+            alpha *= 0.5;
+        }
+        if (!found) {
+            res.converged = true;
+            res.message = "Converged: no step taken";
+            break;
+        }
+        double step = 0.0;
+        for (size_t i = 0; i < n; ++i) step += (xn[i] - x[i]) * (xn[i] - x[i]);
+        if (std::sqrt(step) < 1e-8) {
+            if (++stall >= 3) {
+                res.converged = true;
+                res.message = "Converged: step stalled";
+                break;
+            }
+        } else {
+            stall = 0;
+        }
+        x = std::move(xn);
+        fx = fn;
+        lastAlpha = alpha;
+        recorded = false;
+    }
+    res.params = x;
+    res.finalValue = fobj(x);
+    res.iterations = m_iter;
+    if (!recorded)
+        record(m_iter, res.finalValue, std::numeric_limits<double>::quiet_NaN(), lastAlpha);
+    if (!res.converged && m_iter >= m_maxIter) res.message = "Maximum iterations reached";
+    return res;
+}
+
 }  // namespace lithofwd

@@ -578,4 +578,44 @@ private:
     std::vector<IterationRecord> m_hist;
 };
 
+// Projected Gauss-Newton with an inner conjugate-gradient solve:
+// modules/litho-inversion/src/inversion/gncg_optimizer.cpp:13-194 (the runner's
+// solver = "gncg", runner.cpp:192-202).  Each outer iteration solves H d = -g by CG, where
+// H v is the central difference of the gradient at x -/+ (fd_step / |v|) v (both points
+// projected onto the bounds), falls back to -g when d is not a descent direction, and
+// halves the step from 1 until the projected point does not raise the objective.  The
+// Armijo constant is kept for the interface; the reference's line search does not use it.
+class GNCGOptimizer {
+public:
+    using Objective = std::function<double(const Vec&)>;
+    using Gradient = std::function<Vec(const Vec&)>;
+    // one row per iterate x_k once both f(x_k) and the gradient there are known; the
+    // final row, written when the loop ends at a point whose gradient was not evaluated,
+    // has pgNorm = NaN.  alpha: the step that reached x_k.
+    using IterationRecord = LBFGSBOptimizer::IterationRecord;
+    using IterationCallback = std::function<void(const IterationRecord&)>;
+    OptimizerResult minimize(Objective objective, Gradient gradient, const Vec& x0,
+                             const Vec& lb, const Vec& ub);
+    void setMaxIterations(int n) { m_maxIter = n; }
+    void setTolerance(double t) { m_ftol = t; }
+    void setCGMaxIter(int n) { m_cgMaxIter = n; }
+    void setCGTolerance(double t) { m_cgTol = t; }
+    void setFDStep(double h) { m_fdStep = h; }
+    void setArmijoC1(double c) { m_c1 = c; }
+    void setLineSearchMaxIter(int n) { m_lsMax = n; }
+    void setIterationCallback(IterationCallback cb) { m_onIteration = std::move(cb); }
+    int iterationCount() const { return m_iter; }
+    // inner CG iterations of each outer iteration, in order
+    const std::vector<int>& cgIterations() const { return m_cgIters; }
+    const std::vector<IterationRecord>& history() const { return m_hist; }
+
+private:
+    Vec cgSolve(const std::function<Vec(const Vec&)>& Hv, const Vec& b, int& iters) const;
+    IterationCallback m_onIteration;
+    int m_maxIter = 500, m_cgMaxIter = 50, m_lsMax = 30, m_iter = 0;
+    double m_ftol = 1e-6, m_cgTol = 1e-6, m_fdStep = 2.0, m_c1 = 1e-4;
+    std::vector<int> m_cgIters;
+    std::vector<IterationRecord> m_hist;
+};
+
 }  // namespace lithofwd

@@ -461,14 +461,29 @@ RunnerResult runInversion(std::shared_ptr<LithologyModel> model, RunnerProblem& 
         if (opt.geometry) {
             const int phase = std::min(opt.propertyInterval, opt.maxIterations - total);
             if (phase <= 0) break;
-            LBFGSBOptimizer lb;
-            lb.setMaxIterations(phase);
-            lb.setHistorySize(opt.lbfgsHistory);
-            lb.setTolerance(opt.tolerance);
-            lb.setArmijoC1(opt.armijoC1);
-            lb.setLineSearchMaxIter(opt.lineSearchMaxIter);
-            const OptimizerResult geo = lb.minimize(geoObj, geoGrad, params, lower, upper);
-            log << "geometry phase: " << geo.iterations << " iterations, f = " << geo.finalValue
+            OptimizerResult geo;
+            if (opt.solver == RunnerOptions::Solver::GNCG) {
+                GNCGOptimizer gn;
+                gn.setMaxIterations(phase);
+                gn.setCGMaxIter(opt.gncgCGMaxIter);
+                gn.setCGTolerance(opt.gncgCGTolerance);
+                gn.setFDStep(opt.fdStep);
+                gn.setArmijoC1(opt.armijoC1);
+                gn.setLineSearchMaxIter(opt.lineSearchMaxIter);
+                gn.setTolerance(opt.tolerance);
+                geo = gn.minimize(geoObj, geoGrad, params, lower, upper);
+            } else {
+                LBFGSBOptimizer lb;
+                lb.setMaxIterations(phase);
+                lb.setHistorySize(opt.lbfgsHistory);
+                lb.setTolerance(opt.tolerance);
+                lb.setArmijoC1(opt.armijoC1);
+                lb.setLineSearchMaxIter(opt.lineSearchMaxIter);
+                geo = lb.minimize(geoObj, geoGrad, params, lower, upper);
+            }
+            log << "geometry phase"
+                << (opt.solver == RunnerOptions::Solver::GNCG ? " (gncg)" : "") << ": "
+                << geo.iterations << " iterations, f = " << geo.finalValue
                 << " (" << geo.message << ")\n";
             const Vec prev = params;
             bool accepted = false;

@@ -97,6 +97,13 @@ struct RunnerOptions {
     double tolerance = 1e-6;
     double armijoC1 = 1e-4;
     int lineSearchMaxIter = 50;
+    // geometry solver (runner.h:49-55, runner.cpp:192-212): L-BFGS-B, or Gauss-Newton
+    // with an inner CG solve whose Hessian-vector products are central differences of
+    // the gradient with step fdStep / |v|
+    enum class Solver { LBFGSB = 0, GNCG = 1 } solver = Solver::LBFGSB;
+    int gncgCGMaxIter = 50;
+    double gncgCGTolerance = 1e-6;
+    double fdStep = 1.0;
 };
 
 // InversionIteration (inversion_result.h:10-28), gravity and magnetics.

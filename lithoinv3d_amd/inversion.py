@@ -39,7 +39,19 @@ class _RunnerOptions(ctypes.Structure):
         "property_max_iter", "model_bounds")] + [(n, ctypes.c_double) for n in (
             "model_bounds_multiplier", "depth_margin", "bounds_margin", "gradient_smoothing", "tolerance", "armijo_c1",
             "density_lower", "density_upper", "density_damping", "susceptibility_lower",
-            "susceptibility_upper", "susceptibility_damping", "property_tolerance")]
+            "susceptibility_upper", "susceptibility_damping", "property_tolerance")] + [
+        ("solver", ctypes.c_int32), ("gncg_cg_max_iter", ctypes.c_int32),
+        ("gncg_cg_tolerance", ctypes.c_double), ("fd_step", ctypes.c_double)]
+
+
+class _GncgOptions(ctypes.Structure):
+    _fields_ = [("max_iter", ctypes.c_int32), ("cg_max_iter", ctypes.c_int32),
+                ("cg_tolerance", ctypes.c_double), ("fd_step", ctypes.c_double),
+                ("ftol", ctypes.c_double), ("armijo_c1", ctypes.c_double),
+                ("line_search_max_iter", ctypes.c_int32)]
+
+
+SOLVERS = {"lbfgsb": 0, "gncg": 1}
 
 
 OBJ_CB = ctypes.CFUNCTYPE(ctypes.c_double, _D, ctypes.c_int64, ctypes.c_void_p)
@@ -122,6 +134,10 @@ def load_host():
             ctypes.POINTER(ctypes.c_int32), ctypes.c_char_p, i64]
     L.li_minimize_callbacks.argtypes = mins + [OBJ_CB, GRAD_CB, _P] + tail
     L.li_problem_minimize.argtypes = [_P, _P, _P, _P] + tail
+    gtail = [ctypes.POINTER(_GncgOptions), _P, _P, i64, ctypes.POINTER(i64), _P, i64,
+             ctypes.POINTER(ctypes.c_int32), ctypes.c_char_p, i64]
+    L.li_gncg_minimize_callbacks.argtypes = mins + [OBJ_CB, GRAD_CB, _P] + gtail
+    L.li_problem_gncg_minimize.argtypes = [_P, _P, _P, _P] + gtail
     _lib = L
     return L
 
@@ -372,6 +388,7 @@ class Result:
     iterations: int
     message: str
     history: np.ndarray  # rows: iteration, f, |pg|, alpha, seconds, n_obj, n_grad
+    cg_iterations: np.ndarray = None  # Gauss-Newton CG: inner iterations per outer one
 
     @property
     def seconds(self):
@@ -397,6 +414,48 @@ def _finish(L, call, n, max_iter, *args):
     rc = call(*args, _p(x), _p(hist), max_iter + 2, ctypes.byref(nh), ctypes.byref(it), msg, 256)
     _chk(rc, "minimize")
     return Result(x, it.value, msg.value.decode(), hist[:nh.value].copy())
+
+
+def _gncg_finish(call, n, o, *args):
+    x = np.zeros(n)
+    cap = o.max_iter + 2
+    hist = np.zeros((cap, 7))
+    cg = np.full(max(o.max_iter, 1), -1, dtype=np.int32)  # -1: no CG solve
+    nh = ctypes.c_int64()
+    it = ctypes.c_int32()
+    msg = ctypes.create_string_buffer(256)
+    rc = call(*args, ctypes.byref(o), _p(x), _p(hist), cap, ctypes.byref(nh), _p(cg), len(cg),
+              ctypes.byref(it), msg, 256)
+    _chk(rc, "gncg_minimize")
+    return Result(x, it.value, msg.value.decode(), hist[:nh.value].copy(), cg[cg >= 0].copy())
+
+
+def _gncg_opts(max_iter, cg_max_iter, cg_tol, fd_step, ftol, c1, line_search_max_iter):
+    return _GncgOptions(max_iter=max_iter, cg_max_iter=cg_max_iter, cg_tolerance=cg_tol,
+                        fd_step=fd_step, ftol=ftol, armijo_c1=c1,
+                        line_search_max_iter=line_search_max_iter)
+
+
+def gncg_minimize(fun, grad, x0, lb, ub, max_iter=100, cg_max_iter=50, cg_tol=1e-6,
+                  fd_step=2.0, ftol=1e-6, c1=1e-4, line_search_max_iter=30):
+    """The host Gauss-Newton CG (GNCGOptimizer, gncg_optimizer.cpp:105-194) on Python
+    callables; the defaults are GNCGOptimizer's own (gncg_optimizer.h:41-45).  The
+    result's cg_iterations holds the inner CG iterations of each outer iteration."""
+    L = load_host()
+    x0, lb, ub = _a(x0), _a(lb), _a(ub)
+    n = len(x0)
+
+    def f_cb(xp, nn, user):
+        return float(fun(np.ctypeslib.as_array(xp, (nn,)).copy()))
+
+    def g_cb(xp, nn, gp, user):
+        g = np.asarray(grad(np.ctypeslib.as_array(xp, (nn,)).copy()), dtype=np.float64)
+        np.ctypeslib.as_array(gp, (nn,))[:] = g
+
+    fcb, gcb = OBJ_CB(f_cb), GRAD_CB(g_cb)
+    o = _gncg_opts(max_iter, cg_max_iter, cg_tol, fd_step, ftol, c1, line_search_max_iter)
+    return _gncg_finish(L.li_gncg_minimize_callbacks, n, o, n, _p(x0), _p(lb), _p(ub), fcb,
+                        gcb, None)
 
 
 def minimize(fun, grad, x0, lb, ub, max_iter=100, history=10, ftol=1e-6, c1=1e-4,
@@ -519,13 +578,18 @@ class Problem:
             tolerance=1e-6, armijo_c1=1e-4, line_search_max_iter=50,
             density_bounds=(-1e300, 1e300), density_damping=0.0,
             susceptibility_bounds=(0.0, 1.0), susceptibility_damping=0.0,
-            property_max_iter=50, property_tolerance=1e-6, history_cap=None):
+            property_max_iter=50, property_tolerance=1e-6, history_cap=None,
+            solver="lbfgsb", gncg_cg_max_iter=50, gncg_cg_tolerance=1e-6, fd_step=1.0):
         """InversionRunner::run from the model's current parameters (bounds: the start
         -/+ bounds_margin metres, or with model_bounds = m the model's box grown by m
         percent of its largest side, as applyModelBounds(m)): returns the history
         (rows: iteration, gravity misfit, magnetic misfit, regularization, total, gravity
         RMS, DW gravity x, y, DW magnetic x, y), the final parameters, densities and
-        susceptibilities, and the loop's summary and log."""
+        susceptibilities, and the loop's summary and log.  solver: "lbfgsb" or "gncg"
+        (Gauss-Newton CG with gncg_cg_max_iter, gncg_cg_tolerance and fd_step, the
+        reference's InversionConfig fields of those names)."""
+        if solver not in SOLVERS:
+            raise ValueError(f"solver must be one of {sorted(SOLVERS)}, not {solver!r}")
         o = _RunnerOptions(
             max_iterations=max_iterations, property_interval=property_interval,
             geometry=int(geometry), properties=int(properties),
@@ -539,7 +603,9 @@ class Problem:
             density_lower=density_bounds[0], density_upper=density_bounds[1],
             density_damping=density_damping, susceptibility_lower=susceptibility_bounds[0],
             susceptibility_upper=susceptibility_bounds[1],
-            susceptibility_damping=susceptibility_damping, property_tolerance=property_tolerance)
+            susceptibility_damping=susceptibility_damping, property_tolerance=property_tolerance,
+            solver=SOLVERS[solver], gncg_cg_max_iter=gncg_cg_max_iter,
+            gncg_cg_tolerance=gncg_cg_tolerance, fd_step=fd_step)
         cap = history_cap or 3 * max_iterations + 64
         hist = np.zeros((cap, 10))
         nh = ctypes.c_int64()
@@ -644,3 +710,13 @@ class Problem:
         o = _opts(max_iter, history, ftol, c1, target, h, active_set)
         return _finish(L, L.li_problem_minimize, len(x0), max_iter, self._h, _p(x0), _p(lb),
                        _p(ub), ctypes.byref(o), _p(h))
+
+    def gncg_minimize(self, x0, lb, ub, max_iter=20, cg_max_iter=50, cg_tol=1e-6, fd_step=2.0,
+                      ftol=1e-6, c1=1e-4, line_search_max_iter=30):
+        """GNCGOptimizer::minimize on this problem's objective and GPU gradient: every
+        inner CG iteration costs two gradients (forward + adjoint each)."""
+        L = load_host()
+        x0, lb, ub = _a(x0), _a(lb), _a(ub)
+        o = _gncg_opts(max_iter, cg_max_iter, cg_tol, fd_step, ftol, c1, line_search_max_iter)
+        return _gncg_finish(L.li_problem_gncg_minimize, len(x0), o, self._h, _p(x0), _p(lb),
+                            _p(ub))
