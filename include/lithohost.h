@@ -50,6 +50,13 @@ int li_model_fix_exterior(void* h, int interior_freedom);
 int li_model_fix_exterior_faces(void* h, int freedom);
 /* the current freedom of each vertex (n_v values) */
 int li_model_freedom(void* h, uint8_t* freedom);
+/* InversionExporter::exportClosedVolume over the units at the current geometry
+ * (exporters.cpp:769-813): each unit's closed surface -- its front facets as stored, its
+ * back facets reversed, its vertices in ascending shared index -- as the GOCAD TSurf
+ * <dir>/<base>_<name>.ts (header name <base>_<name>), name = names[g] for g < n_names, else
+ * "group_<g>"; units without facets are skipped; dir is created.  n_written may be NULL. */
+int li_model_export_units(void* h, const char* dir, const char* base, const char* const* names,
+                          int32_t n_names, int32_t* n_written);
 /* setVertexFreedom per vertex (n_v values, same codes, plus 4 ALONG_VECTOR: one
  * parameter, the vertex's projection on its move vector, surface_mesh.cpp:125-128,
  * 194-196) */
@@ -191,6 +198,14 @@ typedef struct {
     int32_t gncg_cg_max_iter;    /* inner CG iterations (runner.h:54, default 50) */
     double gncg_cg_tolerance;    /* CG residual tolerance (runner.h:55, default 1e-6) */
     double fd_step;              /* Hessian-vector difference step (runner.h:39, default 1) */
+    int32_t disable_line_search; /* L-BFGS-B takes the full projected step (runner.h:115-116) */
+    /* iterationExportDir / groupExportNames (runner.h:139-155, runner.cpp:319-348): when
+     * non-NULL and non-empty, every accepted geometry phase writes each unit's closed
+     * surface to <dir>/iter_NNN/invert_<name>.ts, name = group_export_names[g] for g below
+     * n_group_export_names, else "group_<g>" (li_model_export_units) */
+    const char* iteration_export_dir;
+    const char* const* group_export_names;
+    int32_t n_group_export_names;
 } LiRunnerOptions;
 /* hist rows of 10: iteration, gravity misfit, magnetic misfit, regularization, total,
  * gravity RMS, Durbin-Watson gravity x, y, magnetic x, y (InversionIteration,
@@ -232,6 +247,8 @@ typedef struct {
     double target;  /* stop when f <= target; NaN: none */
     int32_t has_h0; /* h0 holds a diagonal initial inverse Hessian */
     int32_t active_set;
+    int32_t disable_line_search; /* the full projected step, no objective calls in the
+                                    loop (lbfgsb_optimizer.cpp:90-96); target ignored */
 } LiOptions;
 /* hist rows of 7: iteration, f, |projected gradient|, step, seconds, objective calls,
  * gradient calls; msg: the stop reason */

@@ -196,6 +196,18 @@ int li_model_freedom(void* h, uint8_t* freedom)
     });
 }
 
+int li_model_export_units(void* h, const char* dir, const char* base, const char* const* names,
+                          int32_t n_names, int32_t* n_written)
+{
+    return guard([&] {
+        if (!dir || !*dir) throw std::invalid_argument("li_model_export_units: empty directory");
+        std::vector<std::string> nm;
+        for (int32_t i = 0; i < n_names; ++i) nm.emplace_back(names[i] ? names[i] : "");
+        const auto w = exportUnitVolumes(*((ModelHandle*)h)->m, dir, base ? base : "invert", nm);
+        if (n_written) *n_written = (int32_t)w.size();
+    });
+}
+
 int li_model_set_freedom(void* h, const uint8_t* freedom)
 {
     return guard([&] {
@@ -516,6 +528,10 @@ int li_problem_run(void* h, const LiRunnerOptions* o, void* label_grid, double* 
         ro.gncgCGMaxIter = o->gncg_cg_max_iter;
         ro.gncgCGTolerance = o->gncg_cg_tolerance;
         ro.fdStep = o->fd_step;
+        ro.disableLineSearch = o->disable_line_search != 0;
+        if (o->iteration_export_dir) ro.iterationExportDir = o->iteration_export_dir;
+        for (int32_t i = 0; i < o->n_group_export_names; ++i)
+            ro.groupExportNames.emplace_back(o->group_export_names[i] ? o->group_export_names[i] : "");
         ro.densityOptions = {o->density_damping, o->density_lower, o->density_upper,
                              o->property_max_iter, o->property_tolerance};
         ro.susceptibilityOptions = {o->susceptibility_damping, o->susceptibility_lower,
@@ -800,6 +816,7 @@ static int run_minimize(LBFGSBOptimizer::Objective obj, LBFGSBOptimizer::Gradien
         if (!std::isnan(o->target)) opt.setTargetObjective(o->target);
         if (o->has_h0 && h0) opt.setInitialHessianDiagonal(Vec(h0, h0 + n));
         opt.setActiveSetAware(o->active_set != 0);
+        opt.setDisableLineSearch(o->disable_line_search != 0);
         const auto r = opt.minimize(obj, grad, Vec(x0, x0 + n), Vec(lb, lb + n), Vec(ub, ub + n));
         std::memcpy(x_out, r.params.data(), sizeof(double) * n);
         const auto& H = opt.history();

@@ -1742,7 +1742,10 @@ OptimizerResult LBFGSBOptimizer::minimize(Objective objective, Gradient gradient
     auto fgrad = [&](const Vec& x) { ++nGrad; return gradient(x); };
     Vec x = project(x0, lb, ub);
     Vec g = fgrad(x);
-    double f = fobj(x);
+    // with the line search off the reference never evaluates f inside the loop
+    // (lbfgsb_optimizer.cpp:90-96): the records carry NaN until the final value
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    double f = m_noLineSearch ? nan : fobj(x);
     double pg0 = 0.0;
     int stall = 0;
     auto pgnorm = [&](const Vec& xx, const Vec& gg) {
@@ -1758,7 +1761,7 @@ OptimizerResult LBFGSBOptimizer::minimize(Objective objective, Gradient gradient
     if (m_onIteration) m_onIteration(m_hist.back());
     for (int k = 0; k < m_maxIter; ++k) {
         m_iter = k + 1;
-        if (m_hasTarget && f <= m_target) {
+        if (m_hasTarget && !m_noLineSearch && f <= m_target) {
             res.converged = true;
             res.message = "Reached target objective";
             break;
@@ -1784,23 +1787,32 @@ OptimizerResult LBFGSBOptimizer::minimize(Objective objective, Gradient gradient
         const double gd = dot(g, d);
         Vec xn, gn;
         double fn = f;
-        if (gd >= 0.0) {
+        bool accepted = false;
+        if (m_noLineSearch) {
+            // the full projected step, always (:90-96)
+            xn.resize(x.size());
+            for (size_t i = 0; i < x.size(); ++i) xn[i] = x[i] + d[i];
+            xn = project(xn, lb, ub);
+            fn = nan;
+            accepted = true;
+        } else if (gd >= 0.0) {
             res.converged = true;
             res.message = "Converged: no step taken";
             break;
         }
-        bool accepted = false;
-        for (int it = 0; it < m_lsMax && alpha >= 1e-16; ++it) {
+        for (int it = 0; !accepted && it < m_lsMax && alpha >= 1e-16; ++it) {
             Vec xt(x.size());
             for (size_t i = 0; i < x.size(); ++i) xt[i] = x[i] + alpha * d[i];
             xt = project(xt, lb, ub);
+            // the trial is evaluated before its projected decrease is checked, as
+            // there (:117-125), so the objective sees the same points
+            const double ft = fobj(xt);
             double dec = 0.0;
             for (size_t i = 0; i < x.size(); ++i) dec += g[i] * (xt[i] - x[i]);
             if (dec >= 0.0) {
                 alpha *= 0.5;
                 continue;
             }
-            const double ft = fobj(xt);
             if (ft <= f + m_c1 * dec) {
                 xn = xt;
                 fn = ft;
@@ -1820,7 +1832,8 @@ OptimizerResult LBFGSBOptimizer::minimize(Objective objective, Gradient gradient
         if (alpha < 1e-8) {
             if (++stall >= 3) {
                 res.converged = true;
-                res.message = "Converged: line search stalled";
+                res.message = "Converged: line search stalled (alpha < 1e-8 for " +
+                              std::to_string(stall) + " consecutive iterations)";
                 break;
             }
         } else {
@@ -1850,6 +1863,10 @@ OptimizerResult LBFGSBOptimizer::minimize(Objective objective, Gradient gradient
         if (m_onIteration) m_onIteration(m_hist.back());
     }
     if (res.message.empty()) res.message = "Maximum iterations reached";
+    if (m_noLineSearch) {
+        f = fobj(x);  // :255
+        m_hist.back().f = f;
+    }
     res.params = x;
     res.finalValue = f;
     res.iterations = m_iter;

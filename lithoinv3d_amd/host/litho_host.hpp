@@ -559,6 +559,10 @@ public:
     // two-loop recursion and the step (off by default: the reference projects the full
     // L-BFGS step, which can stop descending once bounds are active).
     void setActiveSetAware(bool on) { m_activeSet = on; }
+    // InversionConfig::disableLineSearch (lbfgsb_optimizer.cpp:90-96): every iteration
+    // takes the full projected step without evaluating the objective; f is evaluated once
+    // at the end (history records before that carry NaN; a target is not checked)
+    void setDisableLineSearch(bool on) { m_noLineSearch = on; }
     // called with each iteration's record as it is appended (record 0: the start)
     using IterationCallback = std::function<void(const IterationRecord&)>;
     void setIterationCallback(IterationCallback cb) { m_onIteration = std::move(cb); }
@@ -575,6 +579,7 @@ private:
     double m_ftol = 1e-6, m_c1 = 1e-4, m_target = 0.0;
     bool m_hasTarget = false;
     bool m_activeSet = false;
+    bool m_noLineSearch = false;
     std::vector<IterationRecord> m_hist;
 };
 

@@ -3,6 +3,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <filesystem>
 #include <numeric>
 #include <sstream>
 #include <stdexcept>
@@ -455,6 +457,7 @@ RunnerResult runInversion(std::shared_ptr<LithologyModel> model, RunnerProblem& 
     }
 
     int total = 0;
+    int exportCounter = 0;
     bool converged = false;
     while (total < opt.maxIterations && !converged &&
            (int)res.history.size() < 3 * opt.maxIterations) {
@@ -479,6 +482,7 @@ RunnerResult runInversion(std::shared_ptr<LithologyModel> model, RunnerProblem& 
                 lb.setTolerance(opt.tolerance);
                 lb.setArmijoC1(opt.armijoC1);
                 lb.setLineSearchMaxIter(opt.lineSearchMaxIter);
+                lb.setDisableLineSearch(opt.disableLineSearch);
                 geo = lb.minimize(geoObj, geoGrad, params, lower, upper);
             }
             log << "geometry phase"
@@ -518,6 +522,13 @@ RunnerResult runInversion(std::shared_ptr<LithologyModel> model, RunnerProblem& 
                 log << "all validation retries failed, reverting to the previous model\n";
             } else {
                 params = best;
+                // the model holds the accepted geometry (runner.cpp:571-575)
+                if (!opt.iterationExportDir.empty()) {
+                    char sub[24];
+                    std::snprintf(sub, sizeof(sub), "iter_%03d", exportCounter++);
+                    exportUnitVolumes(*model, opt.iterationExportDir + "/" + sub, "invert",
+                                      opt.groupExportNames);
+                }
             }
             if (geo.iterations == 0) {
                 converged = true;
@@ -555,6 +566,50 @@ RunnerResult runInversion(std::shared_ptr<LithologyModel> model, RunnerProblem& 
     }
     res.log = log.str();
     return res;
+}
+
+SurfaceMeshData unitClosedMesh(const LithologyModel& model, int unit)
+{
+    const auto& F = model.faces();
+    const auto& front = model.front();
+    const auto& back = model.back();
+    const auto& V = model.vertices();
+    std::vector<uint32_t> tris;
+    for (size_t f = 0; f < model.faceCount(); ++f) {
+        if (front[f] == unit) tris.insert(tris.end(), {F[3 * f], F[3 * f + 1], F[3 * f + 2]});
+    }
+    for (size_t f = 0; f < model.faceCount(); ++f) {
+        if (back[f] == unit) tris.insert(tris.end(), {F[3 * f], F[3 * f + 2], F[3 * f + 1]});
+    }
+    std::vector<uint32_t> used(tris);
+    std::sort(used.begin(), used.end());
+    used.erase(std::unique(used.begin(), used.end()), used.end());
+    SurfaceMeshData m;
+    m.vertices.reserve(3 * used.size());
+    for (uint32_t v : used) m.vertices.insert(m.vertices.end(), {V[3 * v], V[3 * v + 1], V[3 * v + 2]});
+    m.triangles.resize(tris.size());
+    for (size_t i = 0; i < tris.size(); ++i)
+        m.triangles[i] = (uint32_t)(std::lower_bound(used.begin(), used.end(), tris[i]) - used.begin());
+    return m;
+}
+
+std::vector<std::string> exportUnitVolumes(const LithologyModel& model, const std::string& dir,
+                                           const std::string& base,
+                                           const std::vector<std::string>& names)
+{
+    std::error_code ec;
+    std::filesystem::create_directories(dir, ec);
+    if (ec) throw std::runtime_error("exportUnitVolumes: cannot create " + dir + ": " + ec.message());
+    std::vector<std::string> written;
+    for (int g = 0; g < model.groupCount(); ++g) {
+        const SurfaceMeshData m = unitClosedMesh(model, g);
+        if (m.vertexCount() == 0) continue;
+        const std::string name = g < (int)names.size() ? names[g] : "group_" + std::to_string(g);
+        const std::string path = dir + "/" + base + "_" + name + ".ts";
+        writeTSurf(m, path, base + "_" + name);
+        written.push_back(path);
+    }
+    return written;
 }
 
 }  // namespace lithofwd

@@ -78,6 +78,18 @@ UnitNeighbours unitNeighbours(const LithologyModel& model);
 // unit's value, as the reference's loop over group meshes does.
 Vec smoothGradient(const LithologyModel& model, const UnitNeighbours& nb, const Vec& g, double w);
 
+// One unit's closed surface out of the shared facet set, as the reference holds it per
+// group mesh: the facets with the unit in front as stored, those with the unit behind
+// reversed (v0, v2, v1), and the vertices they use in ascending shared index.
+SurfaceMeshData unitClosedMesh(const LithologyModel& model, int unit);
+// InversionExporter::exportClosedVolume over the units (exporters.cpp:769-813, as the
+// runner's iteration export calls it, runner.cpp:319-348): <dir>/<base>_<name>.ts with
+// name = names[g] where the list has one, else "group_<g>"; units without facets are
+// skipped; dir is created.  Returns the files written.
+std::vector<std::string> exportUnitVolumes(const LithologyModel& model, const std::string& dir,
+                                           const std::string& base,
+                                           const std::vector<std::string>& names);
+
 // InversionConfig (runner.h:27-170), the fields the loop here uses.
 struct RunnerOptions {
     int maxIterations = 500;
@@ -104,6 +116,11 @@ struct RunnerOptions {
     int gncgCGMaxIter = 50;
     double gncgCGTolerance = 1e-6;
     double fdStep = 1.0;
+    bool disableLineSearch = false;  // L-BFGS-B: the full projected step (runner.h:115-116)
+    // iterationExportDir / groupExportNames (runner.h:139-155): after each accepted
+    // geometry phase every unit's closed surface goes to <dir>/iter_NNN/invert_<name>.ts
+    std::string iterationExportDir;
+    std::vector<std::string> groupExportNames;
 };
 
 // InversionIteration (inversion_result.h:10-28), gravity and magnetics.

@@ -29,7 +29,7 @@ class _Options(ctypes.Structure):
     _fields_ = [("max_iter", ctypes.c_int32), ("history", ctypes.c_int32),
                 ("ftol", ctypes.c_double), ("armijo_c1", ctypes.c_double),
                 ("target", ctypes.c_double), ("has_h0", ctypes.c_int32),
-                ("active_set", ctypes.c_int32)]
+                ("active_set", ctypes.c_int32), ("disable_line_search", ctypes.c_int32)]
 
 
 class _RunnerOptions(ctypes.Structure):
@@ -41,7 +41,10 @@ class _RunnerOptions(ctypes.Structure):
             "density_lower", "density_upper", "density_damping", "susceptibility_lower",
             "susceptibility_upper", "susceptibility_damping", "property_tolerance")] + [
         ("solver", ctypes.c_int32), ("gncg_cg_max_iter", ctypes.c_int32),
-        ("gncg_cg_tolerance", ctypes.c_double), ("fd_step", ctypes.c_double)]
+        ("gncg_cg_tolerance", ctypes.c_double), ("fd_step", ctypes.c_double),
+        ("disable_line_search", ctypes.c_int32), ("iteration_export_dir", ctypes.c_char_p),
+        ("group_export_names", ctypes.POINTER(ctypes.c_char_p)),
+        ("n_group_export_names", ctypes.c_int32)]
 
 
 class _GncgOptions(ctypes.Structure):
@@ -108,6 +111,9 @@ def load_host():
                                                   ctypes.c_double, ctypes.c_double]
     L.li_model_fix_exterior_faces.argtypes = [_P, ctypes.c_int]
     L.li_model_freedom.argtypes = [_P, _P]
+    L.li_model_export_units.argtypes = [_P, ctypes.c_char_p, ctypes.c_char_p,
+                                        ctypes.POINTER(ctypes.c_char_p), ctypes.c_int32,
+                                        ctypes.POINTER(ctypes.c_int32)]
     L.li_model_model_bounds.argtypes = [_P, ctypes.c_double, _P, _P]
     L.li_label_grid_create.argtypes = [_P, _P, _P, _P, ctypes.POINTER(_P)]
     L.li_label_grid_destroy.argtypes = [_P]
@@ -187,6 +193,18 @@ class Model:
         out = np.zeros(self.n_v, np.uint8)
         _chk(load_host().li_model_freedom(self._h, _p(out)), "freedom")
         return out
+
+    def export_units(self, directory, base="invert", names=None):
+        """Every unit's closed surface at the current geometry as GOCAD TSurf files,
+        <directory>/<base>_<name>.ts (InversionExporter::exportClosedVolume), name =
+        names[g] where given, else group_<g>; returns the number of files written."""
+        names = list(names or [])
+        arr = (ctypes.c_char_p * max(len(names), 1))(*[n.encode() for n in names])
+        n = ctypes.c_int32()
+        _chk(load_host().li_model_export_units(self._h, os.fspath(directory).encode(),
+                                               base.encode(), arr, len(names),
+                                               ctypes.byref(n)), "export_units")
+        return n.value
 
     def fix_exterior(self, interior=Z_ONLY):
         _chk(load_host().li_model_fix_exterior(self._h, interior), "fix_exterior")
@@ -399,9 +417,10 @@ class Result:
         return float(self.history[-1, 1]) if len(self.history) else math.nan
 
 
-def _opts(max_iter, history, ftol, c1, target, h0, active_set=False):
+def _opts(max_iter, history, ftol, c1, target, h0, active_set=False, disable_line_search=False):
     o = _Options(max_iter, history, ftol, c1, math.nan if target is None else target,
-                 1 if h0 is not None else 0, 1 if active_set else 0)
+                 1 if h0 is not None else 0, 1 if active_set else 0,
+                 1 if disable_line_search else 0)
     return o
 
 
@@ -459,8 +478,10 @@ def gncg_minimize(fun, grad, x0, lb, ub, max_iter=100, cg_max_iter=50, cg_tol=1e
 
 
 def minimize(fun, grad, x0, lb, ub, max_iter=100, history=10, ftol=1e-6, c1=1e-4,
-             target=None, h0=None, active_set=False):
-    """The host L-BFGS-B on Python callables (for tests of the optimizer itself)."""
+             target=None, h0=None, active_set=False, disable_line_search=False):
+    """The host L-BFGS-B on Python callables (for tests of the optimizer itself).
+    disable_line_search: the reference's disableLineSearch, every step the full projected
+    one and f evaluated only at the end (history rows before the last hold NaN)."""
     L = load_host()
     x0, lb, ub = _a(x0), _a(lb), _a(ub)
     n = len(x0)
@@ -474,7 +495,7 @@ def minimize(fun, grad, x0, lb, ub, max_iter=100, history=10, ftol=1e-6, c1=1e-4
 
     fcb, gcb = OBJ_CB(f_cb), GRAD_CB(g_cb)
     h = None if h0 is None else _a(h0)
-    o = _opts(max_iter, history, ftol, c1, target, h, active_set)
+    o = _opts(max_iter, history, ftol, c1, target, h, active_set, disable_line_search)
     return _finish(L, L.li_minimize_callbacks, n, max_iter, n, _p(x0), _p(lb), _p(ub), fcb, gcb,
                    None, ctypes.byref(o), _p(h))
 
@@ -483,17 +504,31 @@ class Problem:
     """Joint gravity + magnetic objective (JointObjective) over the GPU forward and
     adjoint for one station set: Phi = sum (w_g r_g)^2/2 + alpha sum (w_t r_t)^2/2
     (+ reference-model term, + surface smoothness, + borehole depth-to-contact misfit),
-    w = 1/std."""
+    w = 1/std, and w = 1 where std <= 0 (objective.cpp:26).  gravity_uncertainty /
+    magnetic_uncertainty > 0 replace every std <= 0 (or a missing std array) first, as the
+    runner's set-up does with InversionConfig's fields of those names (runner.cpp:88-101)."""
 
     def __init__(self, model: Model, stations, gz_obs, gz_std=None, tmi_obs=None, tmi_std=None,
                  inc=-60.0, dec=0.0, field=50000.0, alpha_mag=1.0, precision="fp64",
                  ref_weight=0.0, device=0, smooth_weight=0.0, boreholes=None,
-                 borehole_weight=1.0, borehole_radius=250.0):
+                 borehole_weight=1.0, borehole_radius=250.0, gravity_uncertainty=0.0,
+                 magnetic_uncertainty=0.0):
         L = load_host()
         self.model = model
         st = _a(stations).reshape(-1, 3)
         self.n_s = len(st)
-        arrs = [st, _a(gz_obs), None if gz_std is None else _a(gz_std),
+
+        def default_std(std, sigma):
+            if sigma <= 0.0:
+                return None if std is None else _a(std)
+            std = np.zeros(self.n_s) if std is None else _a(std).copy()
+            std[std <= 0.0] = sigma
+            return std
+
+        gz_std = default_std(gz_std, gravity_uncertainty)
+        if tmi_obs is not None:
+            tmi_std = default_std(tmi_std, magnetic_uncertainty)
+        arrs = [st, _a(gz_obs), gz_std,
                 None if tmi_obs is None else _a(tmi_obs), None if tmi_std is None else _a(tmi_std)]
         self._keep = arrs
         self._h = ctypes.c_void_p()
@@ -579,7 +614,8 @@ class Problem:
             density_bounds=(-1e300, 1e300), density_damping=0.0,
             susceptibility_bounds=(0.0, 1.0), susceptibility_damping=0.0,
             property_max_iter=50, property_tolerance=1e-6, history_cap=None,
-            solver="lbfgsb", gncg_cg_max_iter=50, gncg_cg_tolerance=1e-6, fd_step=1.0):
+            solver="lbfgsb", gncg_cg_max_iter=50, gncg_cg_tolerance=1e-6, fd_step=1.0,
+            disable_line_search=False, iteration_export_dir=None, group_export_names=None):
         """InversionRunner::run from the model's current parameters (bounds: the start
         -/+ bounds_margin metres, or with model_bounds = m the model's box grown by m
         percent of its largest side, as applyModelBounds(m)): returns the history
@@ -587,7 +623,9 @@ class Problem:
         RMS, DW gravity x, y, DW magnetic x, y), the final parameters, densities and
         susceptibilities, and the loop's summary and log.  solver: "lbfgsb" or "gncg"
         (Gauss-Newton CG with gncg_cg_max_iter, gncg_cg_tolerance and fd_step, the
-        reference's InversionConfig fields of those names)."""
+        reference's InversionConfig fields of those names).  iteration_export_dir: after
+        each accepted geometry phase every unit's closed surface is written to
+        <dir>/iter_NNN/invert_<name>.ts (name from group_export_names, else group_<g>)."""
         if solver not in SOLVERS:
             raise ValueError(f"solver must be one of {sorted(SOLVERS)}, not {solver!r}")
         o = _RunnerOptions(
@@ -605,7 +643,14 @@ class Problem:
             susceptibility_upper=susceptibility_bounds[1],
             susceptibility_damping=susceptibility_damping, property_tolerance=property_tolerance,
             solver=SOLVERS[solver], gncg_cg_max_iter=gncg_cg_max_iter,
-            gncg_cg_tolerance=gncg_cg_tolerance, fd_step=fd_step)
+            gncg_cg_tolerance=gncg_cg_tolerance, fd_step=fd_step,
+            disable_line_search=int(disable_line_search))
+        names = [n.encode() for n in (group_export_names or [])]
+        name_arr = (ctypes.c_char_p * max(len(names), 1))(*names)
+        if iteration_export_dir:
+            o.iteration_export_dir = os.fspath(iteration_export_dir).encode()
+        o.group_export_names = ctypes.cast(name_arr, ctypes.POINTER(ctypes.c_char_p))
+        o.n_group_export_names = len(names)
         cap = history_cap or 3 * max_iterations + 64
         hist = np.zeros((cap, 10))
         nh = ctypes.c_int64()
@@ -703,11 +748,11 @@ class Problem:
         _chk(L.li_problem_set_iteration_callback(self._h, self._iter_cb, None), "callback")
 
     def minimize(self, x0, lb, ub, max_iter=50, history=10, ftol=1e-6, c1=1e-4, target=None,
-                 h0=None, active_set=False):
+                 h0=None, active_set=False, disable_line_search=False):
         L = load_host()
         x0, lb, ub = _a(x0), _a(lb), _a(ub)
         h = None if h0 is None else _a(h0)
-        o = _opts(max_iter, history, ftol, c1, target, h, active_set)
+        o = _opts(max_iter, history, ftol, c1, target, h, active_set, disable_line_search)
         return _finish(L, L.li_problem_minimize, len(x0), max_iter, self._h, _p(x0), _p(lb),
                        _p(ub), ctypes.byref(o), _p(h))
 

@@ -467,3 +467,33 @@ def test_threaded_parameter_map_and_pullback(mode):
     g_map = np.array([np.sum(w[off[d]:off[d + 1]] * vg[vert[off[d]:off[d + 1]], axis[d]])
                       for d in range(n)])
     np.testing.assert_allclose(g, g_map, rtol=1e-12, atol=1e-12 * np.abs(g_map).max())
+
+
+def test_disabled_line_search_full_steps():
+    """InversionConfig::disableLineSearch (lbfgsb_optimizer.cpp:90-96): f = x^2/4 from 4.
+    Step 1 is -g = -2 (x = 2, g = 1); then s = -2, y = -1, gamma = s.y/y.y = 2 and the
+    two-loop gives d = -2: x = 0 exactly, where the projected gradient is 0.  The
+    objective is evaluated once, for the final value; every gradient at an iterate."""
+    calls = {"f": 0, "g": 0}
+
+    def f(x):
+        calls["f"] += 1
+        return 0.25 * float(x @ x)
+
+    def g(x):
+        calls["g"] += 1
+        return 0.5 * x
+
+    r = inversion.minimize(f, g, np.array([4.0]), np.array([-10.0]), np.array([10.0]),
+                           max_iter=10, disable_line_search=True)
+    assert r.x[0] == 0.0 and r.iterations == 3
+    assert r.message == "Converged: projected gradient below tolerance"
+    assert (calls["f"], calls["g"]) == (1, 3)
+    assert np.isnan(r.history[:-1, 1]).all() and r.history[-1, 1] == 0.0
+    np.testing.assert_array_equal(r.history[:, 3], [0.0, 1.0, 1.0])
+    # with the line search on, the same problem takes the same steps (both are accepted
+    # by the Armijo test) and evaluates f at the start and at each trial
+    calls.update(f=0, g=0)
+    r2 = inversion.minimize(f, g, np.array([4.0]), np.array([-10.0]), np.array([10.0]),
+                            max_iter=10)
+    assert r2.x[0] == 0.0 and r2.iterations == 3 and calls["f"] == 3
