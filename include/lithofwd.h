@@ -120,13 +120,28 @@ int lf_colnorm2(lf_ctx* ctx, int axis, int64_t n_dof, const int64_t* dof_off,
                 const int32_t* dof_vertex, const double* dof_weight, double* out_gz,
                 double* out_tmi);
 
+/* Jacobian-vector product along a vertex tangent: gz[i] = sum_v dgz_i/dx_v . vtan_v and
+ * tmi[i] likewise (vtan: 3*n_vertices doubles, xyz per vertex), at the context's current
+ * geometry and properties, in one pass over the station x facet pairs: the matrix-free
+ * counterpart of J v with the reference's Jacobian (computeGravityAnalyticJacobian /
+ * computeMagneticAnalyticJacobian, gravity_forward.cpp:184-292,
+ * magnetic_forward.cpp:196-306, before the vertex -> DOF mapping), which its magnetic
+ * weight calibration multiplies densely (joint_objective.cpp:281-393).  Same arithmetic
+ * per pair as lf_adjoint (the pair's seeds for a unit gz / TMI weight), contracted with
+ * the facet constants' tangents; FP64 sums over facets per station.  Either output may
+ * be NULL. */
+int lf_jvp(lf_ctx* ctx, const double* vtan, double* gz, double* tmi);
+/* Device-pointer variant (3*n_vertices tangent doubles, n_stations outputs each). */
+int lf_jvp_device(lf_ctx* ctx, const double* d_vtan, double* d_gz, double* d_tmi);
+
 /* Waits for the context stream. */
 int lf_sync(lf_ctx* ctx);
 
 /* Per-kernel HIP-event timing on the context stream.  When enabled, each launch is
  * bracketed by events; lf_timing() returns, for kernel class k (0 = face setup,
  * 1 = forward, 2 = forward reduce, 3 = adjoint pairs, 4 = adjoint finalize,
- * 5 = vertex gather), the summed milliseconds and launch count since the last reset. */
+ * 5 = vertex gather, 6 = Jacobian-vector product), the summed milliseconds and launch
+ * count since the last reset. */
 int lf_set_timing(lf_ctx* ctx, int enable);
 int lf_timing(lf_ctx* ctx, double* ms, int64_t* launches, int n_classes);
 

@@ -84,6 +84,10 @@ int li_model_params(void* h, double* out);
 int li_model_apply(void* h, const double* params, double* vertices_out);
 /* vertex gradient (3*n_v) -> parameter gradient, the transpose of li_model_apply's map */
 int li_model_pullback(void* h, const double* vgrad, double* out);
+/* parameter tangent -> vertex tangent (3*n_v): the linear part of li_model_apply's map,
+ * whose transpose li_model_pullback is (no counterpart in the reference, which forms
+ * Jacobian columns by moving one parameter at a time) */
+int li_model_tangent(void* h, const double* dparams, double* out);
 /* that map as CSR per parameter; call with vert and w NULL for the entry count first */
 int li_model_dof_map(void* h, int64_t* off, int32_t* vert, double* w, int64_t* n_entries);
 /* geometry bounds params -/+ margin/axisScale (runner.cpp:241-248) */
@@ -139,9 +143,18 @@ int li_problem_set_remanence_bounds(void* h, double magnitude_lower, double magn
 /* JointObjective::calibrateDataWeights (joint_objective.cpp:281-393), matrix-free: sets
  * alpha_mag from the ratio of the largest eigenvalues of J^T W^2 J of gravity and of
  * magnetics at params.  out[7]: both eigenvalues, their ratio, the new alpha_mag, the two
- * power-iteration counts, 1 if the gradient-norm fallback ran.  eps: central-difference
- * step in parameter units; seed: the power iterations' start vectors. */
+ * power-iteration counts, 1 if the gradient-norm fallback ran.  eps <= 0: J v exactly,
+ * one Jacobian-vector product pass (lf_jvp) in the problem's precision; eps > 0: J v by
+ * central differences of two FP64 forwards with that step in parameter units (and J^T u
+ * in FP64).  seed: the power iterations' start vectors.  li_problem_run calibrates with
+ * eps = 0 and seed 1. */
 int li_problem_calibrate(void* h, const double* params, uint64_t seed, double eps, double* out);
+/* J dp at params for a problem with magnetic data: gz_out and tmi_out (n_stations each)
+ * the change of the data along the parameter tangent dp, one lf_jvp pass (the product the
+ * reference's calibrateDataWeights forms with its dense Jacobians, joint_objective.cpp:
+ * 281-393) */
+int li_problem_jvp(void* h, const double* params, const double* dparams, double* gz_out,
+                   double* tmi_out);
 /* 1 (default with magnetic data): one GPU context for gz and TMI, one adjoint pass and a
  * cached forward; 0: the reference-shaped JointObjective over separate forwards */
 int li_problem_set_fused(void* h, int on);

@@ -679,6 +679,86 @@ __global__ __launch_bounds__(256) void k_station_facet_grad(
     }
 }
 
+// ------------------------------------------------- Jacobian-vector products
+
+// J v along a vertex tangent dX (3 per vertex): the facet constants of the records move
+// by dq[m] = sum_k T[m][k] dX_k (T as in the finalize, columns in the facet's own corner
+// order), and a pair's contribution to gz_s moves by sum_m seeds_g[m] dq[m], where
+// seeds_g are the pair's seeds for a unit weight on gz (likewise TMI).
+constexpr int kSeedsPad = 20;
+
+template <class T>
+__global__ __launch_bounds__(128) void k_facet_tangent(
+    int64_t n_f, const double* __restrict__ verts, const uint32_t* __restrict__ tri,
+    const int32_t* __restrict__ front, const int32_t* __restrict__ back,
+    const double* __restrict__ density, const double* __restrict__ mag, double fx, double fy,
+    double fz, int tmi_mode, const double* __restrict__ vtan, T* __restrict__ dq)
+{
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n_f) return;
+    double P[3][3], drho, Md[3], Ms[3];
+    load_facet(f, verts, tri, front, back, density, mag, P, drho, Md, Ms);
+    double dX[9];
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t vi = tri[3 * f + k];
+        for (int j = 0; j < 3; ++j) dX[3 * k + j] = vtan[3 * (int64_t)vi + j];
+    }
+    const double Fd[3] = {fx, fy, fz};
+    double acc[kSeeds];
+    for (int m = 0; m < kSeeds; ++m) acc[m] = 0.0;
+    const bool ok = facet_T_columns(P, drho, Md, Ms, Fd, tmi_mode == 1,
+                                    [&](int k, const double* col) {
+                                        for (int m = 0; m < kSeeds; ++m) acc[m] += col[m] * dX[k];
+                                    });
+    T* out = dq + f * kSeedsPad;
+    for (int m = 0; m < kSeeds; ++m) out[m] = ok ? (T)acc[m] : T(0);
+    out[kSeeds] = T(0);
+}
+
+// lane = station, facets of the block's chunk wave-uniform: per pair the seeds of a unit
+// gz weight and of a unit TMI weight, each contracted with the facet's dq in T and added
+// into the lane's FP64 sums (a closed surface's facet terms cancel against each other);
+// one partial per (facet chunk, station), summed by k_reduce_chunks.
+template <class T>
+__global__ __launch_bounds__(256) void k_jvp_pairs(
+    const FaceFrame<T>* __restrict__ faces, const T* __restrict__ dq, int64_t n_f,
+    int64_t faces_per_chunk, const double* __restrict__ sx, const double* __restrict__ sy,
+    const double* __restrict__ sz, int64_t n_s, double* __restrict__ part_gz,
+    double* __restrict__ part_tmi)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int chunk = blockIdx.y;
+    const bool active = s < n_s;
+    const int64_t sc = active ? s : n_s - 1;
+    const double ox = sx[sc], oy = sy[sc], oz = sz[sc];
+    const int64_t f0 = (int64_t)chunk * faces_per_chunk;
+    const int64_t f1 = min(n_f, f0 + faces_per_chunk);
+    double g = 0.0, t = 0.0;
+    for (int64_t f = f0; f < f1; ++f) {
+        const FaceFrame<T> F = faces[f];
+        const T* __restrict__ q = dq + f * kSeedsPad;
+        // one weight at a time, not unrolled: the two seed sets are never live together
+#pragma unroll 1
+        for (int w = 0; w < 2; ++w) {
+            T a[kSeeds];
+#pragma unroll
+            for (int m = 0; m < kSeeds; ++m) a[m] = T(0);
+            pair_seeds(F, ox, oy, oz, T(w == 0 ? 1 : 0), T(w == 0 ? 0 : 1), a);
+            T p = T(0);
+#pragma unroll
+            for (int m = 0; m < kSeeds; ++m) p = fma(a[m], q[m], p);
+            if (w == 0)
+                g += (double)p;
+            else
+                t += (double)p;
+        }
+    }
+    if (active) {
+        part_gz[(int64_t)chunk * n_s + s] = g;
+        part_tmi[(int64_t)chunk * n_s + s] = t;
+    }
+}
+
 // D[s][v] = sum of G[s][corner entries of v]
 __global__ __launch_bounds__(256) void k_gather_block(const float* __restrict__ G,
                                                       const int64_t* __restrict__ off,

@@ -499,6 +499,10 @@ struct lf_ctx {
         double *dw = nullptr, *og = nullptr, *ot = nullptr;
         int64_t cap_dof = -1, cap_ent = -1;
     } cn;
+    // Jacobian-vector products (lf_jvp): the facets' constant tangents and a vertex
+    // tangent copied from the host, allocated on the first call
+    void* d_dq = nullptr;
+    double* d_vtan = nullptr;
     // timing
     bool timing = false;
     struct Ev { int cls; hipEvent_t a, b; };
@@ -684,6 +688,8 @@ void lf_destroy(lf_ctx* c)
                    c->cn.doff, c->cn.dvert, c->cn.dw, c->cn.og, c->cn.ot};
     for (void* p : cnb)
         if (p) hipFree(p);
+    if (c->d_dq) hipFree(c->d_dq);
+    if (c->d_vtan) hipFree(c->d_vtan);
     adjoint_free(c->adj);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -1037,6 +1043,81 @@ int lf_colnorm2(lf_ctx* c, int axis, int64_t n_dof, const int64_t* dof_off,
     }
     CN(hipStreamSynchronize(c->stream), "sync");
 #undef CN
+    return LF_OK;
+}
+
+int lf_jvp_device(lf_ctx* c, const double* d_vtan, double* d_gz, double* d_tmi)
+{
+    if (check_ctx(c)) return LF_EINVAL;
+    if (!d_vtan && c->n_v) return set_err(c, "lf_jvp_device: null tangent", LF_EINVAL);
+    LF_TRY(c, hipSetDevice(c->device));
+    int rc = ensure_faces(c);
+    if (rc) return rc;
+    if (c->n_s == 0 || (!d_gz && !d_tmi)) return LF_OK;
+    const unsigned sblocks = (unsigned)((c->n_s + 255) / 256);
+    if (c->n_f == 0) {
+        if (d_gz) hipLaunchKernelGGL(k_fill_zero, dim3(sblocks), dim3(256), 0, c->stream, d_gz, c->n_s);
+        if (d_tmi) hipLaunchKernelGGL(k_fill_zero, dim3(sblocks), dim3(256), 0, c->stream, d_tmi, c->n_s);
+        LF_TRY(c, hipGetLastError());
+        return LF_OK;
+    }
+    const bool f32 = c->precision == LF_FP32C;
+    if (!c->d_dq)
+        LF_TRY(c, hipMalloc(&c->d_dq, (f32 ? sizeof(float) : sizeof(double)) * kSeedsPad * c->n_f));
+    {
+        TimedLaunch tl(c, 6);
+        const unsigned fb = (unsigned)((c->n_f + 127) / 128);
+        const dim3 grid(sblocks, (unsigned)c->n_chunks);
+        if (f32) {
+            hipLaunchKernelGGL(k_facet_tangent<float>, dim3(fb), dim3(128), 0, c->stream, c->n_f,
+                               c->d_verts, c->d_tri, c->d_front, c->d_back, c->d_density, c->d_mag,
+                               c->fdir[0], c->fdir[1], c->fdir[2], c->tmi_mode, d_vtan,
+                               (float*)c->d_dq);
+            hipLaunchKernelGGL(k_jvp_pairs<float>, grid, dim3(256), 0, c->stream, c->d_f32,
+                               (const float*)c->d_dq, c->n_f, c->faces_per_chunk, c->d_sx, c->d_sy,
+                               c->d_sz, c->n_s, c->d_part_gz, c->d_part_tmi);
+        } else {
+            hipLaunchKernelGGL(k_facet_tangent<double>, dim3(fb), dim3(128), 0, c->stream, c->n_f,
+                               c->d_verts, c->d_tri, c->d_front, c->d_back, c->d_density, c->d_mag,
+                               c->fdir[0], c->fdir[1], c->fdir[2], c->tmi_mode, d_vtan,
+                               (double*)c->d_dq);
+            hipLaunchKernelGGL(k_jvp_pairs<double>, grid, dim3(256), 0, c->stream, c->d_fr64,
+                               (const double*)c->d_dq, c->n_f, c->faces_per_chunk, c->d_sx,
+                               c->d_sy, c->d_sz, c->n_s, c->d_part_gz, c->d_part_tmi);
+        }
+        LF_TRY(c, hipGetLastError());
+    }
+    {
+        TimedLaunch tl(c, 2);
+        if (d_gz)
+            hipLaunchKernelGGL(k_reduce_chunks, dim3(sblocks), dim3(256), 0, c->stream,
+                               c->d_part_gz, c->n_chunks, c->n_s, d_gz);
+        if (d_tmi)
+            hipLaunchKernelGGL(k_reduce_chunks, dim3(sblocks), dim3(256), 0, c->stream,
+                               c->d_part_tmi, c->n_chunks, c->n_s, d_tmi);
+        LF_TRY(c, hipGetLastError());
+    }
+    return LF_OK;
+}
+
+int lf_jvp(lf_ctx* c, const double* vtan, double* gz, double* tmi)
+{
+    if (check_ctx(c)) return LF_EINVAL;
+    if (!vtan && c->n_v) return set_err(c, "lf_jvp: null tangent", LF_EINVAL);
+    LF_TRY(c, hipSetDevice(c->device));
+    if (!c->d_vtan && c->n_v) LF_TRY(c, hipMalloc((void**)&c->d_vtan, sizeof(double) * 3 * c->n_v));
+    if (c->n_v)
+        LF_TRY(c, hipMemcpyAsync(c->d_vtan, vtan, sizeof(double) * 3 * c->n_v,
+                                 hipMemcpyHostToDevice, c->stream));
+    int rc = lf_jvp_device(c, c->d_vtan, gz ? c->d_out_gz : nullptr, tmi ? c->d_out_tmi : nullptr);
+    if (rc) return rc;
+    if (gz && c->n_s)
+        LF_TRY(c, hipMemcpyAsync(gz, c->d_out_gz, sizeof(double) * c->n_s, hipMemcpyDeviceToHost,
+                                 c->stream));
+    if (tmi && c->n_s)
+        LF_TRY(c, hipMemcpyAsync(tmi, c->d_out_tmi, sizeof(double) * c->n_s,
+                                 hipMemcpyDeviceToHost, c->stream));
+    LF_TRY(c, hipStreamSynchronize(c->stream));
     return LF_OK;
 }
 

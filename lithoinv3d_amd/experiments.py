@@ -20,9 +20,13 @@
   * ``compare_gncg``      the runner's two geometry solvers, L-BFGS-B and Gauss-Newton CG,
                           from the same start on one model: objective against seconds and
                           gradient evaluations.
+  * ``jvp_timing``        one Jacobian-vector product (FP32C, FP64) on a named workload
+                          against two FP64 forwards; ``calibration_timing``: the
+                          magnetic-weight calibration with each J v.
 
 CLI:  python -m lithoinv3d_amd.experiments learned-h0 --train 200 --test 20 --out FILE
       python -m lithoinv3d_amd.experiments solvers --seed 2000 --out FILE
+      python -m lithoinv3d_amd.experiments jvp --out FILE
       python -m lithoinv3d_amd.experiments large --max-iter 3
 """
 from __future__ import annotations
@@ -273,6 +277,63 @@ def compare_gncg(prob, lbfgs_iter=100, gncg_iter=8, cg_max_iter=50, fd_step=1.0,
     return out
 
 
+def jvp_timing(name="contact192x192x64_st256", reps=2, seed=1):
+    """One Jacobian-vector product along a random vertex tangent on a named workload, FP32C
+    and FP64, against the two FP64 forwards a central difference needs: seconds per call,
+    the per-kernel milliseconds (HIP events) and the FP32C product's deviation from the
+    FP64 one."""
+    cfg = workload.build(name)
+    m, st = cfg["model"], cfg["stations"]
+    fh = synth.field_unit_vector(cfg["inc"], cfg["dec"])
+    vt = np.random.default_rng(seed).normal(size=(m.n_vertices, 3))
+    out = {"workload": name, "n_vertices": m.n_vertices, "n_facets": m.n_faces,
+           "n_stations": len(st), "pairs": m.n_faces * len(st)}
+    res = {}
+    for prec in ("fp32c", "fp64"):
+        with ForwardContext(m, st, fh, cfg["field"], precision=prec) as ctx:
+            ctx.forward()
+            ctx.sync()
+            ctx.timing()
+            ctx.set_timing(True)
+            n = reps if prec == "fp32c" else 1
+            t0 = time.perf_counter()
+            for _ in range(n):
+                res[prec] = ctx.jvp(vt)
+            t1 = time.perf_counter()
+            for _ in range(2):
+                ctx.forward()
+            ctx.sync()
+            t2 = time.perf_counter()
+            ctx.set_timing(False)
+            tim = {k: v for k, v in ctx.timing().items() if v[1]}
+        out[prec] = dict(jvp_seconds=(t1 - t0) / n, two_forwards_seconds=t2 - t1,
+                         kernel_ms=tim)
+        log(f"{prec}: J v {(t1 - t0) / n:.3f} s, two forwards {t2 - t1:.3f} s, kernels {tim}")
+    for i, k in enumerate(("gz", "tmi")):
+        a, b = res["fp32c"][i], res["fp64"][i]
+        out[f"fp32c_vs_fp64_{k}"] = float(np.abs(a - b).max() / np.abs(b).max())
+    return out
+
+
+def calibration_timing(prob):
+    """Problem.calibrate at the start of one contact problem: the exact Jacobian-vector
+    product in FP32C and FP64 against the central differences of FP64 forwards."""
+    synth_data(prob)
+    out = {"seed": prob["seed"], "n_dof": prob["n_dof"]}
+    for prec, eps in (("fp32c", 0.0), ("fp64", 0.0), ("fp64", 1e-3)):
+        P = make_problem(prob, prec)
+        P.gradient(prob["p0"])
+        t0 = time.perf_counter()
+        r = P.calibrate(prob["p0"], seed=1, eps=eps)
+        sec = time.perf_counter() - t0
+        P.close()
+        key = f"{prec}_{'exact' if eps <= 0 else 'central_differences'}"
+        out[key] = dict(seconds=sec, **{k: (float(v) if not isinstance(v, bool) else v)
+                                        for k, v in r.items()})
+        log(f"calibration {key}: {sec:.2f} s, {r}")
+    return out
+
+
 def borehole_picks(m, M, ptrue, n=50, seed=5, sigma=5.0, step=5.0, margin=100.0):
     """Depth-to-contact picks of n seeded vertical boreholes at the true geometry: along
     each vertical line the unit is sampled every `step` metres (point classification);
@@ -509,6 +570,11 @@ def main(argv=None):
     d.add_argument("--jsonl", required=True)
     d.add_argument("--net", required=True, help="the network file of the runs (.npz)")
     d.add_argument("--out", default=None)
+    jv = sub.add_parser("jvp", help="Jacobian-vector product timing and calibration cost")
+    jv.add_argument("--workload", default="contact192x192x64_st256")
+    jv.add_argument("--calibration-seed", type=int, default=2000,
+                    help="the 64x64x32 contact problem calibrated (negative: skip)")
+    jv.add_argument("--out", default=None)
     sv = sub.add_parser("solvers", help="L-BFGS-B against Gauss-Newton CG on one model")
     sv.add_argument("--seed", type=int, default=2000)
     sv.add_argument("--shape", type=int, nargs=3, default=(64, 64, 32))
@@ -535,6 +601,14 @@ def main(argv=None):
         res = large_inversion(args.workload, args.stride, args.max_iter,
                               n_boreholes=args.boreholes, h0=args.h0,
                               train_models=args.train_models, target=args.target)
+        if args.out:
+            json.dump(res, open(args.out, "w"), indent=1)
+        print(json.dumps(res))
+        return
+    if args.cmd == "jvp":
+        res = {"jvp": jvp_timing(args.workload)}
+        if args.calibration_seed >= 0:
+            res["calibration"] = calibration_timing(contact_problem(args.calibration_seed))
         if args.out:
             json.dump(res, open(args.out, "w"), indent=1)
         print(json.dumps(res))

@@ -42,6 +42,7 @@ struct Problem {
     bool useFused = true;
     double inc = 0.0, dec = 0.0, field = 0.0;
     int device = 0;
+    Precision prec = Precision::FP64;
     // magnetic property phase: remanence mode and the bounds of its parameters
     // (runner.h:98-107)
     RemanentMagnetizationMode remMode = RemanentMagnetizationMode::EffectiveSusceptibility;
@@ -268,6 +269,15 @@ int li_model_pullback(void* h, const double* vgrad, double* out)
     });
 }
 
+int li_model_tangent(void* h, const double* dparams, double* out)
+{
+    return guard([&] {
+        auto& m = *((ModelHandle*)h)->m;
+        const Vec t = m.vertexTangent(Vec(dparams, dparams + m.totalDofCount()));
+        std::memcpy(out, t.data(), sizeof(double) * t.size());
+    });
+}
+
 // parameter -> vertex map as CSR; call with vert/w NULL to get the entry count first
 int li_model_dof_map(void* h, int64_t* off, int32_t* vert, double* w, int64_t* n_entries)
 {
@@ -347,6 +357,7 @@ int li_problem_create(void** out, void* model, int64_t n_s, const double* statio
         P->dec = dec;
         P->field = field;
         P->device = device;
+        P->prec = prec;
         P->grav = std::make_shared<GravityForward>(P->model, P->gdata, prec, device);
         P->gobj = std::make_shared<ObjectiveFunction>(P->grav, P->gdata);
         if (ref_weight > 0.0) {
@@ -464,13 +475,27 @@ int li_problem_calibrate(void* h, const double* params, uint64_t seed, double ep
         const double a0 = P->joint->alphaMag();
         const CalibrationResult r = calibrateMagneticWeight(P->model, P->gdata, P->mdata, P->inc,
                                                             P->dec, P->field, p, a0, seed, eps,
-                                                            P->device, P->remMode);
+                                                            P->device, P->remMode, P->prec);
         P->joint->setAlphaMag(r.alpha);
         if (P->fused) P->fused->setAlphaMag(r.alpha);
         const double o[7] = {r.eigGravity, r.eigMagnetic, r.ratio, r.alpha,
                              (double)r.iterationsGravity, (double)r.iterationsMagnetic,
                              r.fallback ? 1.0 : 0.0};
         std::memcpy(out, o, sizeof(o));
+    });
+}
+
+int li_problem_jvp(void* h, const double* params, const double* dparams, double* gz_out,
+                   double* tmi_out)
+{
+    return guard([&] {
+        auto* P = (Problem*)h;
+        if (!P->fused) throw std::runtime_error("li_problem_jvp: problem has no magnetic data");
+        const size_t n = P->model->totalDofCount();
+        Vec gz, tmi;
+        P->fused->jvp(Vec(params, params + n), Vec(dparams, dparams + n), gz, tmi);
+        if (gz_out) std::memcpy(gz_out, gz.data(), sizeof(double) * gz.size());
+        if (tmi_out) std::memcpy(tmi_out, tmi.data(), sizeof(double) * tmi.size());
     });
 }
 
@@ -565,7 +590,7 @@ int li_problem_run(void* h, const LiRunnerOptions* o, void* label_grid, double* 
         rp.calibrate = [P](const Vec& p) {
             const CalibrationResult r = calibrateMagneticWeight(
                 P->model, P->gdata, P->mdata, P->inc, P->dec, P->field, p, P->joint->alphaMag(),
-                1, 1e-3, P->device, P->remMode);
+                1, 0.0, P->device, P->remMode, P->prec);
             P->joint->setAlphaMag(r.alpha);
             if (P->fused) P->fused->setAlphaMag(r.alpha);
             return r.alpha;

@@ -562,6 +562,37 @@ void LithologyModel::applyParameterVector(const Vec& params)
     }
 }
 
+Vec LithologyModel::vertexTangent(const Vec& dp) const
+{
+    if (dp.size() != m_dofVertex.size())
+        throw std::invalid_argument("vertexTangent: wrong parameter count");
+    Vec t(3 * vertexCount(), 0.0);
+    for (size_t d = 0; d < dp.size(); ++d) {
+        double* x = &t[3 * (size_t)m_dofVertex[d]];
+        if (dofAlongVector(d)) {
+            const double* u = &m_moveVector[3 * (size_t)m_dofVertex[d]];
+            for (int a = 0; a < 3; ++a) x[a] += dp[d] * u[a];
+        } else {
+            x[m_dofAxis[d]] = dp[d] * m_axisScale[m_dofAxis[d]];
+        }
+    }
+    if (m_cpStride <= 0) return t;
+    const int64_t nv = (int64_t)vertexCount();
+#pragma omp parallel for schedule(static) if (nv >= kParallelVertices)
+    for (int64_t vi = 0; vi < nv; ++vi) {
+        const size_t v = (size_t)vi;
+        if (m_isCP[v]) continue;
+        double d3[3] = {0.0, 0.0, 0.0};
+        for (int64_t e = m_wOff[v]; e < m_wOff[v + 1]; ++e) {
+            const size_t c = (size_t)m_wCp[e];
+            for (int a = 0; a < 3; ++a) d3[a] += m_w[e] * t[3 * c + a];
+        }
+        if (m_cpInterp == ControlPointInterpolation::Displacement) projectFree(v, d3);
+        for (int a = 0; a < 3; ++a) t[3 * v + a] = d3[a];
+    }
+    return t;
+}
+
 Vec LithologyModel::pullBackVertexGradient(const Vec& vg) const
 {
     // dPhi/dx_cp = dPhi/dx_cp (direct) + sum over its dependants w * dPhi/dx_v (projected on
@@ -1403,17 +1434,30 @@ Vec FusedJointObjective::jtw(const Vec& params, const Vec& wg, const Vec& wt)
     return m_model->pullBackVertexGradient(vg);
 }
 
+void FusedJointObjective::jvp(const Vec& params, const Vec& dp, Vec& gz, Vec& tmi)
+{
+    m_model->applyParameterVector(params);
+    check(lf_set_vertices(m_ctx, m_model->vertices().data()), m_ctx, "lf_set_vertices");
+    m_cacheValid = false;
+    const Vec vt = m_model->vertexTangent(dp);
+    gz.assign(m_g.size(), 0.0);
+    tmi.assign(m_m.size(), 0.0);
+    check(lf_jvp(m_ctx, vt.data(), gz.data(), tmi.data()), m_ctx, "lf_jvp");
+}
+
 CalibrationResult calibrateMagneticWeight(std::shared_ptr<LithologyModel> model,
                                           const GravityData& gdata, const MagneticData& mdata,
                                           double inc_deg, double dec_deg, double field_nT,
                                           const Vec& params, double alpha_mag, uint64_t seed,
-                                          double eps, int device, RemanentMagnetizationMode mode)
+                                          double eps, int device, RemanentMagnetizationMode mode,
+                                          Precision prec)
 {
     CalibrationResult out;
     out.alpha = alpha_mag;
     if (mdata.empty() || alpha_mag <= 0.0) return out;
+    const bool exact = !(eps > 0.0);
     FusedJointObjective f(model, gdata, mdata, inc_deg, dec_deg, field_nT, alpha_mag,
-                          Precision::FP64, device);
+                          exact ? prec : Precision::FP64, device);
     if (mode != RemanentMagnetizationMode::EffectiveSusceptibility) f.setRemanenceMode(mode);
     const size_t np = params.size(), ns = gdata.size();
     Vec wgt_g(ns), wgt_m(ns);
@@ -1421,8 +1465,15 @@ CalibrationResult calibrateMagneticWeight(std::shared_ptr<LithologyModel> model,
         wgt_g[i] = gdata[i].g_std > 0 ? 1.0 / (gdata[i].g_std * gdata[i].g_std) : 1.0;
         wgt_m[i] = mdata[i].t_std > 0 ? 1.0 / (mdata[i].t_std * mdata[i].t_std) : 1.0;
     }
-    // W^2 (J v) for one data type, J v by central differences along v
+    // W^2 (J v) for one data type, J v exact or by central differences along v
     auto wjv = [&](const Vec& v, bool mag) {
+        Vec w(ns);
+        if (exact) {
+            Vec jg, jt;
+            f.jvp(params, v, jg, jt);
+            for (size_t i = 0; i < ns; ++i) w[i] = mag ? jt[i] * wgt_m[i] : jg[i] * wgt_g[i];
+            return w;
+        }
         Vec pp(np), pm(np), gp, tp, gm, tm;
         for (size_t j = 0; j < np; ++j) {
             pp[j] = params[j] + eps * v[j];
@@ -1430,7 +1481,6 @@ CalibrationResult calibrateMagneticWeight(std::shared_ptr<LithologyModel> model,
         }
         f.forward(pp, gp, tp);
         f.forward(pm, gm, tm);
-        Vec w(ns);
         for (size_t i = 0; i < ns; ++i)
             w[i] = (mag ? (tp[i] - tm[i]) * wgt_m[i] : (gp[i] - gm[i]) * wgt_g[i]) / (2.0 * eps);
         return w;

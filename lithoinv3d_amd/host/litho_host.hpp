@@ -161,6 +161,9 @@ public:
     void applyParameterVector(const Vec& params);
     // vertex gradient (3*n_v) -> parameter gradient (transpose of the map above)
     Vec pullBackVertexGradient(const Vec& vgrad) const;
+    // parameter tangent -> vertex tangent (3*n_v): the linear part of
+    // applyParameterVector, whose transpose pullBackVertexGradient is
+    Vec vertexTangent(const Vec& dparams) const;
     // The linear map itself, per parameter: parameter d moves coordinate dofAxis(d) of
     // vertex vert[e] by w[e] (metres per parameter unit) for e in [off[d], off[d+1]).
     // Defined for axis parameters only (an ALONG_VECTOR parameter moves a vertex along a
@@ -458,6 +461,8 @@ public:
     void setRemanenceMode(RemanentMagnetizationMode m) { m_mode = m; refreshProperties(); }
     // sum_i wg[i] dgz_i/dp + wt[i] dTMI_i/dp at params (either weight vector may be empty)
     Vec jtw(const Vec& params, const Vec& wg, const Vec& wt);
+    // J dp at params: the change of gz and TMI along the parameter tangent dp (lf_jvp)
+    void jvp(const Vec& params, const Vec& dp, Vec& gz, Vec& tmi);
 
 private:
     void ensureForward(const Vec& params);
@@ -504,9 +509,11 @@ Vec invertUnitSusceptibilities(MagneticForward& fwd, const MagneticData& data,
 
 // JointObjective::calibrateDataWeights (joint_objective.cpp:281-393) without forming J:
 // power iteration (at most 50 steps, relative tolerance 1e-8) for the largest eigenvalue
-// of J^T W^2 J of gravity and of magnetics (W = 1/std, or 1 where std is 0), J v from
-// central differences of an FP64 forward (step eps in parameter units along the unit
-// vector v) and J^T u from the adjoint; alpha_mag is multiplied by the eigenvalue ratio
+// of J^T W^2 J of gravity and of magnetics (W = 1/std, or 1 where std is 0), J^T u from
+// the adjoint and J v either exactly, from the Jacobian-vector product kernel in the
+// given precision (eps <= 0), or from central differences of an FP64 forward (step eps in
+// parameter units along the unit vector v; then the FP64 adjoint too, whatever the
+// precision); alpha_mag is multiplied by the eigenvalue ratio
 // clipped to [1e-6, 1e6], or, if the ratio is unusable, by the ratio of the two data-misfit
 // gradient norms.  The power iterations start from a seeded random unit vector (the
 // reference uses Eigen's unseeded Random()).
@@ -519,9 +526,10 @@ CalibrationResult calibrateMagneticWeight(std::shared_ptr<LithologyModel> model,
                                           const GravityData& gdata, const MagneticData& mdata,
                                           double inc_deg, double dec_deg, double field_nT,
                                           const Vec& params, double alpha_mag,
-                                          uint64_t seed = 1, double eps = 1e-3, int device = 0,
+                                          uint64_t seed = 1, double eps = 0.0, int device = 0,
                                           RemanentMagnetizationMode mode =
-                                              RemanentMagnetizationMode::EffectiveSusceptibility);
+                                              RemanentMagnetizationMode::EffectiveSusceptibility,
+                                          Precision prec = Precision::FP64);
 
 // modules/litho-inversion/include/litho_invert/inversion/optimizer.h:7-36
 struct OptimizerResult {

@@ -86,6 +86,8 @@ def load_host():
     L.li_model_params.argtypes = [_P, _P]
     L.li_model_apply.argtypes = [_P, _P, _P]
     L.li_model_pullback.argtypes = [_P, _P, _P]
+    L.li_model_tangent.argtypes = [_P, _P, _P]
+    L.li_problem_jvp.argtypes = [_P, _P, _P, _P, _P]
     L.li_model_bounds.argtypes = [_P, _P, ctypes.c_double, _P, _P]
     L.li_model_smoothness.argtypes = [_P, _P, ctypes.c_double, ctypes.POINTER(ctypes.c_double), _P]
     L.li_problem_add_smoothness.argtypes = [_P, ctypes.c_double]
@@ -249,6 +251,14 @@ class Model:
         g = _a(vgrad).reshape(-1)
         out = np.zeros(self.n_dof)
         _chk(load_host().li_model_pullback(self._h, _p(g), _p(out)), "pullback")
+        return out
+
+    def tangent(self, dparams):
+        """Vertex tangent (n_v, 3) of the parameter tangent dparams: the linear part of
+        apply(), the transpose of pullback()."""
+        d = _a(dparams)
+        out = np.zeros((self.n_v, 3))
+        _chk(load_host().li_model_tangent(self._h, _p(d), _p(out)), "tangent")
         return out
 
     def dof_map(self):
@@ -592,11 +602,20 @@ class Problem:
              "invert_properties")
         return out if per == 1 else out.reshape(-1, per)
 
-    def calibrate(self, params=None, seed=1, eps=1e-3):
+    def jvp(self, params, dparams):
+        """(J_gz dp, J_TMI dp) at params: the data's change along the parameter tangent
+        dparams, one Jacobian-vector product pass on the GPU (problems with magnetic data)."""
+        p, d = _a(params), _a(dparams)
+        gz, tmi = np.zeros(self.n_s), np.zeros(self.n_s)
+        _chk(load_host().li_problem_jvp(self._h, _p(p), _p(d), _p(gz), _p(tmi)), "jvp")
+        return gz, tmi
+
+    def calibrate(self, params=None, seed=1, eps=0.0):
         """Sets alpha_mag from the ratio of the largest eigenvalues of J^T W^2 J of gravity
         and of magnetics at params (default: the model's current parameters), W = 1/std,
-        found matrix-free by power iteration (J v by central differences of an FP64
-        forward, J^T u by the adjoint).  Returns a dict with both eigenvalues, their ratio,
+        found matrix-free by power iteration (J v by the Jacobian-vector product kernel in
+        the problem's precision, or with eps > 0 by central differences of two FP64
+        forwards; J^T u by the adjoint).  Returns a dict with both eigenvalues, their ratio,
         the new alpha_mag, the iteration counts and whether the gradient-norm fallback ran."""
         p = self.model.params() if params is None else np.ascontiguousarray(params, np.float64)
         out = np.zeros(7)

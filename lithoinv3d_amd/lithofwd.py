@@ -17,14 +17,14 @@ LIB_PATH = os.path.join(_HERE, "liblithofwd.so")
 LF_FP64, LF_FP32C = 0, 1
 LF_TMI_REFERENCE, LF_TMI_TEXTBOOK = 0, 1
 LF_WANT_GZ, LF_WANT_TMI = 1, 2
-N_TIMING_CLASSES = 6
+N_TIMING_CLASSES = 7
 TIMING_CLASSES = ("face_setup", "forward", "forward_reduce", "adjoint_pairs",
-                  "adjoint_finalize", "vertex_gather")
+                  "adjoint_finalize", "vertex_gather", "jvp")
 
 EXPORTED = (
     "lf_create", "lf_destroy", "lf_last_error", "lf_set_vertices", "lf_set_vertices_device",
     "lf_set_properties", "lf_forward", "lf_forward_device", "lf_adjoint", "lf_adjoint_device",
-    "lf_sync", "lf_set_timing", "lf_timing", "lf_stats", "lf_colnorm2", "lf_forward_noalloc",
+    "lf_jvp", "lf_jvp_device", "lf_sync", "lf_set_timing", "lf_timing", "lf_stats", "lf_colnorm2", "lf_forward_noalloc",
 )
 # the learned-preconditioner MLP (declared in mlp.py)
 EXPORTED_MLP = (
@@ -104,6 +104,8 @@ def load_library(path: str = LIB_PATH):
     L.lf_forward_device.argtypes = [P, P, P]
     L.lf_adjoint.argtypes = [P, P, P, P]
     L.lf_adjoint_device.argtypes = [P, P, P, P]
+    L.lf_jvp.argtypes = [P, P, P, P]
+    L.lf_jvp_device.argtypes = [P, P, P, P]
     L.lf_sync.argtypes = [P]
     L.lf_set_timing.argtypes = [P, ctypes.c_int]
     L.lf_timing.argtypes = [P, _D, ctypes.POINTER(i64), ctypes.c_int]
@@ -245,6 +247,16 @@ class ForwardContext:
                                                 ctypes.c_void_p(wt_ptr or None),
                                                 ctypes.c_void_p(grad_ptr)),
                     "lf_adjoint_device")
+
+    def jvp(self, vtan):
+        """(dgz, dTMI) along the vertex tangent vtan (n_v, 3): J v in one pass."""
+        v = _c(vtan, np.float64).reshape(-1)
+        if len(v) != 3 * self.n_v:
+            raise ValueError("tangent length != 3 * n_vertices")
+        gz = np.zeros(self.n_s)
+        tmi = np.zeros(self.n_s)
+        self._check(self._lib.lf_jvp(self._ctx, _ptr(v), _ptr(gz), _ptr(tmi)), "lf_jvp")
+        return gz, tmi
 
     def colnorm2(self, axis, dof_off, dof_vertex, dof_weight):
         """(sum_i (dgz_i/dp_j)^2, sum_i (dTMI_i/dp_j)^2) for parameters p_j that move

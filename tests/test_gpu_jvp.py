@@ -1,0 +1,146 @@
+"""The Jacobian-vector product (lf_jvp, li_problem_jvp): J v of gz and TMI along a vertex
+or parameter tangent in one pass over the station x facet pairs, the product the
+reference's magnetic-weight calibration forms with dense finite-difference Jacobians
+(joint_objective.cpp:281-393; gravity_forward.cpp:184-292, magnetic_forward.cpp:196-306):
+
+  * FP64 against central differences of the FP64 forward along the same tangent
+    (rel. 1e-7) and against the oracle's Jacobian (the reference's one-sided differences,
+    rel. 5e-5, the bound the adjoint's test uses);
+  * <J v, w> = <v, J^T w> with the adjoint, FP64 to 1e-12 and FP32C to 1e-5 of the sum
+    of the terms' magnitudes;
+  * FP32C against FP64 (1e-4 of the largest value), on a model with remanence too;
+  * the problem-level product through the parameter map (control points), and the
+    calibration with the exact product against the one with central differences.
+"""
+import numpy as np
+import pytest
+
+from lithoinv3d_amd import inversion, synth
+from lithoinv3d_amd.lithofwd import ForwardContext
+
+pytestmark = pytest.mark.gpu
+
+INC, DEC, FIELD = -60.0, 10.0, 50000.0
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def model_and_stations(seed=3):
+    rng = np.random.default_rng(seed)
+    labels = rng.integers(0, 3, size=(6, 5, 4)).astype(np.int32)
+    m = synth.label_grid_model(labels, (-75.0, -62.5, -31.0), (25.0, 25.0, 25.0),
+                               [0.25, -0.15, 0.4], [0.02, 0.003, 0.04])
+    st = synth.topo_stations(7, 180.0)
+    return m, st
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+def test_jvp_fp64_vs_differences_and_oracle(oracle):
+    m, st = model_and_stations()
+    fh = synth.field_unit_vector(INC, DEC)
+    rng = np.random.default_rng(1)
+    vt = rng.normal(size=(m.n_vertices, 3))
+    say("jvp fp64 vs central differences")
+    with ForwardContext(m, st, fh, FIELD, precision="fp64") as ctx:
+        jg, jt = ctx.jvp(vt)
+        h = 1e-3
+        ctx.set_vertices(m.vertices + h * vt)
+        gp, tp = ctx.forward()
+        ctx.set_vertices(m.vertices - h * vt)
+        gm, tm = ctx.forward()
+    fg, ft = (gp - gm) / (2 * h), (tp - tm) / (2 * h)
+    print(f"vs central differences: gz {rel(jg, fg):.2e}, TMI {rel(jt, ft):.2e}")
+    assert rel(jg, fg) < 1e-7 and rel(jt, ft) < 1e-7
+    om = oracle.OracleModel.from_litho(m, FIELD, fh)
+    Jg = om.gravity_vertex_jacobian(st).reshape(len(st), -1)
+    Jt = om.magnetic_vertex_jacobian(fh, st).reshape(len(st), -1)
+    eg, et = rel(jg, Jg @ vt.ravel()), rel(jt, Jt @ vt.ravel())
+    print(f"vs the oracle's one-sided Jacobian: gz {eg:.2e}, TMI {et:.2e}")
+    assert eg < 5e-5 and et < 5e-5
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32c", 1e-5)])
+def test_jvp_is_the_adjoints_transpose(prec, tol):
+    m, st = model_and_stations(4)
+    fh = synth.field_unit_vector(INC, DEC)
+    rng = np.random.default_rng(2)
+    vt = rng.normal(size=(m.n_vertices, 3))
+    wg, wt = rng.normal(size=len(st)), rng.normal(size=len(st))
+    say(f"<J v, w> = <v, J^T w>, {prec}")
+    with ForwardContext(m, st, fh, FIELD, precision=prec) as ctx:
+        jg, jt = ctx.jvp(vt)
+        ag = ctx.adjoint(wg, None)
+        at = ctx.adjoint(None, wt)
+    # both sides are sums of terms of either sign that cancel to a small result: the
+    # difference is bounded relative to the sum of the terms' magnitudes
+    for name, j, w, a in (("gz", jg, wg, ag), ("TMI", jt, wt, at)):
+        lhs, rhs = j @ w, np.sum(vt * a)
+        scale = max(np.abs(j) @ np.abs(w), np.sum(np.abs(vt * a)))
+        print(f"{prec} {name}: {lhs:.12e} vs {rhs:.12e}, |diff| / scale "
+              f"{abs(lhs - rhs) / scale:.2e}")
+        assert abs(lhs - rhs) <= tol * scale
+
+
+@pytest.mark.parametrize("remanence", [False, True])
+def test_jvp_fp32c_vs_fp64(remanence):
+    m, st = model_and_stations(5)
+    fh = synth.field_unit_vector(INC, DEC)
+    mag = None
+    if remanence:
+        rng = np.random.default_rng(9)
+        mag = rng.normal(0, 2.0, (m.n_units, 3))
+    rng = np.random.default_rng(3)
+    vt = rng.normal(size=(m.n_vertices, 3))
+    out = {}
+    for prec in ("fp64", "fp32c"):
+        say(f"jvp {prec}, remanence {remanence}")
+        with ForwardContext(m, st, fh, FIELD, precision=prec, magnetization=mag) as ctx:
+            out[prec] = ctx.jvp(vt)
+    eg, et = rel(out["fp32c"][0], out["fp64"][0]), rel(out["fp32c"][1], out["fp64"][1])
+    print(f"fp32c vs fp64: gz {eg:.2e}, TMI {et:.2e}")
+    assert eg < 1e-4 and et < 1e-4
+
+
+def test_problem_jvp_and_calibration():
+    m, st = model_and_stations(6)
+    M = inversion.Model(m)
+    M.fix_exterior(inversion.Z_ONLY)
+    M.set_axis_scale(10.0)
+    n = M.set_cp_stride(2, 25.0)
+    p0 = M.params()
+    rng = np.random.default_rng(4)
+    dp = rng.normal(size=n)
+    k = len(st)
+    gz_obs = rng.normal(0, 0.3, k)
+    tmi_obs = rng.normal(0, 20.0, k)
+    res = {}
+    for prec in ("fp64", "fp32c"):
+        P = inversion.Problem(M, st, gz_obs, np.full(k, 0.05), tmi_obs, np.full(k, 1.0),
+                              inc=INC, dec=DEC, field=FIELD, precision=prec)
+        say(f"problem jvp {prec}")
+        jg, jt = P.jvp(p0, dp)
+        if prec == "fp64":
+            h = 1e-3
+            gp, tp = P.forward(p0 + h * dp)
+            gm, tm = P.forward(p0 - h * dp)
+            eg, et = rel(jg, (gp - gm) / (2 * h)), rel(jt, (tp - tm) / (2 * h))
+            print(f"problem jvp vs central differences: gz {eg:.2e}, TMI {et:.2e}")
+            assert eg < 1e-7 and et < 1e-7
+        res[prec] = (jg, jt)
+        say(f"calibration {prec}: exact J v and central differences")
+        exact = P.calibrate(p0, seed=1)
+        P.close()
+        P = inversion.Problem(M, st, gz_obs, np.full(k, 0.05), tmi_obs, np.full(k, 1.0),
+                              inc=INC, dec=DEC, field=FIELD, precision=prec)
+        fd = P.calibrate(p0, seed=1, eps=1e-3)
+        P.close()
+        print(prec, exact, fd)
+        for key in ("eig_gravity", "eig_magnetic", "alpha_mag"):
+            assert exact[key] == pytest.approx(fd[key], rel=1e-4 if prec == "fp32c" else 1e-6)
+    assert rel(res["fp32c"][0], res["fp64"][0]) < 1e-4
+    assert rel(res["fp32c"][1], res["fp64"][1]) < 1e-4

@@ -108,6 +108,7 @@ def test_control_points_map_and_transpose(stride):
     rng = np.random.default_rng(stride)
     dp = rng.normal(size=n)
     dV = M.apply(p0 + dp) - M.apply(p0)
+    np.testing.assert_allclose(M.tangent(dp), dV, rtol=0, atol=1e-9 * max(1.0, np.abs(dV).max()))
     # every vertex moves by a convex combination of control-point displacements
     assert np.abs(dV[:, 2]).max() <= np.abs(dp).max() * 10.0 + 1e-9
     # exterior (FIXED) vertices stay where they are
@@ -372,6 +373,8 @@ def test_position_form_control_points_against_reference(stride):
     # the map is affine in p: the pullback is its transpose
     dp = rng.normal(size=len(p0))
     dX = M.apply(p0 + dp) - M.apply(p0)
+    # the linear part of the affine map: li_model_tangent
+    np.testing.assert_allclose(M.tangent(dp), dX, rtol=0, atol=1e-9 * max(1.0, np.abs(dX).max()))
     vg = rng.normal(size=(m.n_vertices, 3))
     assert np.sum(vg * dX) == pytest.approx(M.pullback(vg) @ dp, rel=1e-12)
     # the displacement form is still there and leaves the start geometry unchanged
@@ -428,6 +431,8 @@ def test_along_vector_freedom():
     vg = rng.normal(size=(m.n_vertices, 3))
     dp = rng.normal(size=len(p0))
     dX = M.apply(p0 + dp) - M.apply(p0)
+    # the linear part of the affine map: li_model_tangent
+    np.testing.assert_allclose(M.tangent(dp), dX, rtol=0, atol=1e-9 * max(1.0, np.abs(dX).max()))
     assert np.sum(vg * dX) == pytest.approx(M.pullback(vg) @ dp, rel=1e-12)
     p = p0 + rng.normal(0, 1.0, len(p0))
     val, g = M.smoothness(p, 1.0)
@@ -460,6 +465,8 @@ def test_threaded_parameter_map_and_pullback(mode):
     rng = np.random.default_rng(7)
     dp = rng.normal(size=n)
     dX = M.apply(p0 + dp) - M.apply(p0)
+    # the linear part of the affine map: li_model_tangent
+    np.testing.assert_allclose(M.tangent(dp), dX, rtol=0, atol=1e-9 * max(1.0, np.abs(dX).max()))
     vg = rng.normal(size=(m.n_vertices, 3))
     g = M.pullback(vg)
     assert np.sum(vg * dX) == pytest.approx(g @ dp, rel=1e-12)
