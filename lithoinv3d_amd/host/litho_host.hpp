@@ -5,7 +5,7 @@
 // ReferenceModelRegularization), so that the GPU path drops in under the same names.
 //
 // Vectors are std::vector<double> and matrices column-major (the layout of Eigen's
-// MatrixXd); INTEGRATION.md shows the Eigen adapter a reference build would add.
+// MatrixXd), so an Eigen caller maps its VectorXd / MatrixXd storage onto them directly.
 //
 // Differences from the reference, by design:
 //   * the model keeps ONE shared vertex array (the reference keeps one per group mesh

@@ -15,7 +15,7 @@ FP32 accumulation) from local geometry and B itself.  The subset undersamples th
 response of shallow parameters, whose footprint is narrower than the subset's spacing;
 the network learns that, halving the error of B alone.
 The objective stays exact, so the plain solve (H0 = gamma * I) remains both the fallback
-and the judge of the gain.  Training targets are the exact column norms over all
+and the yardstick of the gain.  Training targets are the exact column norms over all
 stations (lf_colnorm2).
 """
 from __future__ import annotations
