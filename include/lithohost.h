@@ -307,6 +307,29 @@ int li_problem_gncg_minimize(void* h, const double* x0, const double* lb, const 
                              int64_t hist_cap, int64_t* n_hist, int32_t* cg_iters,
                              int64_t cg_cap, int32_t* iterations, char* msg, int64_t msg_cap);
 
+/* ---- run configuration files (modules/litho-io/src/io/ini_config.cpp) ---- */
+
+/* IniConfig::load ("[section]", "key = value" or "key: value", '#' / ';' comments);
+ * a file that cannot be opened is an error */
+int li_ini_load(const char* path, void** out);
+/* the same parser on text in memory; text NULL gives an empty configuration */
+int li_ini_parse(const char* text, void** out);
+void li_ini_destroy(void* h);
+/* IniConfig::getString: *found 0 and value "" when absent; value cut to cap - 1 bytes */
+int li_ini_get(void* h, const char* section, const char* key, char* value, int64_t cap,
+               int32_t* found);
+/* getDouble / getInt / getBool: def for a missing or empty value or (numbers) no leading
+ * number; booleans true for "true", "yes", "1", "on" in any case */
+int li_ini_get_double(void* h, const char* section, const char* key, double def, double* out);
+int li_ini_get_int(void* h, const char* section, const char* key, int32_t def, int32_t* out);
+int li_ini_get_bool(void* h, const char* section, const char* key, int32_t def, int32_t* out);
+/* IniConfig::setString / save */
+int li_ini_set(void* h, const char* section, const char* key, const char* value);
+int li_ini_save(void* h, const char* path);
+/* every entry as "section<TAB>key<TAB>value<LF>" lines, sections and keys sorted; *need
+ * gets the size with the terminating zero (call with cap 0 first) */
+int li_ini_dump(void* h, char* buf, int64_t cap, int64_t* need);
+
 /* ---- file formats (modules/litho-io) ---- */
 
 /* .ts (surface_loader.cpp:93-148) or .obj (:9-91) by extension (:150-168) */

@@ -5,6 +5,7 @@
 #include <cstring>
 #include <memory>
 #include <stdexcept>
+#include <sstream>
 #include <string>
 
 #include "../../include/lithohost.h"
@@ -77,6 +78,9 @@ struct ModelHandle {
 struct MeshHandle {
     SurfaceMeshData m;
     std::string warnings;
+};
+struct IniHandle {
+    IniConfig c;
 };
 struct StationsHandle {
     std::vector<StationRecord> s;
@@ -964,6 +968,77 @@ int li_problem_set_iteration_callback(void* h, li_iter_cb cb, void* user)
 }
 
 // ------------------------------------------------------------------ file formats
+
+int li_ini_load(const char* path, void** out)
+{
+    return guard([&] {
+        auto h = std::make_unique<IniHandle>();
+        if (!path || !h->c.load(path))
+            throw std::runtime_error(std::string("li_ini_load: cannot open ") + (path ? path : "(null)"));
+        *out = h.release();
+    });
+}
+
+int li_ini_parse(const char* text, void** out)
+{
+    return guard([&] {
+        auto h = std::make_unique<IniHandle>();
+        if (text) {
+            std::istringstream in(text);
+            h->c.parse(in);
+        }
+        *out = h.release();
+    });
+}
+
+void li_ini_destroy(void* h) { delete (IniHandle*)h; }
+
+int li_ini_get(void* h, const char* section, const char* key, char* value, int64_t cap,
+               int32_t* found)
+{
+    return guard([&] {
+        const auto& c = ((IniHandle*)h)->c;
+        const bool f = c.has(section, key);
+        if (found) *found = f ? 1 : 0;
+        copy_text(f ? c.getString(section, key) : std::string(), value, cap);
+    });
+}
+
+int li_ini_get_double(void* h, const char* section, const char* key, double def, double* out)
+{
+    return guard([&] { *out = ((IniHandle*)h)->c.getDouble(section, key, def); });
+}
+
+int li_ini_get_int(void* h, const char* section, const char* key, int32_t def, int32_t* out)
+{
+    return guard([&] { *out = ((IniHandle*)h)->c.getInt(section, key, def); });
+}
+
+int li_ini_get_bool(void* h, const char* section, const char* key, int32_t def, int32_t* out)
+{
+    return guard([&] { *out = ((IniHandle*)h)->c.getBool(section, key, def != 0) ? 1 : 0; });
+}
+
+int li_ini_set(void* h, const char* section, const char* key, const char* value)
+{
+    return guard([&] { ((IniHandle*)h)->c.setString(section, key, value ? value : ""); });
+}
+
+int li_ini_save(void* h, const char* path)
+{
+    return guard([&] { ((IniHandle*)h)->c.save(path); });
+}
+
+int li_ini_dump(void* h, char* buf, int64_t cap, int64_t* need)
+{
+    return guard([&] {
+        std::string t;
+        for (const auto& sec : ((IniHandle*)h)->c.data())
+            for (const auto& kv : sec.second) t += sec.first + "\t" + kv.first + "\t" + kv.second + "\n";
+        if (need) *need = (int64_t)t.size() + 1;
+        copy_text(t, buf, cap);
+    });
+}
 
 int li_mesh_load(const char* path, void** out)
 {

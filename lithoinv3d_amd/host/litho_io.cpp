@@ -409,4 +409,120 @@ AssembledModel assembleUnitMeshes(const std::vector<SurfaceMeshData>& meshes)
     return M;
 }
 
+// ---------------------------------------------------------------- ini files
+
+namespace {
+std::string trim_ws(const std::string& s)
+{
+    size_t a = 0, b = s.size();
+    while (a < b && std::isspace((unsigned char)s[a])) ++a;
+    while (b > a && std::isspace((unsigned char)s[b - 1])) --b;
+    return s.substr(a, b - a);
+}
+}  // namespace
+
+bool IniConfig::load(const std::string& path)
+{
+    std::ifstream f(path);
+    if (!f.is_open()) return false;
+    m_data.clear();
+    parse(f);
+    return true;
+}
+
+void IniConfig::parse(std::istream& in)
+{
+    std::string line, section;
+    while (std::getline(in, line)) {
+        const std::string t = trim_ws(line);
+        if (t.empty() || t[0] == '#' || t[0] == ';') continue;
+        if (t[0] == '[') {
+            const size_t e = t.find(']');
+            if (e == std::string::npos) continue;
+            section = trim_ws(t.substr(1, e - 1));
+            m_data[section];
+            continue;
+        }
+        size_t sep = t.find('=');
+        if (sep == std::string::npos) sep = t.find(':');
+        if (sep == std::string::npos) continue;
+        const std::string key = trim_ws(t.substr(0, sep));
+        if (!key.empty() && !section.empty()) m_data[section][key] = trim_ws(t.substr(sep + 1));
+    }
+}
+
+void IniConfig::save(const std::string& path) const
+{
+    std::ofstream f(path);
+    if (!f.is_open()) throw std::runtime_error("IniConfig::save: cannot write " + path);
+    for (const auto& sec : m_data) {
+        f << "[" << sec.first << "]\n";
+        for (const auto& kv : sec.second) f << kv.first << " = " << kv.second << "\n";
+        f << "\n";
+    }
+}
+
+bool IniConfig::hasSection(const std::string& section) const
+{
+    return m_data.count(section) != 0;
+}
+
+std::vector<std::string> IniConfig::sections() const
+{
+    std::vector<std::string> out;
+    for (const auto& sec : m_data) out.push_back(sec.first);
+    return out;
+}
+
+bool IniConfig::has(const std::string& section, const std::string& key) const
+{
+    const auto it = m_data.find(section);
+    return it != m_data.end() && it->second.count(key) != 0;
+}
+
+std::string IniConfig::getString(const std::string& section, const std::string& key,
+                                 const std::string& def) const
+{
+    const auto it = m_data.find(section);
+    if (it == m_data.end()) return def;
+    const auto kv = it->second.find(key);
+    return kv == it->second.end() ? def : kv->second;
+}
+
+double IniConfig::getDouble(const std::string& section, const std::string& key, double def) const
+{
+    const std::string v = getString(section, key);
+    if (v.empty()) return def;
+    try {
+        return std::stod(v);
+    } catch (...) {
+        return def;
+    }
+}
+
+int IniConfig::getInt(const std::string& section, const std::string& key, int def) const
+{
+    const std::string v = getString(section, key);
+    if (v.empty()) return def;
+    try {
+        return std::stoi(v);
+    } catch (...) {
+        return def;
+    }
+}
+
+bool IniConfig::getBool(const std::string& section, const std::string& key, bool def) const
+{
+    std::string v = getString(section, key);
+    if (v.empty()) return def;
+    for (auto& c : v) c = (char)std::tolower((unsigned char)c);
+    return v == "true" || v == "yes" || v == "1" || v == "on";
+}
+
+void IniConfig::setString(const std::string& section, const std::string& key,
+                          const std::string& value)
+{
+    m_data[section][key] = value;
+}
+
 }  // namespace lithofwd

@@ -5,6 +5,8 @@
 // layout the GPU forward consumes.
 #pragma once
 #include <cstdint>
+#include <iosfwd>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -70,5 +72,32 @@ struct AssembledModel {
     int64_t mergedVertices = 0;   // input vertices that mapped onto an earlier one
 };
 AssembledModel assembleUnitMeshes(const std::vector<SurfaceMeshData>& meshes);
+
+// The run configuration file of modules/litho-io/src/io/ini_config.cpp: "[section]"
+// headers, "key = value" or "key: value" lines (the first '=' splits, else the first ':'),
+// both sides trimmed, '#' and ';' comment lines, blank lines and lines before the first
+// section ignored, a repeated key keeps its last value.  The typed getters return the
+// default for a missing or empty value; numbers parse their leading part as std::stod /
+// std::stoi do (default when none), booleans are true for "true", "yes", "1", "on" in any
+// case and false for anything else.
+class IniConfig {
+public:
+    bool load(const std::string& path);  // false when the file cannot be opened
+    void parse(std::istream& in);
+    void save(const std::string& path) const;  // sections and keys in sorted order
+    bool hasSection(const std::string& section) const;
+    std::vector<std::string> sections() const;
+    const std::map<std::string, std::map<std::string, std::string>>& data() const { return m_data; }
+    bool has(const std::string& section, const std::string& key) const;
+    std::string getString(const std::string& section, const std::string& key,
+                          const std::string& def = "") const;
+    double getDouble(const std::string& section, const std::string& key, double def) const;
+    int getInt(const std::string& section, const std::string& key, int def) const;
+    bool getBool(const std::string& section, const std::string& key, bool def) const;
+    void setString(const std::string& section, const std::string& key, const std::string& value);
+
+private:
+    std::map<std::string, std::map<std::string, std::string>> m_data;
+};
 
 }  // namespace lithofwd

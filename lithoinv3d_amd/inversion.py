@@ -510,6 +510,184 @@ def minimize(fun, grad, x0, lb, ub, max_iter=100, history=10, ftol=1e-6, c1=1e-4
                    None, ctypes.byref(o), _p(h))
 
 
+# InversionConfig::fromIni (modules/litho-inversion/src/inversion/ini_config_wiring.cpp:
+# 6-131): (attribute, section, key, kind, default).  kind "s" strings, "f" getDouble,
+# "i" getInt, "b" getBool; the enumerations are read as strings and normalized as there.
+_INI_FIELDS = (
+    ("max_iterations", "inversion", "max_iterations", "i", 500),
+    ("tolerance", "inversion", "tolerance", "f", 1e-6),
+    ("smoothness_weight", "inversion", "lambda", "f", 1.0),
+    ("omega", "inversion", "omega", "f", 1e6),
+    ("lbfgs_history", "inversion", "lbfgs_history", "i", 10),
+    ("fd_step", "inversion", "fd_step", "f", 1.0),
+    ("armijo_c1", "inversion", "armijo_c1", "f", 1e-4),
+    ("line_search_max_iter", "inversion", "line_search_max_iter", "i", 50),
+    ("disable_line_search", "inversion", "disable_line_search", "b", False),
+    ("control_point_stride", "inversion", "control_point_stride", "i", 0),
+    ("enable_property_inversion", "inversion", "enable_property_inversion", "b", False),
+    ("enable_geometry_inversion", "inversion", "enable_geometry_inversion", "b", True),
+    ("property_inversion_interval", "inversion", "property_inversion_interval", "i", 50),
+    ("property_inversion_max_iter", "inversion", "property_inversion_max_iter", "i", 20),
+    ("property_damping", "inversion", "property_damping", "f", 0.01),
+    ("vertex_freedom", "inversion", "vertex_freedom", "s", "z_only"),
+    ("mesh_boundary_mode", "inversion", "mesh_boundary_mode", "s", "free"),
+    ("solver", "inversion", "solver", "s", "lbfgsb"),
+    ("gncg_cg_max_iter", "inversion", "gncg_cg_max_iter", "i", 50),
+    ("gncg_cg_tolerance", "inversion", "gncg_cg_tolerance", "f", 1e-6),
+    ("enable_eigenvalue_scaling", "inversion", "enable_eigenvalue_scaling", "b", True),
+    ("density_min", "gravity", "density_min", "f", 1.0),
+    ("density_max", "gravity", "density_max", "f", 6.0),
+    ("gravity_uncertainty", "gravity", "gravity_uncertainty", "f", 0.0),
+    ("magnetic_weight", "magnetic", "magnetic_weight", "f", 1.0),
+    ("inclination", "magnetic", "inclination", "f", 75.0),
+    ("declination", "magnetic", "declination", "f", -20.0),
+    ("field_nT", "magnetic", "field_nT", "f", 55000.0),
+    ("magnetic_uncertainty", "magnetic", "magnetic_uncertainty", "f", 0.0),
+    ("susceptibility_min", "magnetic", "susceptibility_min", "f", 0.0),
+    ("susceptibility_max", "magnetic", "susceptibility_max", "f", 1.0),
+    ("remanence_min", "magnetic", "remanence_min", "f", 0.0),
+    ("remanence_max", "magnetic", "remanence_max", "f", 10.0),
+    ("remanence_component_min", "magnetic", "remanence_component_min", "f", -10.0),
+    ("remanence_component_max", "magnetic", "remanence_component_max", "f", 10.0),
+    ("remanence_mode", "magnetic", "remanence_mode", "s", "effective_susceptibility"),
+    ("enable_padding_group", "padding", "enable_padding_group", "b", False),
+    ("padding_density_initial", "padding", "padding_density_initial", "f", 2.67),
+    ("padding_density_lower", "padding", "padding_density_lower", "f", 1.0),
+    ("padding_density_upper", "padding", "padding_density_upper", "f", 6.0),
+    ("padding_depth", "padding", "padding_depth", "f", -100000.0),
+    ("padding_susceptibility_initial", "padding", "padding_susceptibility_initial", "f", 0.0),
+    ("padding_susceptibility_lower", "padding", "padding_susceptibility_lower", "f", 0.0),
+    ("padding_susceptibility_upper", "padding", "padding_susceptibility_upper", "f", 1.0),
+    ("padding_conductivity_initial", "padding", "padding_conductivity_initial", "f", 1e-4),
+    ("padding_conductivity_lower", "padding", "padding_conductivity_lower", "f", 1e-5),
+    ("padding_conductivity_upper", "padding", "padding_conductivity_upper", "f", 1e2),
+    ("topography_mode", "topography", "mode", "s", "none"),
+    ("dem_file", "topography", "dem_file", "s", ""),
+    ("datum_elevation", "topography", "datum_elevation", "f", 0.0),
+    ("bouguer_density", "topography", "bouguer_density", "f", 2.67),
+    ("padding_rings", "topography", "padding_rings", "i", 0),
+    ("padding_cell_size", "topography", "padding_cell_size", "f", 0.0),
+    ("invert_halfspace_properties", "topography", "invert_halfspace_properties", "b", False),
+    ("projected_beyond_survey_area", "topography", "projected_beyond_survey_area", "s",
+     "independent_flat"),
+    ("enable_reference_model", "regularization", "enable_reference_model", "b", False),
+    ("lambda_ref", "regularization", "lambda_ref", "f", 0.1),
+    ("gradient_smoothing_weight", "regularization", "gradient_smoothing_weight", "f", 0.0),
+    ("enable_depth_bounds", "bounds", "enable_depth_bounds", "b", False),
+    ("depth_bound_margin", "bounds", "depth_bound_margin", "f", 500.0),
+    ("fail_on_missing_meshes", "data", "fail_on_missing_meshes", "b", True),
+    ("iteration_export_dir", "output", "iteration_export_dir", "s", ""),
+    ("line_search_export_dir", "output", "line_search_export_dir", "s", ""),
+)
+# the enumerations as fromIni maps them: any other value is the first (default) one
+_INI_CHOICES = {
+    "vertex_freedom": ("z_only", "xyz_free", "xy_free"),
+    "remanence_mode": ("effective_susceptibility", "fixed_vector", "vector"),
+    "topography_mode": ("none", "raw", "terrain_corrected"),
+    "projected_beyond_survey_area": ("independent_flat", "projected_beyond_survey"),
+}
+_FREEDOM_OF = {"z_only": Z_ONLY, "xyz_free": XYZ_FREE, "xy_free": XY_FREE}
+
+
+def _ini_config_class():
+    fields = [(a, type(d) if not isinstance(d, bool) else bool, dataclasses.field(default=d))
+              for a, _, _, _, d in _INI_FIELDS]
+    return dataclasses.make_dataclass("InversionConfig", fields)
+
+
+class InversionConfig(_ini_config_class()):
+    """The run configuration of InversionConfig (runner.h:27-170) as its INI file holds it
+    (section and key names of ini_config_wiring.cpp, the reference's defaults).
+    ``smoothness_weight`` is the file's [inversion] lambda.  The padding, topography and
+    EM fields are read and written but drive nothing here (the EM/MT phases they serve are
+    not built).
+
+    from_ini / to_ini restate fromIni / toIni; to_ini writes numbers in full precision
+    where the reference's std::to_string keeps six decimals.  apply_to_model,
+    problem_kwargs and run_kwargs hand the settings to Model, Problem and Problem.run the
+    way the reference's runner set-up uses them (runner.cpp:30-212)."""
+
+    @classmethod
+    def from_ini(cls, source):
+        """source: a path, or a litho_io.IniConfig."""
+        from . import litho_io
+
+        ini = source if isinstance(source, litho_io.IniConfig) else litho_io.IniConfig(source)
+        kw = {}
+        for attr, sec, key, kind, default in _INI_FIELDS:
+            if kind == "f":
+                v = ini.get_double(sec, key, default)
+            elif kind == "i":
+                v = ini.get_int(sec, key, default)
+            elif kind == "b":
+                v = ini.get_bool(sec, key, default)
+            else:
+                v = ini.get(sec, key, default)
+                if attr in _INI_CHOICES and v not in _INI_CHOICES[attr]:
+                    v = _INI_CHOICES[attr][0]
+            kw[attr] = v
+        return cls(**kw)
+
+    def to_ini(self, path=None):
+        """The configuration as an IniConfig (written to path when given)."""
+        from . import litho_io
+
+        ini = litho_io.IniConfig()
+        for attr, sec, key, kind, _ in _INI_FIELDS:
+            v = getattr(self, attr)
+            ini.set(sec, key, ("true" if v else "false") if kind == "b" else
+                    repr(float(v)) if kind == "f" else str(v))
+        if path is not None:
+            ini.save(path)
+        return ini
+
+    def apply_to_model(self, model: "Model"):
+        """Vertex freedom through the reference's exterior classification, axis scale 10 m
+        on every axis, and the control-point stride (runner.cpp:34-71): returns the
+        parameter count (a stride that leaves no parameters falls back to none, as
+        there)."""
+        model.fix_exterior_faces(_FREEDOM_OF[self.vertex_freedom])
+        model.set_axis_scale(10.0)
+        if self.control_point_stride > 0:
+            n = model.set_cp_stride(self.control_point_stride)
+            if n == 0:
+                model.set_cp_stride(0)
+        return model.n_dof
+
+    def problem_kwargs(self):
+        """Problem(...) arguments: the inducing field, alpha_mag, the smoothness (lambda)
+        and reference-model (lambda_ref when enabled) weights, the default data
+        uncertainties, and omega as the depth-to-contact term's weight."""
+        return dict(inc=self.inclination, dec=self.declination, field=self.field_nT,
+                    alpha_mag=self.magnetic_weight, smooth_weight=self.smoothness_weight,
+                    ref_weight=self.lambda_ref if self.enable_reference_model else 0.0,
+                    borehole_weight=self.omega, gravity_uncertainty=self.gravity_uncertainty,
+                    magnetic_uncertainty=self.magnetic_uncertainty)
+
+    def run_kwargs(self):
+        """Problem.run(...) arguments (runner.cpp:192-706 reads these fields)."""
+        return dict(
+            max_iterations=self.max_iterations,
+            property_interval=self.property_inversion_interval,
+            geometry=self.enable_geometry_inversion,
+            properties=self.enable_property_inversion,
+            eigenvalue_scaling=self.enable_eigenvalue_scaling,
+            gradient_smoothing=self.gradient_smoothing_weight,
+            depth_bounds=self.enable_depth_bounds, depth_margin=self.depth_bound_margin,
+            lbfgs_history=self.lbfgs_history, tolerance=self.tolerance,
+            armijo_c1=self.armijo_c1, line_search_max_iter=self.line_search_max_iter,
+            density_bounds=(self.density_min, self.density_max),
+            density_damping=self.property_damping,
+            susceptibility_bounds=(self.susceptibility_min, self.susceptibility_max),
+            susceptibility_damping=self.property_damping,
+            property_max_iter=self.property_inversion_max_iter,
+            solver="gncg" if self.solver == "gncg" else "lbfgsb",
+            gncg_cg_max_iter=self.gncg_cg_max_iter,
+            gncg_cg_tolerance=self.gncg_cg_tolerance, fd_step=self.fd_step,
+            disable_line_search=self.disable_line_search,
+            iteration_export_dir=self.iteration_export_dir or None)
+
+
 class Problem:
     """Joint gravity + magnetic objective (JointObjective) over the GPU forward and
     adjoint for one station set: Phi = sum (w_g r_g)^2/2 + alpha sum (w_t r_t)^2/2

@@ -58,6 +58,18 @@ def _lib():
         L.li_assembled_get.argtypes = [_P, _P, _P, _P, _P]
         L.li_assembled_destroy.argtypes = [_P]
         L.li_assembled_destroy.restype = None
+        cs = ctypes.c_char_p
+        L.li_ini_load.argtypes = [cs, ctypes.POINTER(_P)]
+        L.li_ini_parse.argtypes = [cs, ctypes.POINTER(_P)]
+        L.li_ini_destroy.argtypes = [_P]
+        L.li_ini_destroy.restype = None
+        L.li_ini_get.argtypes = [_P, cs, cs, cs, _i64, ctypes.POINTER(ctypes.c_int32)]
+        L.li_ini_get_double.argtypes = [_P, cs, cs, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
+        L.li_ini_get_int.argtypes = [_P, cs, cs, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+        L.li_ini_get_bool.argtypes = [_P, cs, cs, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+        L.li_ini_set.argtypes = [_P, cs, cs, cs]
+        L.li_ini_save.argtypes = [_P, cs]
+        L.li_ini_dump.argtypes = [_P, cs, _i64, ctypes.POINTER(_i64)]
         _declared = True
     return L
 
@@ -263,3 +275,67 @@ def litho_model(meshes, density, susceptibility) -> synth.LithoModel:
     a = assemble(meshes)
     return synth.LithoModel(a.vertices, a.faces, a.front, a.back,
                             np.asarray(density, np.float64), np.asarray(susceptibility, np.float64))
+
+
+class IniConfig:
+    """A run configuration file (IniConfig, modules/litho-io/src/io/ini_config.cpp) held
+    by the host library: typed getters with the reference's defaults and parsing rules."""
+
+    def __init__(self, path=None, text=None):
+        L = _lib()
+        self._h = _P()
+        if path is not None:
+            _chk(L.li_ini_load(_path(path), ctypes.byref(self._h)), f"read_ini({path})")
+        else:
+            _chk(L.li_ini_parse(None if text is None else text.encode(), ctypes.byref(self._h)),
+                 "parse_ini")
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            _lib().li_ini_destroy(self._h)
+            self._h = _P()
+
+    def get(self, section, key, default=None):
+        buf = ctypes.create_string_buffer(1 << 16)
+        found = ctypes.c_int32()
+        _chk(_lib().li_ini_get(self._h, section.encode(), key.encode(), buf, 1 << 16,
+                               ctypes.byref(found)), "ini get")
+        return buf.value.decode() if found.value else default
+
+    def get_double(self, section, key, default):
+        out = ctypes.c_double()
+        _chk(_lib().li_ini_get_double(self._h, section.encode(), key.encode(), float(default),
+                                      ctypes.byref(out)), "ini get_double")
+        return out.value
+
+    def get_int(self, section, key, default):
+        out = ctypes.c_int32()
+        _chk(_lib().li_ini_get_int(self._h, section.encode(), key.encode(), int(default),
+                                   ctypes.byref(out)), "ini get_int")
+        return out.value
+
+    def get_bool(self, section, key, default):
+        out = ctypes.c_int32()
+        _chk(_lib().li_ini_get_bool(self._h, section.encode(), key.encode(), int(bool(default)),
+                                    ctypes.byref(out)), "ini get_bool")
+        return bool(out.value)
+
+    def set(self, section, key, value):
+        _chk(_lib().li_ini_set(self._h, section.encode(), key.encode(), str(value).encode()),
+             "ini set")
+
+    def save(self, path):
+        _chk(_lib().li_ini_save(self._h, _path(path)), f"write_ini({path})")
+
+    def entries(self):
+        """{section: {key: value}} of every entry."""
+        L = _lib()
+        need = _i64()
+        _chk(L.li_ini_dump(self._h, None, 0, ctypes.byref(need)), "ini dump")
+        buf = ctypes.create_string_buffer(max(need.value, 1))
+        _chk(L.li_ini_dump(self._h, buf, need.value, ctypes.byref(need)), "ini dump")
+        out = {}
+        for line in buf.value.decode().splitlines():
+            sec, key, val = line.split("\t", 2)
+            out.setdefault(sec, {})[key] = val
+        return out

@@ -1,7 +1,8 @@
 #!/bin/sh
 # Copies the Forrestania data files of the reference checkout into this folder, as test
 # fixtures: the eight unit surfaces and the label grid of the baseline model, the unit
-# property table, and the gravity stations with the magnetic data sampled at them.
+# property table, the gravity stations with the magnetic data sampled at them, and the
+# two run configuration files of the Forrestania app.
 # Data only; nothing under the reference is run.
 set -e
 REF=${REF:-/root/reference}
@@ -14,4 +15,5 @@ cp "$BASE/meshes/label_grid.bin" "$HERE/"
 cp "$BASE/cluster_properties.csv" "$HERE/"
 cp "$REF/datasets/Forrestania/Forrestania_Gravity_Station_trim_.csv" "$HERE/"
 cp "$REF/datasets/Forrestania/Forrestania_mag_at_gravity_stations.csv" "$HERE/"
-chmod u+w "$HERE"/*.ts "$HERE"/*.bin "$HERE"/*.csv
+cp "$REF/apps/forrestania/test_config.ini" "$REF/apps/forrestania/test_config_real.ini" "$HERE/"
+chmod u+w "$HERE"/*.ts "$HERE"/*.bin "$HERE"/*.csv "$HERE"/*.ini
