@@ -715,10 +715,103 @@ __global__ __launch_bounds__(128) void k_facet_tangent(
     out[kSeeds] = T(0);
 }
 
-// lane = station, facets of the block's chunk wave-uniform: per pair the seeds of a unit
-// gz weight and of a unit TMI weight, each contracted with the facet's dq in T and added
-// into the lane's FP64 sums (a closed surface's facet terms cancel against each other);
-// one partial per (facet chunk, station), summed by k_reduce_chunks.
+// One pair's change of gz and TMI along the facet constants' tangent dq (forward mode):
+// the forward of pair_seeds, operation for operation, with each quantity's tangent
+// beside it, so that dgz = sum_m seeds_g[m] dq[m] for pair_seeds' own seeds.  The
+// station offset moves with the translation and rotation tangents,
+//   dp = dq[GP] + (py w_xy + pz w_xz, -px w_xy + pz w_yz, -px w_xz - py w_yz)
+// (the transposes of the torque seeds A_jk = s_j p_k - s_k p_j), an in-plane edge
+// direction turns by its angle tangent (d e = dphi (-e_v, e_u)), and N takes 2A's own
+// tangent.  Near edges use the case forms' partial derivatives from edge_ln_grad.
+template <class T>
+__device__ __forceinline__ void pair_tangent(const FaceFrame<T>& F, double ox, double oy,
+                                             double oz, const T* __restrict__ q, T& dgz,
+                                             T& dtmi)
+{
+    const T Px = T(ox - F.ax), Py = T(oy - F.ay), Pz = T(oz - F.az);
+    const T px = fma(Px, F.ux, fma(Py, F.uy, fma(Pz, F.uz, F.du)));
+    const T py = fma(Px, F.vx, fma(Py, F.vy, fma(Pz, F.vz, F.dv)));
+    const T pz = fma(Px, F.nx, fma(Py, F.ny, fma(Pz, F.nz, F.dn)));
+    const T dpx = fma(py, q[S_AXY], fma(pz, q[S_AXZ], q[S_GPX]));
+    const T dpy = fma(pz, q[S_AYZ], fma(-px, q[S_AXY], q[S_GPY]));
+    const T dpz = fma(-py, q[S_AYZ], fma(-px, q[S_AXZ], q[S_GPZ]));
+    const T pz2 = pz * pz, dpz2 = T(2) * pz * dpz;
+    const T rho0 = fma(py, py, pz2), drho0 = fma(T(2) * py, dpy, dpz2);
+    const T da2 = fma(px, px, rho0), dda2 = fma(T(2) * px, dpx, drho0);
+    const T t = F.l0 - px, dt = q[S_L0] - dpx;
+    const T db2 = fma(t, t, rho0), ddb2 = fma(T(2) * t, dt, drho0);
+    const T dcx = F.cu - px, dcy = F.cv - py;
+    const T ddcx = q[S_CU] - dpx, ddcy = q[S_CV] - dpy;
+    const T dc2 = fma(dcx, dcx, fma(dcy, dcy, pz2));
+    const T ddc2 = fma(T(2) * dcx, ddcx, fma(T(2) * dcy, ddcy, dpz2));
+    const T rda = fast_rsqrt(da2), rdb = fast_rsqrt(db2), rdc = fast_rsqrt(dc2);
+    const T da = da2 * rda, db = db2 * rdb, dc = dc2 * rdc;
+    const T dda = T(0.5) * dda2 * rda, ddb = T(0.5) * ddb2 * rdb, ddc = T(0.5) * ddc2 * rdc;
+    const T rab = da2 - F.l0 * px;
+    const T drab = dda2 - (q[S_L0] * px + F.l0 * dpx);
+    const T rac = da2 - (F.cu * px + F.cv * py);
+    const T drac = dda2 - (q[S_CU] * px + F.cu * dpx + q[S_CV] * py + F.cv * dpy);
+    const T rbc = fma(t, dcx, fma(-py, dcy, pz2));
+    const T drbc = dt * dcx + t * ddcx - dpy * dcy - py * ddcy + dpz2;
+    const T D = da * db * dc + rab * dc + rac * db + rbc * da;
+    const T dD = dda * (db * dc + rbc) + ddb * (da * dc + rac) + ddc * (da * db + rab) +
+                 drab * dc + drac * db + drbc * da;
+    const T twoA = F.l0 * F.cv;
+    const T N = -twoA * pz;
+    const T dN = -(q[S_TWOA] * pz + twoA * dpz);
+    T th, dthN, dthD;
+    atan2_grad(N, D, th, dthN, dthD);
+    const T dth = dthN * dN + dthD * dD;
+    const T dp1 = t * F.e1v + py * F.e1u;
+    const T dp2 = dcx * F.e2v - dcy * F.e2u;
+    const T s1b = t * F.e1u - py * F.e1v;
+    const T s1c = dcx * F.e2u + dcy * F.e2v;
+    const T ddp1 = dt * F.e1v + dpy * F.e1u + s1b * q[S_PHI1];
+    const T ddp2 = ddcx * F.e2v - ddcy * F.e2u + s1c * q[S_PHI2];
+    const T q0 = fast_rcp(da + db), q1 = fast_rcp(db + dc), q2 = fast_rcp(dc + da);
+    const T z0 = F.l0 * q0, z1 = F.l1 * q1, z2 = F.l2 * q2;
+    T g0, g1, g2;
+    T lam0 = atanh2_dz(z0, g0);
+    T lam1 = atanh2_dz(z1, g1);
+    T lam2 = atanh2_dz(z2, g2);
+    // dz = q (dell - z (dd1 + dd2))
+    T dlam0 = g0 * q0 * (q[S_L0] - z0 * (dda + ddb));
+    T dlam1 = g1 * q1 * (q[S_L1] - z1 * (ddb + ddc));
+    T dlam2 = g2 * q2 * (q[S_L2] - z2 * (ddc + dda));
+    if (z0 > T(kZSeries) || z1 > T(kZSeries) || z2 > T(kZSeries)) {
+        T gs1, gs2, gd1, gd2, gr;
+        if (z0 > T(kZSeries)) {
+            // s1 = -px, s2 = t, d1 = da, d2 = db, rho^2 = rho0
+            lam0 = edge_ln_grad(-px, F.l0, da, db, rho0, gs1, gs2, gd1, gd2, gr);
+            dlam0 = -gs1 * dpx + gs2 * dt + gd1 * dda + gd2 * ddb + gr * drho0;
+        }
+        if (z1 > T(kZSeries)) {
+            // s1 = s1b, s2 = s1b + l1, d1 = db, d2 = dc, rho^2 = dp1^2 + pz^2
+            const T ds1 = dt * F.e1u - dpy * F.e1v - dp1 * q[S_PHI1];
+            lam1 = edge_ln_grad(s1b, F.l1, db, dc, dp1 * dp1 + pz2, gs1, gs2, gd1, gd2, gr);
+            dlam1 = gs1 * ds1 + gs2 * (ds1 + q[S_L1]) + gd1 * ddb + gd2 * ddc +
+                    gr * (T(2) * dp1 * ddp1 + dpz2);
+        }
+        if (z2 > T(kZSeries)) {
+            // s1 = s1c, s2 = s1c + l2, d1 = dc, d2 = da, rho^2 = dp2^2 + pz^2
+            const T ds1 = ddcx * F.e2u + ddcy * F.e2v - dp2 * q[S_PHI2];
+            lam2 = edge_ln_grad(s1c, F.l2, dc, da, dp2 * dp2 + pz2, gs1, gs2, gd1, gd2, gr);
+            dlam2 = gs1 * ds1 + gs2 * (ds1 + q[S_L2]) + gd1 * ddc + gd2 * dda +
+                    gr * (T(2) * dp2 * ddp2 + dpz2);
+        }
+    }
+    const T es = py * lam0 + dp1 * lam1 + dp2 * lam2;
+    const T des = dpy * lam0 + py * dlam0 + ddp1 * lam1 + dp1 * dlam1 + ddp2 * lam2 + dp2 * dlam2;
+    const T pzth = pz * th;
+    dgz = q[S_K1] * (es + pzth + pzth) + F.K1 * (des + T(2) * (dpz * th + pz * dth));
+    dtmi = q[S_W0] * th + F.W0 * dth + q[S_W1] * lam0 + F.W1 * dlam0 + q[S_W2] * lam1 +
+           F.W2 * dlam1 + q[S_W3] * lam2 + F.W3 * dlam2;
+}
+
+// lane = station, facets of the block's chunk wave-uniform: per pair the forward-mode
+// change of gz and TMI along the facet's dq (pair_tangent), added into the lane's FP64
+// sums (a closed surface's facet terms cancel against each other); one partial per
+// (facet chunk, station), summed by k_reduce_chunks.
 template <class T>
 __global__ __launch_bounds__(256) void k_jvp_pairs(
     const FaceFrame<T>* __restrict__ faces, const T* __restrict__ dq, int64_t n_f,
@@ -737,21 +830,10 @@ __global__ __launch_bounds__(256) void k_jvp_pairs(
     for (int64_t f = f0; f < f1; ++f) {
         const FaceFrame<T> F = faces[f];
         const T* __restrict__ q = dq + f * kSeedsPad;
-        // one weight at a time, not unrolled: the two seed sets are never live together
-#pragma unroll 1
-        for (int w = 0; w < 2; ++w) {
-            T a[kSeeds];
-#pragma unroll
-            for (int m = 0; m < kSeeds; ++m) a[m] = T(0);
-            pair_seeds(F, ox, oy, oz, T(w == 0 ? 1 : 0), T(w == 0 ? 0 : 1), a);
-            T p = T(0);
-#pragma unroll
-            for (int m = 0; m < kSeeds; ++m) p = fma(a[m], q[m], p);
-            if (w == 0)
-                g += (double)p;
-            else
-                t += (double)p;
-        }
+        T pg, pt;
+        pair_tangent(F, ox, oy, oz, q, pg, pt);
+        g += (double)pg;
+        t += (double)pt;
     }
     if (active) {
         part_gz[(int64_t)chunk * n_s + s] = g;
