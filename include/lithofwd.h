@@ -126,10 +126,10 @@ int lf_colnorm2(lf_ctx* ctx, int axis, int64_t n_dof, const int64_t* dof_off,
  * counterpart of J v with the reference's Jacobian (computeGravityAnalyticJacobian /
  * computeMagneticAnalyticJacobian, gravity_forward.cpp:184-292,
  * magnetic_forward.cpp:196-306, before the vertex -> DOF mapping), which its magnetic
- * weight calibration multiplies densely (joint_objective.cpp:281-393).  Same arithmetic
- * per pair as lf_adjoint (the pair's seeds for a unit gz / TMI weight), contracted with
- * the facet constants' tangents; FP64 sums over facets per station.  Either output may
- * be NULL. */
+ * weight calibration multiplies densely (joint_objective.cpp:281-393).  Per pair the
+ * forward of lf_adjoint's pair arithmetic with tangents (forward mode), equal to the
+ * contraction of that pair's adjoint seeds with the facet constants' tangents; FP64 sums
+ * over facets per station.  Either output may be NULL. */
 int lf_jvp(lf_ctx* ctx, const double* vtan, double* gz, double* tmi);
 /* Device-pointer variant (3*n_vertices tangent doubles, n_stations outputs each). */
 int lf_jvp_device(lf_ctx* ctx, const double* d_vtan, double* d_gz, double* d_tmi);
