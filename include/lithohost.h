@@ -269,6 +269,24 @@ int li_minimize_callbacks(int64_t n, const double* x0, const double* lb, const d
                           li_obj_cb f, li_grad_cb g, void* user, const LiOptions* o,
                           const double* h0, double* x_out, double* hist, int64_t hist_cap,
                           int64_t* n_hist, int32_t* iterations, char* msg, int64_t msg_cap);
+/* An L-BFGS-B optimizer that keeps its (s, y, rho) history between calls: the warm start
+ * of lbfgsb_optimizer.h:24-37.  set_clear_on_minimize(0) (setClearOnMinimize(false))
+ * keeps the history through li_lbfgsb_minimize_callbacks (options and outputs as
+ * li_minimize_callbacks); get/set_history move it between optimizers as count pairs,
+ * s and y count x n doubles oldest first (get: pass NULL arrays to read the count);
+ * restart clears it. */
+int li_lbfgsb_create(void** out);
+void li_lbfgsb_destroy(void* h);
+int li_lbfgsb_set_clear_on_minimize(void* h, int32_t clear);
+int li_lbfgsb_minimize_callbacks(void* h, int64_t n, const double* x0, const double* lb,
+                                 const double* ub, li_obj_cb f, li_grad_cb g, void* user,
+                                 const LiOptions* o, const double* h0, double* x_out,
+                                 double* hist, int64_t hist_cap, int64_t* n_hist,
+                                 int32_t* iterations, char* msg, int64_t msg_cap);
+int li_lbfgsb_get_history(void* h, int64_t n, int32_t* count, double* s, double* y, double* rho);
+int li_lbfgsb_set_history(void* h, int64_t n, int32_t count, const double* s, const double* y,
+                          const double* rho);
+int li_lbfgsb_restart(void* h);
 /* sets (cb non-NULL) or clears the problem's iteration callback */
 int li_problem_set_iteration_callback(void* h, li_iter_cb cb, void* user);
 int li_problem_minimize(void* h, const double* x0, const double* lb, const double* ub,

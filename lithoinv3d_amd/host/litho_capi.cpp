@@ -827,16 +827,20 @@ static int run_minimize(LBFGSBOptimizer::Objective obj, LBFGSBOptimizer::Gradien
                         int64_t n, const double* x0, const double* lb, const double* ub,
                         const LiOptions* o, const double* h0, double* x_out, double* hist,
                         int64_t hist_cap, int64_t* n_hist, int32_t* iterations, char* msg,
-                        int64_t msg_cap, li_iter_cb cb = nullptr, void* cb_user = nullptr)
+                        int64_t msg_cap, li_iter_cb cb = nullptr, void* cb_user = nullptr,
+                        LBFGSBOptimizer* shared = nullptr)
 {
     return guard([&] {
-        LBFGSBOptimizer opt;
+        LBFGSBOptimizer local;
+        LBFGSBOptimizer& opt = shared ? *shared : local;
         if (cb) {
             opt.setIterationCallback([cb, cb_user](const LBFGSBOptimizer::IterationRecord& r) {
                 double row[7];
                 minimize_row(r, row);
                 cb(0, row, 7, cb_user);
             });
+        } else {
+            opt.setIterationCallback(LBFGSBOptimizer::IterationCallback());
         }
         opt.setMaxIterations(o->max_iter);
         opt.setHistorySize(o->history);
@@ -873,6 +877,77 @@ int li_minimize_callbacks(int64_t n, const double* x0, const double* lb, const d
     };
     return run_minimize(obj, grad, n, x0, lb, ub, o, h0, x_out, hist, hist_cap, n_hist, iterations,
                         msg, msg_cap);
+}
+
+struct LbfgsbHandle {
+    LBFGSBOptimizer opt;
+};
+
+int li_lbfgsb_create(void** out)
+{
+    return guard([&] { *out = new LbfgsbHandle(); });
+}
+
+void li_lbfgsb_destroy(void* h) { delete (LbfgsbHandle*)h; }
+
+int li_lbfgsb_set_clear_on_minimize(void* h, int32_t clear)
+{
+    return guard([&] { ((LbfgsbHandle*)h)->opt.setClearOnMinimize(clear != 0); });
+}
+
+int li_lbfgsb_minimize_callbacks(void* h, int64_t n, const double* x0, const double* lb,
+                                 const double* ub, li_obj_cb f, li_grad_cb g, void* user,
+                                 const LiOptions* o, const double* h0, double* x_out,
+                                 double* hist, int64_t hist_cap, int64_t* n_hist,
+                                 int32_t* iterations, char* msg, int64_t msg_cap)
+{
+    auto obj = [&](const Vec& x) { return f(x.data(), (int64_t)x.size(), user); };
+    auto grad = [&](const Vec& x) {
+        Vec out(x.size());
+        g(x.data(), (int64_t)x.size(), out.data(), user);
+        return out;
+    };
+    return run_minimize(obj, grad, n, x0, lb, ub, o, h0, x_out, hist, hist_cap, n_hist, iterations,
+                        msg, msg_cap, nullptr, nullptr, &((LbfgsbHandle*)h)->opt);
+}
+
+int li_lbfgsb_get_history(void* h, int64_t n, int32_t* count, double* s, double* y, double* rho)
+{
+    return guard([&] {
+        std::vector<Vec> S, Y;
+        std::vector<double> R;
+        ((LbfgsbHandle*)h)->opt.getHistory(S, Y, R);
+        *count = (int32_t)S.size();
+        for (size_t k = 0; k < S.size(); ++k) {
+            if ((int64_t)S[k].size() != n)
+                throw std::invalid_argument("li_lbfgsb_get_history: n differs from the history's");
+            if (s) std::memcpy(s + k * n, S[k].data(), sizeof(double) * n);
+            if (y) std::memcpy(y + k * n, Y[k].data(), sizeof(double) * n);
+            if (rho) rho[k] = R[k];
+        }
+    });
+}
+
+int li_lbfgsb_set_history(void* h, int64_t n, int32_t count, const double* s, const double* y,
+                          const double* rho)
+{
+    return guard([&] {
+        if (count < 0 || n < 0 || (count > 0 && (!s || !y || !rho)))
+            throw std::invalid_argument("li_lbfgsb_set_history: invalid arguments");
+        std::vector<Vec> S(count), Y(count);
+        std::vector<double> R(count);
+        for (int32_t k = 0; k < count; ++k) {
+            S[k].assign(s + k * n, s + (k + 1) * n);
+            Y[k].assign(y + k * n, y + (k + 1) * n);
+            R[k] = rho[k];
+        }
+        ((LbfgsbHandle*)h)->opt.setHistory(S, Y, R);
+    });
+}
+
+int li_lbfgsb_restart(void* h)
+{
+    return guard([&] { ((LbfgsbHandle*)h)->opt.restart(); });
 }
 
 int li_problem_minimize(void* h, const double* x0, const double* lb, const double* ub,

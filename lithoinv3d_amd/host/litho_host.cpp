@@ -1783,9 +1783,14 @@ OptimizerResult LBFGSBOptimizer::minimize(Objective objective, Gradient gradient
     const auto t0 = clk::now();
     OptimizerResult res;
     m_iter = 0;
-    m_s.clear();
-    m_y.clear();
-    m_rho.clear();
+    if (m_clearOnMinimize) {
+        m_s.clear();
+        m_y.clear();
+        m_rho.clear();
+    }
+    for (const Vec& v : m_s)
+        if (v.size() != x0.size())
+            throw std::invalid_argument("LBFGSBOptimizer: history of another parameter count");
     m_hist.clear();
     int nObj = 0, nGrad = 0;
     auto fobj = [&](const Vec& x) { ++nObj; return objective(x); };
@@ -1921,6 +1926,25 @@ OptimizerResult LBFGSBOptimizer::minimize(Objective objective, Gradient gradient
     res.finalValue = f;
     res.iterations = m_iter;
     return res;
+}
+
+void LBFGSBOptimizer::setHistory(const std::vector<Vec>& s, const std::vector<Vec>& y,
+                                 const std::vector<double>& rho)
+{
+    if (s.size() != y.size() || s.size() != rho.size())
+        throw std::invalid_argument("LBFGSBOptimizer::setHistory: s, y and rho differ in length");
+    m_s = s;
+    m_y = y;
+    m_rho = rho;
+}
+
+void LBFGSBOptimizer::restart()
+{
+    // lbfgsb_optimizer.cpp:265-272; the next minimize() starts from its own x0
+    m_s.clear();
+    m_y.clear();
+    m_rho.clear();
+    m_iter = 0;
 }
 
 // =================================================================== GN-CG

@@ -571,6 +571,26 @@ public:
     // takes the full projected step without evaluating the objective; f is evaluated once
     // at the end (history records before that carry NaN; a target is not checked)
     void setDisableLineSearch(bool on) { m_noLineSearch = on; }
+    // Warm start (lbfgsb_optimizer.h:24-37): with clearOnMinimize false the (s, y, rho)
+    // pairs survive minimize() and the next run's first direction comes from them;
+    // getHistory / setHistory move them between optimizers (oldest first); restart
+    // clears them.  setReferenceObjective is kept for the interface: the reference stores
+    // the value and its convergence test does not read it (lbfgsb_optimizer.cpp:186-189).
+    void setClearOnMinimize(bool v) { m_clearOnMinimize = v; }
+    bool clearOnMinimize() const { return m_clearOnMinimize; }
+    void setHistory(const std::vector<Vec>& s, const std::vector<Vec>& y,
+                    const std::vector<double>& rho);
+    void getHistory(std::vector<Vec>& s, std::vector<Vec>& y, std::vector<double>& rho) const
+    {
+        s = m_s;
+        y = m_y;
+        rho = m_rho;
+    }
+    bool hasHistory() const { return !m_s.empty(); }
+    void restart();
+    int historySize() const { return m_historySize; }
+    void setReferenceObjective(double f) { m_fRef = f; }
+    double referenceObjective() const { return m_fRef; }
     // called with each iteration's record as it is appended (record 0: the start)
     using IterationCallback = std::function<void(const IterationRecord&)>;
     void setIterationCallback(IterationCallback cb) { m_onIteration = std::move(cb); }
@@ -588,6 +608,8 @@ private:
     bool m_hasTarget = false;
     bool m_activeSet = false;
     bool m_noLineSearch = false;
+    bool m_clearOnMinimize = true;
+    double m_fRef = 1.0;
     std::vector<IterationRecord> m_hist;
 };
 

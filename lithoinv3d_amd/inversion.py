@@ -142,6 +142,14 @@ def load_host():
             ctypes.POINTER(ctypes.c_int32), ctypes.c_char_p, i64]
     L.li_minimize_callbacks.argtypes = mins + [OBJ_CB, GRAD_CB, _P] + tail
     L.li_problem_minimize.argtypes = [_P, _P, _P, _P] + tail
+    L.li_lbfgsb_create.argtypes = [ctypes.POINTER(_P)]
+    L.li_lbfgsb_destroy.argtypes = [_P]
+    L.li_lbfgsb_destroy.restype = None
+    L.li_lbfgsb_set_clear_on_minimize.argtypes = [_P, ctypes.c_int32]
+    L.li_lbfgsb_minimize_callbacks.argtypes = [_P] + mins + [OBJ_CB, GRAD_CB, _P] + tail
+    L.li_lbfgsb_get_history.argtypes = [_P, i64, ctypes.POINTER(ctypes.c_int32), _P, _P, _P]
+    L.li_lbfgsb_set_history.argtypes = [_P, i64, ctypes.c_int32, _P, _P, _P]
+    L.li_lbfgsb_restart.argtypes = [_P]
     gtail = [ctypes.POINTER(_GncgOptions), _P, _P, i64, ctypes.POINTER(i64), _P, i64,
              ctypes.POINTER(ctypes.c_int32), ctypes.c_char_p, i64]
     L.li_gncg_minimize_callbacks.argtypes = mins + [OBJ_CB, GRAD_CB, _P] + gtail
@@ -485,6 +493,64 @@ def gncg_minimize(fun, grad, x0, lb, ub, max_iter=100, cg_max_iter=50, cg_tol=1e
     o = _gncg_opts(max_iter, cg_max_iter, cg_tol, fd_step, ftol, c1, line_search_max_iter)
     return _gncg_finish(L.li_gncg_minimize_callbacks, n, o, n, _p(x0), _p(lb), _p(ub), fcb,
                         gcb, None)
+
+
+class LBFGSB:
+    """An L-BFGS-B optimizer object with the reference's warm start
+    (lbfgsb_optimizer.h:24-37): clear_on_minimize=False keeps the (s, y, rho) history
+    from one minimize() to the next; history() / set_history() move it between
+    optimizers; restart() clears it."""
+
+    def __init__(self, clear_on_minimize=True):
+        L = load_host()
+        self._h = ctypes.c_void_p()
+        _chk(L.li_lbfgsb_create(ctypes.byref(self._h)), "lbfgsb create")
+        _chk(L.li_lbfgsb_set_clear_on_minimize(self._h, int(bool(clear_on_minimize))),
+             "lbfgsb clear")
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            load_host().li_lbfgsb_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def minimize(self, fun, grad, x0, lb, ub, max_iter=100, history=10, ftol=1e-6, c1=1e-4,
+                 target=None, h0=None, active_set=False, disable_line_search=False):
+        L = load_host()
+        x0, lb, ub = _a(x0), _a(lb), _a(ub)
+        n = len(x0)
+
+        def f_cb(xp, nn, user):
+            return float(fun(np.ctypeslib.as_array(xp, (nn,)).copy()))
+
+        def g_cb(xp, nn, gp, user):
+            g = np.asarray(grad(np.ctypeslib.as_array(xp, (nn,)).copy()), dtype=np.float64)
+            np.ctypeslib.as_array(gp, (nn,))[:] = g
+
+        fcb, gcb = OBJ_CB(f_cb), GRAD_CB(g_cb)
+        h = None if h0 is None else _a(h0)
+        o = _opts(max_iter, history, ftol, c1, target, h, active_set, disable_line_search)
+        return _finish(L, L.li_lbfgsb_minimize_callbacks, n, max_iter, self._h, n, _p(x0),
+                       _p(lb), _p(ub), fcb, gcb, None, ctypes.byref(o), _p(h))
+
+    def history(self, n):
+        """(s, y, rho): the stored pairs, oldest first, for n parameters."""
+        L = load_host()
+        c = ctypes.c_int32()
+        _chk(L.li_lbfgsb_get_history(self._h, n, ctypes.byref(c), None, None, None), "history")
+        s, y, r = np.zeros((c.value, n)), np.zeros((c.value, n)), np.zeros(c.value)
+        _chk(L.li_lbfgsb_get_history(self._h, n, ctypes.byref(c), _p(s), _p(y), _p(r)),
+             "history")
+        return s, y, r
+
+    def set_history(self, s, y, rho):
+        s, y, rho = _a(s), _a(y), _a(rho)
+        if s.shape != y.shape or s.ndim != 2 or len(rho) != len(s):
+            raise ValueError("set_history: s and y (count, n) and rho (count) must agree")
+        _chk(load_host().li_lbfgsb_set_history(self._h, s.shape[1] if s.ndim == 2 else 0,
+                                               len(rho), _p(s), _p(y), _p(rho)), "set_history")
+
+    def restart(self):
+        _chk(load_host().li_lbfgsb_restart(self._h), "restart")
 
 
 def minimize(fun, grad, x0, lb, ub, max_iter=100, history=10, ftol=1e-6, c1=1e-4,

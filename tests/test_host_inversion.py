@@ -504,3 +504,43 @@ def test_disabled_line_search_full_steps():
     r2 = inversion.minimize(f, g, np.array([4.0]), np.array([-10.0]), np.array([10.0]),
                             max_iter=10)
     assert r2.x[0] == 0.0 and r2.iterations == 3 and calls["f"] == 3
+
+
+def test_lbfgsb_warm_start():
+    """The warm start of lbfgsb_optimizer.h:24-37: with clear_on_minimize off, two runs of
+    3 iterations take the same iterates as one run of 6 (the second run's first direction
+    comes from the kept pairs); the pairs moved to another optimizer with set_history give
+    that optimizer the same continuation; the default cold start begins the second run
+    with steepest descent again; restart clears the pairs."""
+    rng = np.random.default_rng(11)
+    n = 6
+    Q = rng.normal(size=(n, n))
+    A = Q @ Q.T + n * np.eye(n)
+    b = rng.normal(size=n)
+    f = lambda x: 0.5 * x @ A @ x - b @ x
+    g = lambda x: A @ x - b
+    lo, hi = np.full(n, -50.0), np.full(n, 50.0)
+    x0 = np.zeros(n)
+    kw = dict(max_iter=3, ftol=1e-14)
+    one = inversion.minimize(f, g, x0, lo, hi, max_iter=6, ftol=1e-14)
+    warm = inversion.LBFGSB(clear_on_minimize=False)
+    r1 = warm.minimize(f, g, x0, lo, hi, **kw)
+    s, y, rho = warm.history(n)
+    assert len(rho) == 3 and np.allclose(rho, 1.0 / np.sum(s * y, 1), rtol=1e-14)
+    r2 = warm.minimize(f, g, r1.x, lo, hi, **kw)
+    np.testing.assert_array_equal(r2.x, one.x)
+    np.testing.assert_array_equal(r2.history[1:, 1], one.history[4:, 1])
+    moved = inversion.LBFGSB(clear_on_minimize=False)
+    moved.set_history(s, y, rho)
+    r3 = moved.minimize(f, g, r1.x, lo, hi, **kw)
+    np.testing.assert_array_equal(r3.x, r2.x)
+    cold = inversion.LBFGSB()
+    cold.set_history(s, y, rho)
+    r4 = cold.minimize(f, g, r1.x, lo, hi, **kw)  # clears first: steepest descent again
+    r5 = inversion.minimize(f, g, r1.x, lo, hi, **kw)
+    np.testing.assert_array_equal(r4.x, r5.x)
+    assert np.abs(r4.x - r2.x).max() > 1e-9
+    warm.restart()
+    assert len(warm.history(n)[2]) == 0
+    with pytest.raises(Exception):
+        moved.set_history(s, y, rho[:2])
