@@ -258,6 +258,11 @@ class ForwardContext:
         self._check(self._lib.lf_jvp(self._ctx, _ptr(v), _ptr(gz), _ptr(tmi)), "lf_jvp")
         return gz, tmi
 
+    def jvp_device(self, vtan_ptr: int, gz_ptr: int = 0, tmi_ptr: int = 0):
+        self._check(self._lib.lf_jvp_device(self._ctx, ctypes.c_void_p(vtan_ptr),
+                                            ctypes.c_void_p(gz_ptr or None),
+                                            ctypes.c_void_p(tmi_ptr or None)), "lf_jvp_device")
+
     def colnorm2(self, axis, dof_off, dof_vertex, dof_weight):
         """(sum_i (dgz_i/dp_j)^2, sum_i (dTMI_i/dp_j)^2) for parameters p_j that move
         coordinate `axis` of vertices dof_vertex[dof_off[j]:dof_off[j+1]] by the weights."""

@@ -4,6 +4,9 @@ the vertex positions, with the hand-written adjoint kernel as its backward.
     gz, tmi = polyhedral_forward(vertices, ctx)      # vertices: (n_v, 3) float64 on cuda
     (w_gz @ gz + w_tmi @ tmi).backward()             # vertices.grad = adjoint
 
+and the Jacobian-vector product kernel as its forward-mode derivative
+(torch.autograd.forward_ad: a dual vertex tensor gives the tangents of gz and TMI).
+
 The library runs on its own HIP stream; tensors are handed over after the current torch
 stream has finished with them and handed back after the library stream is done.
 """
@@ -31,6 +34,7 @@ class PolyhedralForward(torch.autograd.Function):
         fctx.sync()
         ctx.fctx = fctx
         ctx.save_for_backward(v)
+        ctx.save_for_forward(v)
         return gz, tm
 
     @staticmethod
@@ -47,6 +51,21 @@ class PolyhedralForward(torch.autograd.Function):
         fctx.adjoint_device(wg.data_ptr(), wt.data_ptr(), grad.data_ptr())
         fctx.sync()
         return grad.view(fctx.n_v, 3), None
+
+    @staticmethod
+    def jvp(ctx, v_t, _):
+        fctx = ctx.fctx
+        (v,) = ctx.saved_tensors
+        dev = v.device
+        vt = v_t.to(torch.float64).contiguous()
+        gz_t = torch.zeros(fctx.n_s, dtype=torch.float64, device=dev)
+        tm_t = torch.zeros(fctx.n_s, dtype=torch.float64, device=dev)
+        torch.cuda.current_stream().synchronize()
+        # the tangent is taken at the geometry of this forward
+        fctx.set_vertices_device(v.data_ptr())
+        fctx.jvp_device(vt.data_ptr(), gz_t.data_ptr(), tm_t.data_ptr())
+        fctx.sync()
+        return gz_t, tm_t
 
 
 def polyhedral_forward(vertices: torch.Tensor, fctx: ForwardContext):

@@ -144,3 +144,29 @@ def test_problem_jvp_and_calibration():
             assert exact[key] == pytest.approx(fd[key], rel=1e-4 if prec == "fp32c" else 1e-6)
     assert rel(res["fp32c"][0], res["fp64"][0]) < 1e-4
     assert rel(res["fp32c"][1], res["fp64"][1]) < 1e-4
+
+
+def test_torch_forward_mode_bridge():
+    """torch.autograd.forward_ad through polyhedral_forward: the tangents of gz and TMI
+    are lf_jvp's (the same kernel, bit for bit), and the backward still gives the
+    adjoint."""
+    import torch
+    import torch.autograd.forward_ad as fwAD
+
+    from lithoinv3d_amd.torch_ops import polyhedral_forward
+
+    m, st = model_and_stations(7)
+    fh = synth.field_unit_vector(INC, DEC)
+    rng = np.random.default_rng(8)
+    vt = rng.normal(size=(m.n_vertices, 3))
+    say("torch forward-mode bridge")
+    with ForwardContext(m, st, fh, FIELD, precision="fp64") as ctx:
+        jg, jt = ctx.jvp(vt)
+        v = torch.tensor(m.vertices, dtype=torch.float64, device="cuda")
+        t = torch.tensor(vt, dtype=torch.float64, device="cuda")
+        with fwAD.dual_level():
+            gz, tm = polyhedral_forward(fwAD.make_dual(v, t), ctx)
+            tg = fwAD.unpack_dual(gz).tangent.cpu().numpy()
+            tt = fwAD.unpack_dual(tm).tangent.cpu().numpy()
+    np.testing.assert_array_equal(tg, jg)
+    np.testing.assert_array_equal(tt, jt)
