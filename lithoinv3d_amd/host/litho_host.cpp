@@ -1938,7 +1938,7 @@ void LBFGSBOptimizer::setHistory(const std::vector<Vec>& s, const std::vector<Ve
     m_rho = rho;
 }
 
-void LBFGSBOptimizer::restart()
+void LBFGSBOptimizer::restart(const Vec&)
 {
     // lbfgsb_optimizer.cpp:265-272; the next minimize() starts from its own x0
     m_s.clear();

@@ -540,23 +540,38 @@ struct OptimizerResult {
     std::string message;
 };
 
+// The interface of the geometry solvers (optimizer.h:16-36): minimize over a box, restart
+// (drops what the solver remembers; the next minimize starts from its own x0, so
+// newParams changes nothing else, as in the reference), the iteration limit and count,
+// the projected-gradient tolerance.
+class Optimizer {
+public:
+    using Objective = std::function<double(const Vec&)>;
+    using Gradient = std::function<Vec(const Vec&)>;
+    virtual ~Optimizer() = default;
+    virtual OptimizerResult minimize(Objective objective, Gradient gradient, const Vec& x0,
+                                     const Vec& lb, const Vec& ub) = 0;
+    virtual void restart(const Vec& newParams = Vec()) = 0;
+    virtual void setMaxIterations(int n) = 0;
+    virtual int iterationCount() const = 0;
+    virtual void setTolerance(double t) = 0;
+};
+
 // Projected L-BFGS with Armijo backtracking: modules/litho-inversion/src/inversion/
 // lbfgsb_optimizer.cpp:13-263, with an optional diagonal initial inverse Hessian D:
 // H0 = gamma * D, gamma = s^T y / (y^T D y) (gamma * I with D = I, :40-46), and a first
 // step along -D g.
-class LBFGSBOptimizer {
+class LBFGSBOptimizer : public Optimizer {
 public:
-    using Objective = std::function<double(const Vec&)>;
-    using Gradient = std::function<Vec(const Vec&)>;
     struct IterationRecord {
         int iteration;
         double f, pgNorm, alpha, seconds;
         int objectiveCalls, gradientCalls;
     };
     OptimizerResult minimize(Objective objective, Gradient gradient, const Vec& x0,
-                             const Vec& lb, const Vec& ub);
-    void setMaxIterations(int n) { m_maxIter = n; }
-    void setTolerance(double t) { m_ftol = t; }
+                             const Vec& lb, const Vec& ub) override;
+    void setMaxIterations(int n) override { m_maxIter = n; }
+    void setTolerance(double t) override { m_ftol = t; }
     void setHistorySize(int m) { m_historySize = m; }
     void setArmijoC1(double c) { m_c1 = c; }
     void setLineSearchMaxIter(int n) { m_lsMax = n; }
@@ -587,14 +602,14 @@ public:
         rho = m_rho;
     }
     bool hasHistory() const { return !m_s.empty(); }
-    void restart();
+    void restart(const Vec& newParams = Vec()) override;
     int historySize() const { return m_historySize; }
     void setReferenceObjective(double f) { m_fRef = f; }
     double referenceObjective() const { return m_fRef; }
     // called with each iteration's record as it is appended (record 0: the start)
     using IterationCallback = std::function<void(const IterationRecord&)>;
     void setIterationCallback(IterationCallback cb) { m_onIteration = std::move(cb); }
-    int iterationCount() const { return m_iter; }
+    int iterationCount() const override { return m_iter; }
     const std::vector<IterationRecord>& history() const { return m_hist; }
 
 private:
@@ -620,26 +635,25 @@ private:
 // projected onto the bounds), falls back to -g when d is not a descent direction, and
 // halves the step from 1 until the projected point does not raise the objective.  The
 // Armijo constant is kept for the interface; the reference's line search does not use it.
-class GNCGOptimizer {
+class GNCGOptimizer : public Optimizer {
 public:
-    using Objective = std::function<double(const Vec&)>;
-    using Gradient = std::function<Vec(const Vec&)>;
     // one row per iterate x_k once both f(x_k) and the gradient there are known; the
     // final row, written when the loop ends at a point whose gradient was not evaluated,
     // has pgNorm = NaN.  alpha: the step that reached x_k.
     using IterationRecord = LBFGSBOptimizer::IterationRecord;
     using IterationCallback = std::function<void(const IterationRecord&)>;
     OptimizerResult minimize(Objective objective, Gradient gradient, const Vec& x0,
-                             const Vec& lb, const Vec& ub);
-    void setMaxIterations(int n) { m_maxIter = n; }
-    void setTolerance(double t) { m_ftol = t; }
+                             const Vec& lb, const Vec& ub) override;
+    void restart(const Vec& = Vec()) override { m_iter = 0; }
+    void setMaxIterations(int n) override { m_maxIter = n; }
+    void setTolerance(double t) override { m_ftol = t; }
     void setCGMaxIter(int n) { m_cgMaxIter = n; }
     void setCGTolerance(double t) { m_cgTol = t; }
     void setFDStep(double h) { m_fdStep = h; }
     void setArmijoC1(double c) { m_c1 = c; }
     void setLineSearchMaxIter(int n) { m_lsMax = n; }
     void setIterationCallback(IterationCallback cb) { m_onIteration = std::move(cb); }
-    int iterationCount() const { return m_iter; }
+    int iterationCount() const override { return m_iter; }
     // inner CG iterations of each outer iteration, in order
     const std::vector<int>& cgIterations() const { return m_cgIters; }
     const std::vector<IterationRecord>& history() const { return m_hist; }

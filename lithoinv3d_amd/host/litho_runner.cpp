@@ -456,6 +456,27 @@ RunnerResult runInversion(std::shared_ptr<LithologyModel> model, RunnerProblem& 
             << base.nonAdjacent << "\n";
     }
 
+    // the geometry solver of the configuration (runner.cpp:191-212), one object for every
+    // phase as there; each minimize starts afresh (L-BFGS-B clears its pairs)
+    std::unique_ptr<Optimizer> geoOptimizer;
+    if (opt.solver == RunnerOptions::Solver::GNCG) {
+        auto gn = std::make_unique<GNCGOptimizer>();
+        gn->setCGMaxIter(opt.gncgCGMaxIter);
+        gn->setCGTolerance(opt.gncgCGTolerance);
+        gn->setFDStep(opt.fdStep);
+        gn->setArmijoC1(opt.armijoC1);
+        gn->setLineSearchMaxIter(opt.lineSearchMaxIter);
+        gn->setTolerance(opt.tolerance);
+        geoOptimizer = std::move(gn);
+    } else {
+        auto lb = std::make_unique<LBFGSBOptimizer>();
+        lb->setHistorySize(opt.lbfgsHistory);
+        lb->setTolerance(opt.tolerance);
+        lb->setArmijoC1(opt.armijoC1);
+        lb->setLineSearchMaxIter(opt.lineSearchMaxIter);
+        lb->setDisableLineSearch(opt.disableLineSearch);
+        geoOptimizer = std::move(lb);
+    }
     int total = 0;
     int exportCounter = 0;
     bool converged = false;
@@ -464,27 +485,8 @@ RunnerResult runInversion(std::shared_ptr<LithologyModel> model, RunnerProblem& 
         if (opt.geometry) {
             const int phase = std::min(opt.propertyInterval, opt.maxIterations - total);
             if (phase <= 0) break;
-            OptimizerResult geo;
-            if (opt.solver == RunnerOptions::Solver::GNCG) {
-                GNCGOptimizer gn;
-                gn.setMaxIterations(phase);
-                gn.setCGMaxIter(opt.gncgCGMaxIter);
-                gn.setCGTolerance(opt.gncgCGTolerance);
-                gn.setFDStep(opt.fdStep);
-                gn.setArmijoC1(opt.armijoC1);
-                gn.setLineSearchMaxIter(opt.lineSearchMaxIter);
-                gn.setTolerance(opt.tolerance);
-                geo = gn.minimize(geoObj, geoGrad, params, lower, upper);
-            } else {
-                LBFGSBOptimizer lb;
-                lb.setMaxIterations(phase);
-                lb.setHistorySize(opt.lbfgsHistory);
-                lb.setTolerance(opt.tolerance);
-                lb.setArmijoC1(opt.armijoC1);
-                lb.setLineSearchMaxIter(opt.lineSearchMaxIter);
-                lb.setDisableLineSearch(opt.disableLineSearch);
-                geo = lb.minimize(geoObj, geoGrad, params, lower, upper);
-            }
+            geoOptimizer->setMaxIterations(phase);
+            const OptimizerResult geo = geoOptimizer->minimize(geoObj, geoGrad, params, lower, upper);
             log << "geometry phase"
                 << (opt.solver == RunnerOptions::Solver::GNCG ? " (gncg)" : "") << ": "
                 << geo.iterations << " iterations, f = " << geo.finalValue
