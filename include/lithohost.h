@@ -325,6 +325,22 @@ int li_problem_gncg_minimize(void* h, const double* x0, const double* lb, const 
                              int64_t hist_cap, int64_t* n_hist, int32_t* cg_iters,
                              int64_t cg_cap, int32_t* iterations, char* msg, int64_t msg_cap);
 
+/* ConstraintHandler (constraint_handler.cpp:15-66), the reference's discrete borehole
+ * penalty: weight * sum of v^2, v = |z_bottom - z_top| when the interval's midpoint
+ * (x, y, -(z_top + z_bottom) / 2; depths positive down) does not classify (as
+ * li_model_classify, label_grid may be NULL) into `unit`, else 0; gradient by central
+ * differences with a step of 1 parameter unit, times the weight again as there.
+ * xy: n (x, y) collars.  li_model_constraint_penalty evaluates it at params (violations
+ * and grad may be NULL); li_problem_add_constraints adds it to the problem's objective
+ * (the grid is copied) */
+int li_model_constraint_penalty(void* h, void* label_grid, double bottom_depth, int64_t n,
+                                const double* xy, const double* z_top, const double* z_bottom,
+                                const int32_t* unit, double weight, const double* params,
+                                double* value, double* violations, double* grad);
+int li_problem_add_constraints(void* h, void* label_grid, double bottom_depth, int64_t n,
+                               const double* xy, const double* z_top, const double* z_bottom,
+                               const int32_t* unit, double weight);
+
 /* ---- run configuration files (modules/litho-io/src/io/ini_config.cpp) ---- */
 
 /* IniConfig::load ("[section]", "key = value" or "key: value", '#' / ';' comments);

@@ -436,6 +436,55 @@ int li_problem_add_boreholes(void* h, int64_t n, const double* xyz, const int32_
     });
 }
 
+namespace {
+std::vector<ConstraintRecord> make_constraints(int64_t n, const double* xy, const double* z_top,
+                                               const double* z_bottom, const int32_t* unit)
+{
+    std::vector<ConstraintRecord> c((size_t)n);
+    for (int64_t k = 0; k < n; ++k)
+        c[(size_t)k] = {xy[2 * k], xy[2 * k + 1], z_top[k], z_bottom[k], unit[k]};
+    return c;
+}
+}  // namespace
+
+int li_model_constraint_penalty(void* h, void* label_grid, double bottom_depth, int64_t n,
+                                const double* xy, const double* z_top, const double* z_bottom,
+                                const int32_t* unit, double weight, const double* params,
+                                double* value, double* violations, double* grad)
+{
+    return guard([&] {
+        auto& m = ((ModelHandle*)h)->m;
+        ConstraintHandler c(m, make_constraints(n, xy, z_top, z_bottom, unit),
+                            label_grid ? &((GridHandle*)label_grid)->g : nullptr, bottom_depth);
+        c.setWeight(weight);
+        const Vec p(params, params + m->totalDofCount());
+        *value = c.evaluate(p);
+        if (violations) {
+            const Vec v = c.violations();
+            std::memcpy(violations, v.data(), sizeof(double) * v.size());
+        }
+        if (grad) {
+            const Vec g = c.gradient(p);
+            std::memcpy(grad, g.data(), sizeof(double) * g.size());
+        }
+    });
+}
+
+int li_problem_add_constraints(void* h, void* label_grid, double bottom_depth, int64_t n,
+                               const double* xy, const double* z_top, const double* z_bottom,
+                               const int32_t* unit, double weight)
+{
+    return guard([&] {
+        auto* P = (Problem*)h;
+        auto c = std::make_shared<ConstraintHandler>(
+            P->model, make_constraints(n, xy, z_top, z_bottom, unit),
+            label_grid ? &((GridHandle*)label_grid)->g : nullptr, bottom_depth);
+        c->setWeight(weight);
+        P->gobj->addRegularization(c);
+        if (P->fused) P->fused->addRegularization(c);
+    });
+}
+
 // The property phase (runner.cpp:708-935): which = 0 densities from gravity, 1
 // susceptibilities from TMI; the new unit properties go to out (n_units) and into the
 // model, and every forward of the problem takes them.

@@ -570,6 +570,51 @@ RunnerResult runInversion(std::shared_ptr<LithologyModel> model, RunnerProblem& 
     return res;
 }
 
+ConstraintHandler::ConstraintHandler(std::shared_ptr<LithologyModel> model,
+                                     std::vector<ConstraintRecord> c, const LabelGridData* grid,
+                                     double bottomDepth)
+    : m_model(std::move(model)), m_c(std::move(c)), m_hasGrid(grid != nullptr),
+      m_bottom(bottomDepth)
+{
+    if (grid) m_grid = *grid;
+}
+
+Vec ConstraintHandler::violations() const
+{
+    PointClassifier cls(*m_model, m_hasGrid ? &m_grid : nullptr, m_bottom);
+    Vec v(m_c.size(), 0.0);
+    for (size_t k = 0; k < m_c.size(); ++k) {
+        const ConstraintRecord& c = m_c[k];
+        const int unit = cls.classify(c.x, c.y, -0.5 * (c.z_top + c.z_bottom));
+        if (unit != c.unit) v[k] = std::abs(c.z_bottom - c.z_top);
+    }
+    return v;
+}
+
+double ConstraintHandler::evaluate(const Vec& params)
+{
+    m_model->applyParameterVector(params);
+    double pen = 0.0;
+    for (double v : violations()) pen += v * v;
+    return m_weight * pen;
+}
+
+Vec ConstraintHandler::gradient(const Vec& params)
+{
+    const double h = 1.0;
+    Vec p = params, g(params.size(), 0.0);
+    for (size_t j = 0; j < p.size(); ++j) {
+        p[j] = params[j] + h;
+        const double fp = evaluate(p);
+        p[j] = params[j] - h;
+        const double fm = evaluate(p);
+        p[j] = params[j];
+        g[j] = m_weight * (fp - fm) / (2.0 * h);
+    }
+    m_model->applyParameterVector(params);
+    return g;
+}
+
 SurfaceMeshData unitClosedMesh(const LithologyModel& model, int unit)
 {
     const auto& F = model.faces();

@@ -43,6 +43,32 @@ private:
     std::vector<int32_t> m_binFace;
 };
 
+// ConstraintHandler (modules/litho-inversion/src/inversion/constraint_handler.cpp:15-66),
+// the reference's discrete borehole penalty: weight * sum over intervals of v^2, with
+// v = |z_bottom - z_top| when the interval's midpoint (x, y, -(z_top + z_bottom) / 2;
+// depths positive down, z up) does not classify (PointClassifier at the applied
+// parameters) into the interval's unit, else 0.  Its gradient: central differences of the
+// weighted penalty with a step of 1 parameter unit over every parameter, multiplied by the
+// weight once more, as there (the weight enters the gradient squared).  With a label grid
+// the classification does not follow the geometry and the gradient is 0.  The model is
+// left at params.  BoreholeContactTerm is the differentiable alternative.
+class ConstraintHandler : public Regularization {
+public:
+    ConstraintHandler(std::shared_ptr<LithologyModel> model, std::vector<ConstraintRecord> c,
+                      const LabelGridData* grid = nullptr, double bottomDepth = -5000.0);
+    double evaluate(const Vec& params) override;
+    Vec gradient(const Vec& params) override;
+    // the unweighted violations v at the model's current geometry
+    Vec violations() const;
+
+private:
+    std::shared_ptr<LithologyModel> m_model;
+    std::vector<ConstraintRecord> m_c;
+    bool m_hasGrid = false;
+    LabelGridData m_grid;  // a copy: the term may outlive the caller's grid
+    double m_bottom;
+};
+
 // LithologyModel::validate (lithology_model.cpp:624-798).  With one shared vertex array
 // the shared-vertex-gap and edge-tearing checks (1, 4) hold by construction, and the
 // bounding-box check (5) compares each vertex with a box grown from the same vertices,

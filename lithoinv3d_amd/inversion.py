@@ -124,6 +124,11 @@ def load_host():
                                      ctypes.c_double, ctypes.POINTER(ctypes.c_double), _P, _P]
     L.li_problem_add_boreholes.argtypes = [_P, ctypes.c_int64, _P, _P, _P, _P, ctypes.c_double,
                                            ctypes.c_double]
+    L.li_model_constraint_penalty.argtypes = [_P, _P, ctypes.c_double, i64, _P, _P, _P, _P,
+                                              ctypes.c_double, _P,
+                                              ctypes.POINTER(ctypes.c_double), _P, _P]
+    L.li_problem_add_constraints.argtypes = [_P, _P, ctypes.c_double, i64, _P, _P, _P, _P,
+                                             ctypes.c_double]
     L.li_model_dof_map.argtypes = [_P, _P, _P, _P, ctypes.POINTER(i64)]
     L.li_problem_create.argtypes = [ctypes.POINTER(_P), _P, i64, _P, _P, _P, _P, _P,
                                     ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -260,6 +265,25 @@ class Model:
         out = np.zeros(self.n_dof)
         _chk(load_host().li_model_pullback(self._h, _p(g), _p(out)), "pullback")
         return out
+
+    def constraint_penalty(self, params, constraints, grid=None, weight=1.0,
+                           bottom_depth=-5000.0, gradient=False):
+        """The reference's discrete borehole penalty (ConstraintHandler) at params:
+        (value, per-interval violations, gradient or None).  constraints: a
+        litho_io.Constraints (collars xy, z_top, z_bottom positive down, unit)."""
+        c = constraints
+        xy, zt, zb = _a(c.xy).reshape(-1, 2), _a(c.z_top), _a(c.z_bottom)
+        un = _a(c.unit, np.int32)
+        n = len(zt)
+        p = _a(params)
+        val = ctypes.c_double()
+        viol = np.zeros(n)
+        g = np.zeros(self.n_dof) if gradient else None
+        _chk(load_host().li_model_constraint_penalty(
+            self._h, None if grid is None else grid._h, bottom_depth, n, _p(xy), _p(zt),
+            _p(zb), _p(un), weight, _p(p), ctypes.byref(val), _p(viol), _p(g)),
+            "constraint_penalty")
+        return val.value, viol, g
 
     def tangent(self, dparams):
         """Vertex tangent (n_v, 3) of the parameter tangent dparams: the linear part of
@@ -845,6 +869,17 @@ class Problem:
                                                       max_iter, tol, _p(out)),
              "invert_properties")
         return out if per == 1 else out.reshape(-1, per)
+
+    def add_constraints(self, constraints, grid=None, weight=1.0, bottom_depth=-5000.0):
+        """Adds the reference's discrete borehole penalty (ConstraintHandler, the runner's
+        omega as weight) to the objective; see Model.constraint_penalty."""
+        c = constraints
+        xy, zt, zb = _a(c.xy).reshape(-1, 2), _a(c.z_top), _a(c.z_bottom)
+        un = _a(c.unit, np.int32)
+        self._keep.append((xy, zt, zb, un))
+        _chk(load_host().li_problem_add_constraints(
+            self._h, None if grid is None else grid._h, bottom_depth, len(zt), _p(xy), _p(zt),
+            _p(zb), _p(un), weight), "add_constraints")
 
     def jvp(self, params, dparams):
         """(J_gz dp, J_TMI dp) at params: the data's change along the parameter tangent

@@ -92,3 +92,53 @@ def test_gradient_vs_central_differences(freedom, stride):
         e[j] = h
         fd = (M.boreholes(k, p + e, 0.7)[0] - M.boreholes(k, p - e, 0.7)[0]) / (2 * h)
         assert fd == pytest.approx(g[j], rel=1e-5, abs=1e-9 * np.abs(g).max())
+
+
+def test_reference_constraint_penalty():
+    """ConstraintHandler (constraint_handler.cpp:15-66), the reference's discrete term, at a
+    collar outside the bump (units 0 / 1 / 2 at depths 31-81 / 81-131 / 131-181 m):
+    intervals in their own unit cost nothing, an interval in the wrong unit its thickness
+    squared; the gradient is the central difference of the weighted penalty with a step
+    of one parameter unit, times the weight again, nonzero only where a 10 m move of the
+    0|1 contact next to the collar reclassifies the interval 0.5 m above it; with a label
+    grid the classification is the grid's and the gradient 0; the model is left at the
+    parameters."""
+    m = layered_model()
+    M = inversion.Model(m)
+    M.fix_exterior(inversion.Z_ONLY)
+    M.set_axis_scale(10.0)
+    p = M.params()
+    x, y = -60.3, -50.7
+    top = np.array([40.0, 90.0, 90.0, 140.0, 75.0])
+    bot = np.array([60.0, 120.0, 120.0, 170.0, 86.0])
+    unit = np.array([0, 1, 2, 1, 0])
+    c = IO.Constraints(np.tile([x, y], (5, 1)), top, bot, unit)
+    w = 0.5
+    val, viol, g = M.constraint_penalty(p, c, weight=w, gradient=True)
+    np.testing.assert_array_equal(viol, [0.0, 0.0, 30.0, 30.0, 0.0])
+    assert val == pytest.approx(w * (900.0 + 900.0), rel=1e-15)
+    np.testing.assert_array_equal(M.apply(p), m.vertices)  # left at p
+    nz = np.nonzero(g)[0]
+    assert 0 < len(nz) < 20
+    v, ax, _ = M.dofs()
+    X = m.vertices[v[nz]]
+    # only the 0|1 contact (z = -81) within a cell of the collar
+    assert np.all(np.abs(X[:, 2] + 81.0) < 1e-9)
+    assert np.all(np.abs(X[:, 0] - x) <= CELL) and np.all(np.abs(X[:, 1] - y) <= CELL)
+    for j in list(nz) + [0, len(p) - 1]:
+        e = np.zeros(len(p))
+        e[j] = 1.0
+        fp = M.constraint_penalty(p + e, c, weight=w)[0]
+        fm = M.constraint_penalty(p - e, c, weight=w)[0]
+        assert g[j] == pytest.approx(w * (fp - fm) / 2.0, rel=1e-15, abs=0.0)
+    # the label grid (k upward from its bottom) classifies the same at the start and
+    # does not follow the geometry
+    lab = np.zeros((8, 7, 6), np.int32)
+    lab[:, :, 2:] = 1
+    lab[3:5, 3:5, 2] = 0
+    lab[:, :, 4:] = 2
+    grid = inversion.LabelGrid(lab[:, :, ::-1], (-100.0, -87.5, TOP - 6 * CELL),
+                               (CELL, CELL, CELL))
+    val_g, viol_g, g_g = M.constraint_penalty(p, c, grid=grid, weight=w, gradient=True)
+    np.testing.assert_array_equal(viol_g, viol)
+    assert val_g == val and not np.any(g_g)
