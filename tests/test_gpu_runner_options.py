@@ -8,7 +8,9 @@ objective:
     full projected step along -g, clip(p0 - g(p0), lower, upper);
   * iterationExportDir / groupExportNames (runner.cpp:319-348, 571-575): one directory
     per accepted geometry phase holding every unit's closed surface at that phase's
-    geometry.
+    geometry;
+  * the reference's discrete constraint penalty (ConstraintHandler, weight omega) in the
+    objective: value and gradient add exactly.
 """
 import os
 
@@ -102,3 +104,30 @@ def test_runner_iteration_export(tmp_path):
         np.testing.assert_allclose(mesh.vertices, V[used], rtol=1e-8, atol=1e-6)
         np.testing.assert_array_equal(used[mesh.triangles], tri)
     P.close()
+
+
+def test_problem_with_the_reference_constraint_penalty():
+    """Problem.add_constraints (the reference's ConstraintHandler with the runner's omega):
+    the objective's regularization part and its gradient grow by exactly the term's own
+    value and gradient (Model.constraint_penalty)."""
+    from lithoinv3d_amd import litho_io
+
+    M, st, gz, tm, p0, m = setup(34)
+    k = len(st)
+    lo, hi = m.vertices.min(0), m.vertices.max(0)
+    xy = np.array([[0.5 * (lo[0] + hi[0]) + 3.3, 0.5 * (lo[1] + hi[1]) - 2.1]] * 3)
+    depth = -np.array([hi[2] - 10.0, hi[2] - 40.0, hi[2] - 70.0])
+    c = litho_io.Constraints(xy, depth - 5.0, depth + 5.0, np.array([0, 1, 2]))
+    A = inversion.Problem(M, st, gz, np.full(k, 0.05), tm, np.full(k, 1.0))
+    B = inversion.Problem(M, st, gz, np.full(k, 0.05), tm, np.full(k, 1.0))
+    say("problem with the reference constraint penalty")
+    B.add_constraints(c, weight=2.0)
+    p = p0 + 0.2
+    val, viol, g = M.constraint_penalty(p, c, weight=2.0, gradient=True)
+    ca, cb = A.evaluate(p), B.evaluate(p)
+    print("violations", viol, "penalty", val)
+    np.testing.assert_allclose(cb[:2], ca[:2], rtol=1e-14)
+    assert cb[2] == pytest.approx(ca[2] + val, rel=1e-14, abs=1e-12)
+    np.testing.assert_allclose(B.gradient(p), A.gradient(p) + g, rtol=1e-12, atol=1e-12)
+    A.close()
+    B.close()
