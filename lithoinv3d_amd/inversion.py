@@ -747,7 +747,9 @@ class InversionConfig(_ini_config_class()):
     def problem_kwargs(self):
         """Problem(...) arguments: the inducing field, alpha_mag, the smoothness (lambda)
         and reference-model (lambda_ref when enabled) weights, the default data
-        uncertainties, and omega as the depth-to-contact term's weight."""
+        uncertainties, and omega as the depth-to-contact term's weight (for the
+        reference's own discrete penalty instead: problem.add_constraints(c,
+        weight=config.omega))."""
         return dict(inc=self.inclination, dec=self.declination, field=self.field_nT,
                     alpha_mag=self.magnetic_weight, smooth_weight=self.smoothness_weight,
                     ref_weight=self.lambda_ref if self.enable_reference_model else 0.0,
